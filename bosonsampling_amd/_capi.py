@@ -1,0 +1,73 @@
+"""ctypes binding of libbsmpo.so (the C ABI declared in include/bsmpo.h).
+
+The library is built in-tree by ``bosonsampling_amd.build``; importing this
+module never falls back to anything else: if the library is missing,
+``lib()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbsmpo.so")
+
+C128, C64, BF16X3 = 0, 1, 2
+
+_lib = None
+
+# (name, restype, argtypes) for every entry point of include/bsmpo.h
+_vp, _i, _sz, _i32p, _i64p, _dp = C.c_void_p, C.c_int, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p
+SIGNATURES = {
+    "bsmpo_last_error": (C.c_char_p, []),
+    "bsmpo_abi_version": (C.c_int, []),
+    "bsmpo_bs_table": (C.c_int, [_i, _i, _dp, _dp, _vp]),
+    "bsmpo_plan_query": (C.c_int, [_i, _i, _i32p, _i32p, _i32p, _i, _i, C.POINTER(C.c_size_t),
+                                   C.POINTER(C.c_size_t)]),
+    "bsmpo_plan_build": (C.c_int, [_i, _i, _i32p, _i32p, _i32p, _i, _i, _vp, _sz]),
+    "bsmpo_plan_info_get": (C.c_int, [_vp, _vp]),
+    "bsmpo_plan_sectors": (C.c_int, [_vp, _i32p, _i64p, _i32p, _i32p]),
+    "bsmpo_site_update": (C.c_int, [_i, _i, _vp, _dp, _dp, _dp, _dp, _i32p, _dp, _i32p, _i, _i,
+                                    _vp, _sz, _vp]),
+    "bsmpo_contract": (C.c_int, [_i, _i, _vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i, _vp,
+                                 _sz, _vp]),
+    "bsmpo_observables": (C.c_int, [_i, _i, _i, _dp, _dp, _i32p, _dp, _dp, _dp, _dp, _vp]),
+}
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [("n_sectors", C.c_int), ("n_items", C.c_int), ("c_elems", C.c_int64),
+                ("g_elems", C.c_int64), ("sv_total", C.c_int64), ("flops_alg", C.c_double),
+                ("flops_tau", C.c_double), ("bytes_alg16", C.c_double)]
+
+
+class BsmpoError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BsmpoError(f"{LIB_PATH} not built: run `python -m bosonsampling_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = lib().bsmpo_last_error().decode(errors="replace")
+        raise BsmpoError(f"{what} failed ({rc}): {msg}")
+
+
+def ptr(t) -> int:
+    """Raw address of a torch tensor / numpy array (no copy)."""
+    if hasattr(t, "data_ptr"):
+        return t.data_ptr()
+    return t.ctypes.data

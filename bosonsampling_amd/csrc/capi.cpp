@@ -1,0 +1,78 @@
+// extern "C" entry points of libbsmpo (declared in include/bsmpo.h).
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+
+namespace bsmpo {
+int launch_observables(int M, int d, int chi, const double *Gamma, const double *Lambda,
+                       const int32_t *charge, double *R, double *L, double *marg, double *trace,
+                       hipStream_t stream);
+}
+
+using namespace bsmpo;
+
+namespace {
+
+int upload_plan(const void *h_plan, void *d_work, size_t work_bytes, hipStream_t stream) {
+  const PlanHeader *h = header(h_plan);
+  if (!d_work) return fail(-1, "workspace is NULL");
+  if ((int64_t)work_bytes < h->w_total) return fail(-1, "workspace too small for this plan");
+  if (h->dev_bytes > 0)
+    BSMPO_HIP(hipMemcpyAsync(d_work, reinterpret_cast<const char *>(h_plan) + h->dev_begin,
+                             (size_t)h->dev_bytes, hipMemcpyHostToDevice, stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bsmpo_bs_table(int d, int nb, const double *d_B, double *d_U3, void *stream) {
+  if (nb > 0 && (!d_B || !d_U3)) return fail(-1, "bs_table: NULL pointer");
+  return launch_bs_table(d, nb, d_B, d_U3, (hipStream_t)stream);
+}
+
+int bsmpo_contract(int d, int chi, const void *h_plan, const double *d_Glc, const double *d_Gcr,
+                   const double *d_LR, const double *d_U3, double *d_C, double *d_T,
+                   const double *d_LL, const double *d_LC, int precision, void *d_work,
+                   size_t work_bytes, void *stream) {
+  int rc = validate_plan(h_plan, d, chi);
+  if (rc) return rc;
+  if (!d_Glc || !d_Gcr || !d_LR || !d_U3 || !d_C || !d_T) return fail(-1, "contract: NULL pointer");
+  hipStream_t st = (hipStream_t)stream;
+  if ((rc = upload_plan(h_plan, d_work, work_bytes, st))) return rc;
+  return launch_contract(*header(h_plan), d_work, d_Glc, d_Gcr, d_LR, d_U3, d_C, d_T, d_LL, d_LC,
+                         precision, st);
+}
+
+int bsmpo_site_update(int d, int chi, const void *h_plan, double *d_Glc, double *d_Gcr,
+                      const double *d_LR, const double *d_U3, int32_t *d_charge_c,
+                      double *d_lambda_c, int32_t *d_num_lambda, int edge, int precision,
+                      void *d_work, size_t work_bytes, void *stream) {
+  int rc = validate_plan(h_plan, d, chi);
+  if (rc) return rc;
+  if (!d_Glc || !d_Gcr || !d_LR || !d_U3 || !d_charge_c || !d_lambda_c || !d_num_lambda)
+    return fail(-1, "site_update: NULL pointer");
+  const PlanHeader &h = *header(h_plan);
+  hipStream_t st = (hipStream_t)stream;
+  if ((rc = upload_plan(h_plan, d_work, work_bytes, st))) return rc;
+  char *w = reinterpret_cast<char *>(d_work);
+  if ((rc = launch_contract(h, d_work, d_Glc, d_Gcr, d_LR, d_U3,
+                            reinterpret_cast<double *>(w + h.w_C),
+                            reinterpret_cast<double *>(w + h.w_T), nullptr, nullptr, precision,
+                            st)))
+    return rc;
+  if ((rc = launch_svd(h, d_work, st))) return rc;
+  return launch_truncate(h, d_work, d_Glc, d_Gcr, d_charge_c, d_lambda_c, d_num_lambda, edge, st);
+}
+
+int bsmpo_observables(int M, int d, int chi, const double *d_Gamma, const double *d_Lambda,
+                      const int32_t *d_charge, double *d_R, double *d_L, double *d_marg,
+                      double *d_trace, void *stream) {
+  if (!d_Gamma || !d_Lambda || !d_charge || !d_R || !d_L || !d_marg || !d_trace)
+    return fail(-1, "observables: NULL pointer");
+  return launch_observables(M, d, chi, d_Gamma, d_Lambda, d_charge, d_R, d_L, d_marg, d_trace,
+                            (hipStream_t)stream);
+}
+
+}  // extern "C"

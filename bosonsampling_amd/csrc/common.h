@@ -1,0 +1,87 @@
+// Device-side helpers shared by the kernels: complex arithmetic and the
+// complex 16x16x4 FP64 MFMA tile step (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+
+namespace bsmpo {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+struct cplx {
+  double x, y;
+};
+
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+  return cplx{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__device__ __forceinline__ cplx cmulc(cplx a, cplx b) {  // a * conj(b)
+  return cplx{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+__device__ __forceinline__ cplx cconj(cplx a) { return cplx{a.x, -a.y}; }
+__device__ __forceinline__ cplx cload(const double *p, int64_t i) {
+  const double2 v = reinterpret_cast<const double2 *>(p)[i];
+  return cplx{v.x, v.y};
+}
+__device__ __forceinline__ void cstore(double *p, int64_t i, cplx v) {
+  reinterpret_cast<double2 *>(p)[i] = make_double2(v.x, v.y);
+}
+
+// One K=4 step of a complex 16x16 tile on FP64 MFMA.
+// Lane layout of v_mfma_f64_16x16x4f64 (64 lanes):
+//   A[i][k] from lane i + 16 k,  B[k][j] from lane j + 16 k,
+//   D[4 (l / 16) + r][l % 16] in element r of lane l's accumulator.
+__device__ __forceinline__ void cmfma_f64(dbl4 &accr, dbl4 &acci, cplx a, cplx b) {
+  accr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, accr, 0, 0, 0);
+  accr = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, accr, 0, 0, 0);
+  acci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, acci, 0, 0, 0);
+  acci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, acci, 0, 0, 0);
+}
+
+// Plan arrays as seen by kernels (device copy at the front of the workspace).
+struct DevPlan {
+  const Sector *sectors;
+  const int32_t *rows;
+  const int32_t *cols;
+  const Block *lblk;
+  const Block *cblk;
+  const Block *rblk;
+  const Item *items;
+  const Item *gram_items;
+  const Item *w_items;
+  const Item *v_items;
+  const int32_t *charges;  // CL, CC, CR (3 x chi x 2)
+  const int32_t *sv_sector;
+  const int32_t *jac_order;
+};
+
+__host__ __device__ inline DevPlan dev_plan(const PlanHeader &h, const void *work) {
+  const char *b = reinterpret_cast<const char *>(work);
+  DevPlan p;
+  p.sectors = reinterpret_cast<const Sector *>(b + h.o_sectors);
+  p.rows = reinterpret_cast<const int32_t *>(b + h.o_rows);
+  p.cols = reinterpret_cast<const int32_t *>(b + h.o_cols);
+  p.lblk = reinterpret_cast<const Block *>(b + h.o_lblk);
+  p.cblk = reinterpret_cast<const Block *>(b + h.o_cblk);
+  p.rblk = reinterpret_cast<const Block *>(b + h.o_rblk);
+  p.items = reinterpret_cast<const Item *>(b + h.o_items);
+  p.gram_items = reinterpret_cast<const Item *>(b + h.o_gram_items);
+  p.w_items = reinterpret_cast<const Item *>(b + h.o_w_items);
+  p.v_items = reinterpret_cast<const Item *>(b + h.o_v_items);
+  p.charges = reinterpret_cast<const int32_t *>(b + h.o_charges);
+  p.sv_sector = reinterpret_cast<const int32_t *>(b + h.o_sv_sector);
+  p.jac_order = reinterpret_cast<const int32_t *>(b + h.o_jac_order);
+  return p;
+}
+
+inline int grid_for(int64_t waves_needed) {
+  int64_t blocks = (waves_needed + 3) / 4;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 8192) blocks = 8192;
+  return (int)blocks;
+}
+
+}  // namespace bsmpo

@@ -1,0 +1,109 @@
+// Internal declarations shared by the bsmpo translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+
+#include "../../include/bsmpo.h"
+
+namespace bsmpo {
+
+// ------------------------------------------------------------- errors --
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+#define BSMPO_HIP(call)                                                          \
+  do {                                                                           \
+    hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess)                                                        \
+      return ::bsmpo::fail((int)e_, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// --------------------------------------------------------------- plan --
+// A plan is one contiguous host blob: PlanHeader, then the arrays below at
+// the byte offsets the header records.  The part from `dev_begin` to the end
+// is copied verbatim to the front of the device workspace, so every offset
+// used by kernels is relative to that copy.
+constexpr int32_t kPlanMagic = 0x4253504c;  // "BSPL"
+// Jacobi size classes: n <= 16, 32, 64, 96 run with the Gram matrix in LDS,
+// larger n with it in global memory.
+constexpr int kJacClasses = 5;
+constexpr int kJacLimit[kJacClasses] = {16, 32, 64, 96, 1 << 30};
+
+struct PlanHeader {
+  int32_t magic, d, chi, n_sectors;
+  int32_t n_items, n_lblk, n_cblk, n_rblk;
+  int32_t n_gram_items, n_w_items, n_v_items, pad0;
+  int64_t c_elems, g_elems, sv_total, v_elems, w_elems;
+  int64_t rows_total, cols_total;
+  double flops_alg, flops_tau, bytes_alg16;
+  // host byte offsets of the device-copied region and total blob size
+  int64_t dev_begin, blob_bytes;
+  // byte offsets RELATIVE TO dev_begin (i.e. inside the device copy)
+  int64_t o_sectors, o_rows, o_cols, o_lblk, o_cblk, o_rblk, o_items;
+  int64_t o_gram_items, o_w_items, o_v_items, o_charges, o_sv_sector, o_jac_order;
+  int64_t dev_bytes;  // size of the device-copied region
+  // sectors grouped by Jacobi size class: jac_order[jac_off[c] .. jac_off[c+1])
+  int32_t jac_off[kJacClasses + 1];
+  int32_t jac_max_n[kJacClasses];
+  // workspace layout (byte offsets from the start of d_work)
+  int64_t w_C, w_T, w_G, w_V, w_W, w_S, w_lam, w_flag, w_slot, w_total;
+};
+
+struct Sector {
+  int32_t t0, t1, L, R;
+  int32_t n;           // min(L, R)
+  int32_t row_off;     // into the rows pool
+  int32_t col_off;     // into the cols pool
+  int32_t left_small;  // 1 if L <= R (Gram = T T^H), else 0 (Gram = T^H T)
+  int64_t c_off;       // element offset of C / T (L x R row-major)
+  int64_t g_off;       // element offset of G (n x n column-major)
+  int64_t v_off;       // element offset of V (L x n column-major)
+  int64_t w_off;       // element offset of W (n x R row-major)
+  int64_t s_off;       // element offset of S / flags / slots (n)
+};
+
+struct Block {
+  int32_t c0, c1, start, end;
+};
+
+// contraction / GEMM work item: a 16-row x 16-col tile of a sector output
+struct Item {
+  int32_t sector;
+  int32_t blk;   // left-block index (contraction) or unused
+  int32_t r0;    // first row of the tile (sector-local)
+  int32_t nr;    // rows in the tile (<= 16)
+  int32_t c0;    // first col of the tile (sector-local)
+  int32_t nc;    // cols in the tile (<= 16)
+  int32_t pad0, pad1;
+};
+
+inline const PlanHeader *header(const void *plan) {
+  return reinterpret_cast<const PlanHeader *>(plan);
+}
+template <class T>
+inline const T *host_array(const void *plan, int64_t dev_off) {
+  const PlanHeader *h = header(plan);
+  return reinterpret_cast<const T *>(reinterpret_cast<const char *>(plan) + h->dev_begin + dev_off);
+}
+template <class T>
+inline T *dev_array(void *work, int64_t dev_off) {
+  return reinterpret_cast<T *>(reinterpret_cast<char *>(work) + dev_off);
+}
+
+int validate_plan(const void *plan, int d, int chi);
+
+// -------------------------------------------------------------- kernels --
+// launchers implemented in the .hip files; all enqueue on `stream`.
+int launch_bs_table(int d, int nb, const double *d_B, double *d_U3, hipStream_t stream);
+int launch_contract(const PlanHeader &h, void *d_work, const double *d_Glc,
+                    const double *d_Gcr, const double *d_LR, const double *d_U3, double *d_C,
+                    double *d_T, const double *d_LL, const double *d_LC, int precision,
+                    hipStream_t stream);
+int launch_svd(const PlanHeader &h, void *d_work, hipStream_t stream);
+int launch_truncate(const PlanHeader &h, void *d_work, double *d_Glc, double *d_Gcr,
+                    int32_t *d_charge_c, double *d_lambda_c, int32_t *d_num_lambda, int edge,
+                    hipStream_t stream);
+
+}  // namespace bsmpo

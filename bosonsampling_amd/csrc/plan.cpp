@@ -1,0 +1,356 @@
+// Host-side planning of one site update: charge blocks, tau sectors, packed
+// buffer offsets and the work lists of every stage.
+//
+// Follows the selection rules of the reference:
+//   tau loop bounds   src/beam_splitter_parallel/Rank.py:78-91
+//   sector windows    src/mpo_sort.py:424-432 (rows cl >= tau, cols cr <= tau)
+//   charge blocks     src/mpo_sort.py:215-259 (start index of each charge)
+// but keeps the sectors compact (no padding to multiples of 8, which the
+// reference needs for its FFMA thread tiles, src/mpo_sort.py:263-384): the
+// MFMA kernels mask ragged tiles instead.
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "internal.h"
+
+namespace bsmpo {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+namespace {
+
+struct Built {
+  PlanHeader h;
+  std::vector<Sector> sectors;
+  std::vector<int32_t> rows, cols;
+  std::vector<Block> lblk, cblk, rblk;
+  std::vector<Item> items, gram_items, w_items, v_items;
+  std::vector<int32_t> charges;  // CL, CC, CR: 3 x chi x 2
+  std::vector<int32_t> sv_sector;  // sector of every singular-value slot
+  std::vector<int32_t> jac_order;  // sector indices grouped by Jacobi class
+};
+
+int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+int make_blocks(int d, int chi, const int32_t *ch, std::vector<Block> &out, const char *name) {
+  out.clear();
+  int64_t prev = -1;
+  for (int i = 0; i < chi; ++i) {
+    int32_t c0 = ch[2 * i], c1 = ch[2 * i + 1];
+    if (c0 < 0 || c1 < 0 || c0 > d || c1 > d)
+      return fail(-1, std::string("charge out of range in ") + name);
+    int64_t key = (int64_t)c0 * (d + 1) + c1;
+    if (key < prev) return fail(-1, std::string("charges not lexicographically sorted in ") + name);
+    prev = key;
+    if (c0 >= d || c1 >= d) continue;
+    if (!out.empty() && out.back().c0 == c0 && out.back().c1 == c1 && out.back().end == i)
+      out.back().end = i + 1;
+    else
+      out.push_back(Block{c0, c1, i, i + 1});
+  }
+  return 0;
+}
+
+void add_tiles(std::vector<Item> &items, int sector, int blk, int r_begin, int r_end, int c_begin,
+               int c_end) {
+  for (int r = r_begin; r < r_end; r += 16)
+    for (int c = c_begin; c < c_end; c += 16)
+      items.push_back(Item{sector, blk, r, std::min(16, r_end - r), c, std::min(16, c_end - c), 0, 0});
+}
+
+int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *CR, int only_t0,
+          int only_t1, Built &b) {
+  if (d < 1 || d > 32) return fail(-1, "d must be in [1, 32]");
+  if (chi < 1) return fail(-1, "chi must be positive");
+  std::memset(&b.h, 0, sizeof(PlanHeader));
+  int rc;
+  if ((rc = make_blocks(d, chi, CL, b.lblk, "CL"))) return rc;
+  if ((rc = make_blocks(d, chi, CC, b.cblk, "CC"))) return rc;
+  if ((rc = make_blocks(d, chi, CR, b.rblk, "CR"))) return rc;
+
+  // tau bounds, Rank.py:78-91 (largest valid CL per component, smallest CR)
+  int lc0 = 0, lc1 = 0, sr0 = d, sr1 = d;
+  bool any0 = false, any1 = false;
+  for (int i = 0; i < chi; ++i) {
+    if (CL[2 * i] != d) { lc0 = any0 ? std::max(lc0, CL[2 * i]) : CL[2 * i]; any0 = true; }
+    if (CL[2 * i + 1] != d) { lc1 = any1 ? std::max(lc1, CL[2 * i + 1]) : CL[2 * i + 1]; any1 = true; }
+    sr0 = std::min(sr0, CR[2 * i]);
+    sr1 = std::min(sr1, CR[2 * i + 1]);
+  }
+
+  int64_t c_off = 0, g_off = 0, v_off = 0, w_off = 0, s_off = 0;
+  double f_tau = 0.0, theta_elems = 0.0;
+  for (int t0 = sr0; t0 <= lc0; ++t0) {
+    for (int t1 = sr1; t1 <= lc1; ++t1) {
+      if (only_t0 >= 0 && (t0 != only_t0 || t1 != only_t1)) continue;
+      Sector s{};
+      s.t0 = t0;
+      s.t1 = t1;
+      s.row_off = (int32_t)b.rows.size();
+      s.col_off = (int32_t)b.cols.size();
+      std::vector<std::pair<int, int>> lsel;  // (block index, sector-local row start)
+      int L = 0, R = 0;
+      for (int k = 0; k < (int)b.lblk.size(); ++k) {
+        const Block &bl = b.lblk[k];
+        if (bl.c0 >= t0 && bl.c1 >= t1) {
+          lsel.push_back({k, L});
+          for (int i = bl.start; i < bl.end; ++i) b.rows.push_back(i);
+          L += bl.end - bl.start;
+        }
+      }
+      std::vector<int> rsel;
+      for (int k = 0; k < (int)b.rblk.size(); ++k) {
+        const Block &br = b.rblk[k];
+        if (br.c0 <= t0 && br.c1 <= t1) {
+          rsel.push_back(k);
+          for (int i = br.start; i < br.end; ++i) b.cols.push_back(i);
+          R += br.end - br.start;
+        }
+      }
+      if (L == 0 || R == 0) {  // Rank.py:108-111: skipped sector
+        b.rows.resize(s.row_off);
+        b.cols.resize(s.col_off);
+        continue;
+      }
+      s.L = L;
+      s.R = R;
+      s.n = std::min(L, R);
+      s.left_small = L <= R ? 1 : 0;
+      s.c_off = c_off;
+      s.g_off = g_off;
+      s.v_off = v_off;
+      s.w_off = w_off;
+      s.s_off = s_off;
+      c_off += (int64_t)L * R;
+      g_off += (int64_t)s.n * s.n;
+      v_off += (int64_t)L * s.n;
+      w_off += (int64_t)s.n * R;
+      s_off += s.n;
+      theta_elems += (double)L * R;
+      int sid = (int)b.sectors.size();
+      for (auto &p : lsel) {
+        const Block &bl = b.lblk[p.first];
+        add_tiles(b.items, sid, p.first, p.second, p.second + (bl.end - bl.start), 0, R);
+        // F_tau: sum over (cl, ch, cr) with cl >= ch >= cr inside this sector
+        for (int rk : rsel) {
+          const Block &br = b.rblk[rk];
+          for (const Block &bc : b.cblk)
+            if (bl.c0 >= bc.c0 && bc.c0 >= br.c0 && bl.c1 >= bc.c1 && bc.c1 >= br.c1)
+              f_tau += 8.0 * (bl.end - bl.start) * (double)(bc.end - bc.start) * (br.end - br.start);
+        }
+      }
+      add_tiles(b.gram_items, sid, 0, 0, s.n, 0, s.n);
+      add_tiles(b.w_items, sid, 0, 0, s.n, 0, R);
+      if (!s.left_small) add_tiles(b.v_items, sid, 0, 0, L, 0, s.n);
+      b.sectors.push_back(s);
+    }
+  }
+
+  // algorithmic counts (roofline accounting, complex MAC = 8 flops)
+  double f_alg = 0.0, lc = 0.0, cr = 0.0;
+  for (const Block &bl : b.lblk)
+    for (const Block &bc : b.cblk) {
+      if (!(bl.c0 >= bc.c0 && bl.c1 >= bc.c1)) continue;
+      lc += (double)(bl.end - bl.start) * (bc.end - bc.start);
+      for (const Block &br : b.rblk)
+        if (bc.c0 >= br.c0 && bc.c1 >= br.c1)
+          f_alg += 8.0 * (bl.end - bl.start) * (double)(bc.end - bc.start) * (br.end - br.start);
+    }
+  for (const Block &bc : b.cblk)
+    for (const Block &br : b.rblk)
+      if (bc.c0 >= br.c0 && bc.c1 >= br.c1) cr += (double)(bc.end - bc.start) * (br.end - br.start);
+
+  for (int sid = 0; sid < (int)b.sectors.size(); ++sid)
+    for (int i = 0; i < b.sectors[sid].n; ++i) b.sv_sector.push_back(sid);
+  for (int c = 0; c < kJacClasses; ++c) {
+    b.h.jac_off[c] = (int32_t)b.jac_order.size();
+    b.h.jac_max_n[c] = 0;
+    int lo = c == 0 ? 0 : kJacLimit[c - 1];
+    for (int sid = 0; sid < (int)b.sectors.size(); ++sid) {
+      int n = b.sectors[sid].n;
+      if (n > lo && n <= kJacLimit[c]) {
+        b.jac_order.push_back(sid);
+        b.h.jac_max_n[c] = std::max(b.h.jac_max_n[c], n);
+      }
+    }
+  }
+  b.h.jac_off[kJacClasses] = (int32_t)b.jac_order.size();
+
+  b.charges.resize((size_t)6 * chi);
+  std::memcpy(b.charges.data(), CL, sizeof(int32_t) * 2 * chi);
+  std::memcpy(b.charges.data() + 2 * chi, CC, sizeof(int32_t) * 2 * chi);
+  std::memcpy(b.charges.data() + 4 * chi, CR, sizeof(int32_t) * 2 * chi);
+
+  PlanHeader &h = b.h;
+  h.magic = kPlanMagic;
+  h.d = d;
+  h.chi = chi;
+  h.n_sectors = (int32_t)b.sectors.size();
+  h.n_items = (int32_t)b.items.size();
+  h.n_lblk = (int32_t)b.lblk.size();
+  h.n_cblk = (int32_t)b.cblk.size();
+  h.n_rblk = (int32_t)b.rblk.size();
+  h.n_gram_items = (int32_t)b.gram_items.size();
+  h.n_w_items = (int32_t)b.w_items.size();
+  h.n_v_items = (int32_t)b.v_items.size();
+  h.c_elems = c_off;
+  h.g_elems = g_off;
+  h.sv_total = s_off;
+  h.v_elems = v_off;
+  h.w_elems = w_off;
+  h.rows_total = (int64_t)b.rows.size();
+  h.cols_total = (int64_t)b.cols.size();
+  h.flops_alg = f_alg;
+  h.flops_tau = f_tau;
+  h.bytes_alg16 = 16.0 * (lc + cr + theta_elems);
+
+  h.dev_begin = align_up(sizeof(PlanHeader), 256);
+  int64_t o = 0;
+  auto place = [&](int64_t bytes) {
+    int64_t at = o;
+    o = align_up(o + bytes, 256);
+    return at;
+  };
+  h.o_sectors = place(sizeof(Sector) * b.sectors.size());
+  h.o_rows = place(sizeof(int32_t) * b.rows.size());
+  h.o_cols = place(sizeof(int32_t) * b.cols.size());
+  h.o_lblk = place(sizeof(Block) * b.lblk.size());
+  h.o_cblk = place(sizeof(Block) * b.cblk.size());
+  h.o_rblk = place(sizeof(Block) * b.rblk.size());
+  h.o_items = place(sizeof(Item) * b.items.size());
+  h.o_gram_items = place(sizeof(Item) * b.gram_items.size());
+  h.o_w_items = place(sizeof(Item) * b.w_items.size());
+  h.o_v_items = place(sizeof(Item) * b.v_items.size());
+  h.o_charges = place(sizeof(int32_t) * b.charges.size());
+  h.o_sv_sector = place(sizeof(int32_t) * b.sv_sector.size());
+  h.o_jac_order = place(sizeof(int32_t) * b.jac_order.size());
+  h.dev_bytes = o;
+  h.blob_bytes = h.dev_begin + h.dev_bytes;
+
+  int64_t w = align_up(h.dev_bytes, 256);
+  auto wplace = [&](int64_t bytes) {
+    int64_t at = w;
+    w = align_up(w + std::max<int64_t>(bytes, 1), 256);
+    return at;
+  };
+  h.w_C = wplace(16 * h.c_elems);
+  h.w_T = wplace(16 * h.c_elems);
+  h.w_G = wplace(16 * h.g_elems);
+  h.w_V = wplace(16 * h.v_elems);
+  h.w_W = wplace(16 * h.w_elems);
+  h.w_S = wplace(8 * h.sv_total);
+  h.w_lam = wplace(8 * h.sv_total);
+  h.w_flag = wplace(4 * h.sv_total);
+  h.w_slot = wplace(4 * h.sv_total);
+  h.w_total = w;
+  return 0;
+}
+
+template <class T>
+void put(char *base, int64_t off, const std::vector<T> &v) {
+  if (!v.empty()) std::memcpy(base + off, v.data(), sizeof(T) * v.size());
+}
+
+}  // namespace
+
+int validate_plan(const void *plan, int d, int chi) {
+  if (!plan) return fail(-1, "plan is NULL");
+  const PlanHeader *h = header(plan);
+  if (h->magic != kPlanMagic) return fail(-1, "not a bsmpo plan");
+  if (h->d != d || h->chi != chi) return fail(-1, "plan was built for another d / chi");
+  return 0;
+}
+
+}  // namespace bsmpo
+
+using namespace bsmpo;
+
+extern "C" {
+
+const char *bsmpo_last_error(void) { return bsmpo::g_last_error.c_str(); }
+
+int bsmpo_abi_version(void) { return 1; }
+
+int bsmpo_plan_query(int d, int chi, const int32_t *h_CL, const int32_t *h_CC,
+                     const int32_t *h_CR, int tau0, int tau1, size_t *plan_bytes,
+                     size_t *work_bytes) {
+  if (!h_CL || !h_CC || !h_CR || !plan_bytes || !work_bytes) return fail(-1, "NULL argument");
+  Built b;
+  int rc = build(d, chi, h_CL, h_CC, h_CR, tau0, tau1, b);
+  if (rc) return rc;
+  *plan_bytes = (size_t)b.h.blob_bytes;
+  *work_bytes = (size_t)b.h.w_total;
+  return 0;
+}
+
+int bsmpo_plan_build(int d, int chi, const int32_t *h_CL, const int32_t *h_CC,
+                     const int32_t *h_CR, int tau0, int tau1, void *h_plan, size_t plan_bytes) {
+  if (!h_CL || !h_CC || !h_CR || !h_plan) return fail(-1, "NULL argument");
+  Built b;
+  int rc = build(d, chi, h_CL, h_CC, h_CR, tau0, tau1, b);
+  if (rc) return rc;
+  if ((int64_t)plan_bytes < b.h.blob_bytes) return fail(-1, "plan buffer too small");
+  char *base = reinterpret_cast<char *>(h_plan);
+  std::memset(base, 0, (size_t)b.h.blob_bytes);
+  std::memcpy(base, &b.h, sizeof(PlanHeader));
+  char *dev = base + b.h.dev_begin;
+  put(dev, b.h.o_sectors, b.sectors);
+  put(dev, b.h.o_rows, b.rows);
+  put(dev, b.h.o_cols, b.cols);
+  put(dev, b.h.o_lblk, b.lblk);
+  put(dev, b.h.o_cblk, b.cblk);
+  put(dev, b.h.o_rblk, b.rblk);
+  put(dev, b.h.o_items, b.items);
+  put(dev, b.h.o_gram_items, b.gram_items);
+  put(dev, b.h.o_w_items, b.w_items);
+  put(dev, b.h.o_v_items, b.v_items);
+  put(dev, b.h.o_charges, b.charges);
+  put(dev, b.h.o_sv_sector, b.sv_sector);
+  put(dev, b.h.o_jac_order, b.jac_order);
+  return 0;
+}
+
+int bsmpo_plan_info_get(const void *h_plan, bsmpo_plan_info *info) {
+  if (!h_plan || !info) return fail(-1, "NULL argument");
+  const PlanHeader *h = header(h_plan);
+  if (h->magic != kPlanMagic) return fail(-1, "not a bsmpo plan");
+  info->n_sectors = h->n_sectors;
+  info->n_items = h->n_items;
+  info->c_elems = h->c_elems;
+  info->g_elems = h->g_elems;
+  info->sv_total = h->sv_total;
+  info->flops_alg = h->flops_alg;
+  info->flops_tau = h->flops_tau;
+  info->bytes_alg16 = h->bytes_alg16;
+  return 0;
+}
+
+int bsmpo_plan_sectors(const void *h_plan, int32_t *h_out, int64_t *h_coff, int32_t *h_rows,
+                       int32_t *h_cols) {
+  if (!h_plan) return fail(-1, "NULL argument");
+  const PlanHeader *h = header(h_plan);
+  if (h->magic != kPlanMagic) return fail(-1, "not a bsmpo plan");
+  const Sector *s = host_array<Sector>(h_plan, h->o_sectors);
+  for (int i = 0; i < h->n_sectors; ++i) {
+    if (h_out) {
+      h_out[4 * i] = s[i].t0;
+      h_out[4 * i + 1] = s[i].t1;
+      h_out[4 * i + 2] = s[i].L;
+      h_out[4 * i + 3] = s[i].R;
+    }
+    if (h_coff) h_coff[i] = s[i].c_off;
+  }
+  if (h_rows) std::memcpy(h_rows, host_array<int32_t>(h_plan, h->o_rows), 4 * h->rows_total);
+  if (h_cols) std::memcpy(h_cols, host_array<int32_t>(h_plan, h->o_cols), 4 * h->cols_total);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,662 @@
+// gfx950 kernels of the beam-splitter site update (complex128 path).
+//
+//   k_bs_table     Fock BS elements, FP64            (src/u_kernel.cu:37-70)
+//   k_contract     grouped sector contraction on FP64 MFMA, LR epilogue
+//                  (src/update_MPO_kernel.cu:193-734, Rank.py:114-118)
+//   k_gram         G = T T^H or T^H T per sector on FP64 MFMA
+//   k_jacobi       one-sided Jacobi on G (LDS-resident when it fits)
+//   k_vectors      V = T Y diag(1/sigma) for sectors with L > R
+//   k_wmat         W = V^H C                         (Rank.py:123-124)
+//   k_topk         global top-chi + slot assignment  (Rank.py:139-145,182-196)
+//   k_zero/k_scatter  write Gamma0Out / Gamma1Out    (Rank.py:147-179, Full.py:130-135)
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.h"
+
+namespace bsmpo {
+
+// ============================================================ BS table ==
+struct BsArgs {
+  double sqf[33];  // sqrt(n!) for n = 0..32
+};
+
+__device__ double binom_u64(int n, int k) {
+  if (k < 0 || k > n) return 0.0;
+  if (k > n - k) k = n - k;
+  unsigned long long r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (unsigned long long)(n - k + i) / (unsigned long long)i;
+  return (double)r;
+}
+
+__device__ cplx cpow_int(cplx a, int e) {
+  cplx r{1.0, 0.0};
+  for (int i = 0; i < e; ++i) r = cmul(r, a);
+  return r;
+}
+
+// One block per beam splitter; U3[n1][n2][p] = <n1, n2|U(B)|p, q>, q = n1+n2-p:
+//   sqrt(n1! n2! / (p! q!)) sum_k C(p,k) C(q,n1-k) B00^k B10^(p-k) B01^(n1-k) B11^(q-n1+k)
+__global__ void k_bs_table(int d, const double *B, double *U3, BsArgs args) {
+  const int b = blockIdx.x;
+  const double *Bb = B + 8 * b;
+  const cplx b00{Bb[0], Bb[1]}, b01{Bb[2], Bb[3]}, b10{Bb[4], Bb[5]}, b11{Bb[6], Bb[7]};
+  double *U = U3 + (int64_t)b * d * d * d * 2;
+  for (int e = threadIdx.x; e < d * d * d; e += blockDim.x) {
+    const int n1 = e / (d * d), n2 = (e / d) % d, p = e % d;
+    const int q = n1 + n2 - p;
+    cplx acc{0.0, 0.0};
+    if (q >= 0 && q < d) {
+      const int klo = max(0, n1 - q), khi = min(p, n1);
+      for (int k = klo; k <= khi; ++k) {
+        const double c = binom_u64(p, k) * binom_u64(q, n1 - k);
+        cplx t = cmul(cmul(cpow_int(b00, k), cpow_int(b10, p - k)),
+                      cmul(cpow_int(b01, n1 - k), cpow_int(b11, q - n1 + k)));
+        acc.x += c * t.x;
+        acc.y += c * t.y;
+      }
+      const double f = (args.sqf[n1] * args.sqf[n2]) / (args.sqf[p] * args.sqf[q]);
+      acc.x *= f;
+      acc.y *= f;
+    }
+    cstore(U, e, acc);
+  }
+}
+
+int launch_bs_table(int d, int nb, const double *d_B, double *d_U3, hipStream_t stream) {
+  if (d < 1 || d > 32 || nb < 0) return fail(-1, "bs_table: bad d or nb");
+  if (nb == 0) return 0;
+  BsArgs a;
+  long double f = 1.0L;
+  for (int n = 0; n <= 32; ++n) {
+    if (n > 0) f *= (long double)n;
+    a.sqf[n] = (double)sqrtl(f);
+  }
+  hipLaunchKernelGGL(k_bs_table, dim3(nb), dim3(256), 0, stream, d, d_B, d_U3, a);
+  BSMPO_HIP(hipGetLastError());
+  return 0;
+}
+
+// ========================================================= contraction ==
+// One wave per work item = a 16 x 16 tile of C_tau whose rows lie in one
+// left charge block cl.  The BS weight u(cl, ch, cr; tau) depends on the
+// column's cr and on the centre block ch only, so it is folded into the B
+// operand as it is loaded:  B'[c][j] = Gcr[c][col_j] * u(cl, ch(c), cr_j).
+__global__ __launch_bounds__(256) void k_contract_c128(
+    PlanHeader h, const char *work, const double *__restrict__ Glc,
+    const double *__restrict__ Gcr, const double *__restrict__ LR,
+    const double *__restrict__ U3, double *__restrict__ C, double *__restrict__ T,
+    const double *__restrict__ LL, const double *__restrict__ LC) {
+  const DevPlan P = dev_plan(h, work);
+  const int d = h.d, chi = h.chi;
+  const int lane = threadIdx.x & 63;
+  const int i16 = lane & 15, kq = lane >> 4;
+  const int32_t *CR = P.charges + 4 * chi;
+  const int nwaves = gridDim.x * 4;
+  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < h.n_items; it += nwaves) {
+    const Item item = P.items[it];
+    const Sector s = P.sectors[item.sector];
+    const Block bl = P.lblk[item.blk];
+    const bool rv = i16 < item.nr;
+    const bool cv = i16 < item.nc;
+    const int grow = rv ? P.rows[s.row_off + item.r0 + i16] : 0;
+    const int gcol = cv ? P.cols[s.col_off + item.c0 + i16] : 0;
+    const int cr0 = cv ? CR[2 * gcol] : (1 << 20);
+    const int cr1 = cv ? CR[2 * gcol + 1] : (1 << 20);
+    // smallest column charge in the tile (columns are sorted: lane 0 has it)
+    const int mincr0 = __shfl(cr0, 0), mincr1_all = cr1;
+    int mincr1 = mincr1_all;
+    for (int o = 8; o >= 1; o >>= 1) mincr1 = min(mincr1, __shfl_xor(mincr1, o));
+    const double2 *G1 = reinterpret_cast<const double2 *>(Glc);
+    const double2 *G2 = reinterpret_cast<const double2 *>(Gcr);
+    const double2 *U = reinterpret_cast<const double2 *>(U3);
+    dbl4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
+    for (int cb = 0; cb < h.n_cblk; ++cb) {
+      const Block bc = P.cblk[cb];
+      if (bc.c0 > bl.c0 || bc.c1 > bl.c1) continue;
+      if (bc.c0 < mincr0 || bc.c1 < mincr1) continue;
+      cplx u{0.0, 0.0};
+      if (cv && bc.c0 >= cr0 && bc.c1 >= cr1) {
+        const double2 a = U[((bl.c0 - s.t0) * d + (s.t0 - cr0)) * d + (bl.c0 - bc.c0)];
+        const double2 b = U[((bl.c1 - s.t1) * d + (s.t1 - cr1)) * d + (bl.c1 - bc.c1)];
+        u = cmulc(cplx{a.x, a.y}, cplx{b.x, b.y});
+      }
+      for (int k0 = bc.start; k0 < bc.end; k0 += 4) {
+        const int k = k0 + kq;
+        const bool kv = k < bc.end;
+        cplx a{0.0, 0.0}, g{0.0, 0.0};
+        if (kv && rv) {
+          const double2 v = G1[(int64_t)grow * chi + k];
+          a = cplx{v.x, v.y};
+        }
+        if (kv && cv) {
+          const double2 v = G2[(int64_t)k * chi + gcol];
+          const double lc = LC ? LC[k] : 1.0;
+          g = cplx{v.x * lc, v.y * lc};
+        }
+        cmfma_f64(accr, acci, a, cmul(g, u));
+      }
+    }
+    if (cv) {
+      const double lr = LR[gcol];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 4 * kq + r;
+        if (row < item.nr) {
+          const int64_t idx = s.c_off + (int64_t)(item.r0 + row) * s.R + item.c0 + i16;
+          const double w = LL ? lr * LL[P.rows[s.row_off + item.r0 + row]] : lr;
+          cstore(C, idx, cplx{accr[r], acci[r]});
+          cstore(T, idx, cplx{accr[r] * w, acci[r] * w});
+        }
+      }
+    }
+  }
+}
+
+int launch_contract(const PlanHeader &h, void *d_work, const double *d_Glc, const double *d_Gcr,
+                    const double *d_LR, const double *d_U3, double *d_C, double *d_T,
+                    const double *d_LL, const double *d_LC, int precision, hipStream_t stream) {
+  if (precision != BSMPO_C128) return fail(-1, "contract: only BSMPO_C128 in this build");
+  if (h.n_items == 0) return 0;
+  hipLaunchKernelGGL(k_contract_c128, dim3(grid_for(h.n_items)), dim3(256), 0, stream, h,
+                     (const char *)d_work, d_Glc, d_Gcr, d_LR, d_U3, d_C, d_T, d_LL, d_LC);
+  BSMPO_HIP(hipGetLastError());
+  return 0;
+}
+
+// ================================================================ Gram ==
+// G (n x n, column-major) = T T^H if L <= R else T^H T, T row-major L x R.
+__global__ __launch_bounds__(256) void k_gram(PlanHeader h, const char *work,
+                                              const double *__restrict__ T,
+                                              double *__restrict__ G) {
+  const DevPlan P = dev_plan(h, work);
+  const int lane = threadIdx.x & 63;
+  const int i16 = lane & 15, kq = lane >> 4;
+  const int nwaves = gridDim.x * 4;
+  const double2 *Tc = reinterpret_cast<const double2 *>(T);
+  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < h.n_gram_items; it += nwaves) {
+    const Item item = P.gram_items[it];
+    const Sector s = P.sectors[item.sector];
+    const double2 *Ts = Tc + s.c_off;
+    const int p = item.r0 + i16, q = item.c0 + i16;
+    const bool pv = i16 < item.nr, qv = i16 < item.nc;
+    dbl4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
+    if (s.left_small) {
+      for (int k0 = 0; k0 < s.R; k0 += 4) {
+        const int k = k0 + kq;
+        cplx a{0.0, 0.0}, b{0.0, 0.0};
+        if (k < s.R) {
+          if (pv) { const double2 v = Ts[(int64_t)p * s.R + k]; a = cplx{v.x, v.y}; }
+          if (qv) { const double2 v = Ts[(int64_t)q * s.R + k]; b = cplx{v.x, -v.y}; }
+        }
+        cmfma_f64(accr, acci, a, b);
+      }
+    } else {
+      for (int k0 = 0; k0 < s.L; k0 += 4) {
+        const int k = k0 + kq;
+        cplx a{0.0, 0.0}, b{0.0, 0.0};
+        if (k < s.L) {
+          if (pv) { const double2 v = Ts[(int64_t)k * s.R + p]; a = cplx{v.x, -v.y}; }
+          if (qv) { const double2 v = Ts[(int64_t)k * s.R + q]; b = cplx{v.x, v.y}; }
+        }
+        cmfma_f64(accr, acci, a, b);
+      }
+    }
+    if (qv) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 4 * kq + r;
+        if (row < item.nr)
+          cstore(G, s.g_off + (int64_t)q * s.n + item.r0 + row, cplx{accr[r], acci[r]});
+      }
+    }
+  }
+}
+
+// ============================================================== Jacobi ==
+// One-sided (Hestenes) Jacobi on the Hermitian PSD Gram matrix G = X:
+// rotate column pairs until all are orthogonal.  Then X = Y diag(lambda)
+// with Y the eigenvectors of G and lambda = sigma^2 its eigenvalues.
+// Column pairs follow the round-robin tournament; 8 lanes per pair.
+constexpr int kJacSweeps = 40;
+constexpr double kJacTol = 1e-15;
+// singular values with sigma <= kDropRel * sigma_max(sector) are the Gram
+// route's noise floor and are dropped (sigma := 0, vectors 0).
+constexpr double kDropRel2 = 1e-12;  // on lambda = sigma^2
+
+__device__ __forceinline__ void rr_pair(int step, int i, int n2, int &p, int &q) {
+  if (i == 0) {
+    p = n2 - 1;
+    q = step;
+  } else {
+    const int m = n2 - 1;
+    p = (step + i) % m;
+    q = (step - i + m) % m;
+  }
+  if (p > q) { const int t = p; p = q; q = t; }
+}
+
+template <bool kLds>
+__global__ __launch_bounds__(256) void k_jacobi(PlanHeader h, const char *work, int cls_begin,
+                                                double *__restrict__ Gbuf,
+                                                double *__restrict__ Vbuf,
+                                                double *__restrict__ Wbuf,
+                                                double *__restrict__ Sbuf,
+                                                int32_t *__restrict__ flags,
+                                                double *__restrict__ lam,
+                                                int32_t *__restrict__ rank) {
+  extern __shared__ double2 smem[];
+  const DevPlan P = dev_plan(h, work);
+  const int sid = P.jac_order[cls_begin + blockIdx.x];
+  const Sector s = P.sectors[sid];
+  const int n = s.n;
+  double2 *X = kLds ? smem : reinterpret_cast<double2 *>(Gbuf) + s.g_off;
+  __shared__ int any_rot;
+  __shared__ double lam_max;
+  if (n == 0) return;
+  if (kLds) {
+    const double2 *Gs = reinterpret_cast<const double2 *>(Gbuf) + s.g_off;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) X[e] = Gs[e];
+    __syncthreads();
+  }
+  const int group = threadIdx.x >> 3, gl = threadIdx.x & 7;
+  const int ngroups = blockDim.x >> 3;
+  const int n2 = n + (n & 1);
+  for (int sweep = 0; sweep < kJacSweeps && n > 1; ++sweep) {
+    if (threadIdx.x == 0) any_rot = 0;
+    __syncthreads();
+    for (int step = 0; step < n2 - 1; ++step) {
+      for (int pi = group; pi < n2 / 2; pi += ngroups) {
+        int p, q;
+        rr_pair(step, pi, n2, p, q);
+        if (q >= n) continue;
+        double2 *xp = X + (int64_t)p * n, *xq = X + (int64_t)q * n;
+        double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+        for (int i = gl; i < n; i += 8) {
+          const double2 a = xp[i], b = xq[i];
+          al += a.x * a.x + a.y * a.y;
+          be += b.x * b.x + b.y * b.y;
+          gr += a.x * b.x + a.y * b.y;  // conj(a) b
+          gi += a.x * b.y - a.y * b.x;
+        }
+        for (int o = 4; o >= 1; o >>= 1) {
+          al += __shfl_xor(al, o);
+          be += __shfl_xor(be, o);
+          gr += __shfl_xor(gr, o);
+          gi += __shfl_xor(gi, o);
+        }
+        const double ga = sqrt(gr * gr + gi * gi);
+        if (ga > kJacTol * sqrt(al * be) && ga > 0.0) {
+          const double zeta = (be - al) / (2.0 * ga);
+          const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+          const double er = gr / ga, ei = gi / ga;  // e^{i phi}
+          for (int i = gl; i < n; i += 8) {
+            const double2 a = xp[i], b = xq[i];
+            // a' = c a - s conj(e) b ; b' = s e a + c b
+            double2 na, nb;
+            na.x = c * a.x - sn * (er * b.x + ei * b.y);
+            na.y = c * a.y - sn * (er * b.y - ei * b.x);
+            nb.x = c * b.x + sn * (er * a.x - ei * a.y);
+            nb.y = c * b.y + sn * (er * a.y + ei * a.x);
+            xp[i] = na;
+            xq[i] = nb;
+          }
+          if (gl == 0) any_rot = 1;
+        }
+      }
+      __syncthreads();
+    }
+    if (!any_rot) break;
+    __syncthreads();
+  }
+  // eigenvalues = column norms (X = Y diag(lambda)); order them descending,
+  // ties by index, and write sigma = sqrt(lambda), flags and unit vectors to
+  // the sorted positions.  lam / rank scratch: one entry per sv slot.
+  for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    double nr = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double2 a = X[(int64_t)p * n + i];
+      nr += a.x * a.x + a.y * a.y;
+    }
+    lam[s.s_off + p] = sqrt(nr);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int p = 0; p < n; ++p) m = fmax(m, lam[s.s_off + p]);
+    lam_max = m;
+  }
+  for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    const double lp = lam[s.s_off + p];
+    int r = 0;
+    for (int q = 0; q < n; ++q) {
+      const double lq = lam[s.s_off + q];
+      r += (lq > lp || (lq == lp && q < p)) ? 1 : 0;
+    }
+    rank[s.s_off + p] = r;
+  }
+  __syncthreads();
+  double2 *Yout = s.left_small ? reinterpret_cast<double2 *>(Vbuf) + s.v_off
+                               : reinterpret_cast<double2 *>(Wbuf) + s.w_off;
+  const double thr = kDropRel2 * lam_max;
+  for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    const double lp = lam[s.s_off + p];
+    const bool keep = lam_max > 0.0 && lp > thr;
+    const int r = rank[s.s_off + p];
+    Sbuf[s.s_off + r] = keep ? sqrt(lp) : 0.0;
+    flags[s.s_off + r] = keep ? 1 : 0;
+  }
+  for (int p = 0; p < n; ++p) {
+    const double lp = lam[s.s_off + p];
+    const bool keep = lam_max > 0.0 && lp > thr;
+    const double inv = keep ? 1.0 / lp : 0.0;
+    const int r = rank[s.s_off + p];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const double2 a = X[(int64_t)p * n + i];
+      Yout[(int64_t)r * n + i] = make_double2(a.x * inv, a.y * inv);
+    }
+  }
+}
+
+// V = T Y diag(1/sigma) (L > R sectors); Y lives in the W area, n x n col-major.
+__global__ __launch_bounds__(256) void k_vectors(PlanHeader h, const char *work,
+                                                 const double *__restrict__ T,
+                                                 const double *__restrict__ Wbuf,
+                                                 const double *__restrict__ Sbuf,
+                                                 double *__restrict__ Vbuf) {
+  const DevPlan P = dev_plan(h, work);
+  const int lane = threadIdx.x & 63;
+  const int i16 = lane & 15, kq = lane >> 4;
+  const int nwaves = gridDim.x * 4;
+  const double2 *Tc = reinterpret_cast<const double2 *>(T);
+  const double2 *Yc = reinterpret_cast<const double2 *>(Wbuf);
+  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < h.n_v_items; it += nwaves) {
+    const Item item = P.v_items[it];
+    const Sector s = P.sectors[item.sector];
+    const int i = item.r0 + i16, p = item.c0 + i16;
+    const bool iv = i16 < item.nr, pv = i16 < item.nc;
+    dbl4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < s.R; k0 += 4) {
+      const int k = k0 + kq;
+      cplx a{0.0, 0.0}, b{0.0, 0.0};
+      if (k < s.R) {
+        if (iv) { const double2 v = Tc[s.c_off + (int64_t)i * s.R + k]; a = cplx{v.x, v.y}; }
+        if (pv) { const double2 v = Yc[s.w_off + (int64_t)p * s.n + k]; b = cplx{v.x, v.y}; }
+      }
+      cmfma_f64(accr, acci, a, b);
+    }
+    if (pv) {
+      const double sg = Sbuf[s.s_off + p];
+      const double inv = sg > 0.0 ? 1.0 / sg : 0.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 4 * kq + r;
+        if (row < item.nr)
+          cstore(Vbuf, s.v_off + (int64_t)p * s.L + item.r0 + row,
+                 cplx{accr[r] * inv, acci[r] * inv});
+      }
+    }
+  }
+}
+
+// W (n x R row-major) = V^H C, V L x n column-major, C L x R row-major.
+__global__ __launch_bounds__(256) void k_wmat(PlanHeader h, const char *work,
+                                              const double *__restrict__ Vbuf,
+                                              const double *__restrict__ C,
+                                              double *__restrict__ Wbuf) {
+  const DevPlan P = dev_plan(h, work);
+  const int lane = threadIdx.x & 63;
+  const int i16 = lane & 15, kq = lane >> 4;
+  const int nwaves = gridDim.x * 4;
+  const double2 *Vc = reinterpret_cast<const double2 *>(Vbuf);
+  const double2 *Cc = reinterpret_cast<const double2 *>(C);
+  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < h.n_w_items; it += nwaves) {
+    const Item item = P.w_items[it];
+    const Sector s = P.sectors[item.sector];
+    const int p = item.r0 + i16, j = item.c0 + i16;
+    const bool pv = i16 < item.nr, jv = i16 < item.nc;
+    dbl4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < s.L; k0 += 4) {
+      const int k = k0 + kq;
+      cplx a{0.0, 0.0}, b{0.0, 0.0};
+      if (k < s.L) {
+        if (pv) { const double2 v = Vc[s.v_off + (int64_t)p * s.L + k]; a = cplx{v.x, -v.y}; }
+        if (jv) { const double2 v = Cc[s.c_off + (int64_t)k * s.R + j]; b = cplx{v.x, v.y}; }
+      }
+      cmfma_f64(accr, acci, a, b);
+    }
+    if (jv) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 4 * kq + r;
+        if (row < item.nr)
+          cstore(Wbuf, s.w_off + (int64_t)(item.r0 + row) * s.R + j, cplx{accr[r], acci[r]});
+      }
+    }
+  }
+}
+
+int launch_svd(const PlanHeader &h, void *d_work, hipStream_t stream) {
+  char *w = reinterpret_cast<char *>(d_work);
+  double *T = reinterpret_cast<double *>(w + h.w_T);
+  double *C = reinterpret_cast<double *>(w + h.w_C);
+  double *G = reinterpret_cast<double *>(w + h.w_G);
+  double *V = reinterpret_cast<double *>(w + h.w_V);
+  double *W = reinterpret_cast<double *>(w + h.w_W);
+  double *S = reinterpret_cast<double *>(w + h.w_S);
+  int32_t *F = reinterpret_cast<int32_t *>(w + h.w_flag);
+  double *LAM = reinterpret_cast<double *>(w + h.w_lam);
+  int32_t *RK = reinterpret_cast<int32_t *>(w + h.w_slot);
+  if (h.n_sectors == 0) return 0;
+  hipLaunchKernelGGL(k_gram, dim3(grid_for(h.n_gram_items)), dim3(256), 0, stream, h,
+                     (const char *)d_work, (const double *)T, G);
+  BSMPO_HIP(hipGetLastError());
+  static bool attr_set = false;
+  if (!attr_set) {
+    BSMPO_HIP(hipFuncSetAttribute((const void *)k_jacobi<true>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 96 * 16));
+    attr_set = true;
+  }
+  for (int c = 0; c < kJacClasses; ++c) {
+    const int nsec = h.jac_off[c + 1] - h.jac_off[c];
+    if (nsec == 0) continue;
+    if (kJacLimit[c] <= 96) {
+      const size_t lds = (size_t)h.jac_max_n[c] * h.jac_max_n[c] * 16;
+      hipLaunchKernelGGL(k_jacobi<true>, dim3(nsec), dim3(256), lds, stream, h,
+                         (const char *)d_work, h.jac_off[c], G, V, W, S, F, LAM, RK);
+    } else {
+      hipLaunchKernelGGL(k_jacobi<false>, dim3(nsec), dim3(256), 0, stream, h,
+                         (const char *)d_work, h.jac_off[c], G, V, W, S, F, LAM, RK);
+    }
+    BSMPO_HIP(hipGetLastError());
+  }
+  if (h.n_v_items) {
+    hipLaunchKernelGGL(k_vectors, dim3(grid_for(h.n_v_items)), dim3(256), 0, stream, h,
+                       (const char *)d_work, (const double *)T, (const double *)W,
+                       (const double *)S, V);
+    BSMPO_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_wmat, dim3(grid_for(h.n_w_items)), dim3(256), 0, stream, h,
+                     (const char *)d_work, (const double *)V, (const double *)C, W);
+  BSMPO_HIP(hipGetLastError());
+  return 0;
+}
+
+// ========================================================= truncation ==
+// One block: global top-chi over all sectors' singular values (ties by the
+// lower concatenated index), then slots in concatenated order, which is the
+// (tau0, tau1, descending sigma) order of the stable charge sort.
+__device__ int block_sum_int(int v, int *scratch) {
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int t = 0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += scratch[i];
+  __syncthreads();
+  return t;
+}
+
+// exclusive prefix sum over a 256-thread block (Hillis-Steele in LDS)
+__device__ int block_scan_excl(int v, int *sc, int &total) {
+  __syncthreads();
+  sc[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    const int add = threadIdx.x >= (unsigned)o ? sc[threadIdx.x - o] : 0;
+    __syncthreads();
+    sc[threadIdx.x] += add;
+    __syncthreads();
+  }
+  total = sc[255];
+  const int incl = sc[threadIdx.x];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(256) void k_topk(PlanHeader h, const char *work,
+                                              const double *__restrict__ S,
+                                              const int32_t *__restrict__ flags,
+                                              int32_t *__restrict__ slot,
+                                              int32_t *__restrict__ charge_c,
+                                              double *__restrict__ lambda_c,
+                                              int32_t *__restrict__ num_lambda) {
+  __shared__ int scratch[16];
+  const DevPlan P = dev_plan(h, work);
+  const int64_t nt = h.sv_total;
+  const unsigned long long *key = reinterpret_cast<const unsigned long long *>(S);
+  int valid = 0;
+  for (int64_t i = threadIdx.x; i < nt; i += blockDim.x) valid += flags[i] ? 1 : 0;
+  valid = block_sum_int(valid, scratch);
+  const int K = min(valid, h.chi);
+  // K-th largest key among valid entries, by bitwise binary search
+  unsigned long long t = 0ull;
+  if (K > 0) {
+    for (int bit = 62; bit >= 0; --bit) {
+      const unsigned long long cand = t | (1ull << bit);
+      int cnt = 0;
+      for (int64_t i = threadIdx.x; i < nt; i += blockDim.x) cnt += (flags[i] && key[i] >= cand) ? 1 : 0;
+      cnt = block_sum_int(cnt, scratch);
+      if (cnt >= K) t = cand;
+    }
+  }
+  int gt = 0;
+  for (int64_t i = threadIdx.x; i < nt; i += blockDim.x) gt += (flags[i] && key[i] > t) ? 1 : 0;
+  gt = block_sum_int(gt, scratch);
+  const int need_eq = K - gt;
+  // scan in index order: kept = key > t, or key == t among the first need_eq
+  __shared__ int sc[256];
+  int carry_eq = 0, carry_kept = 0;
+  for (int64_t base = 0; base < nt; base += blockDim.x) {
+    const int64_t i = base + threadIdx.x;
+    const bool v = i < nt && flags[i] && K > 0;
+    const bool gtf = v && key[i] > t;
+    const bool eqf = v && key[i] == t;
+    int tot_eq;
+    const int eq_rank = carry_eq + block_scan_excl(eqf ? 1 : 0, sc, tot_eq);
+    const bool kept = gtf || (eqf && eq_rank < need_eq);
+    int tot_kept;
+    const int pos = carry_kept + block_scan_excl(kept ? 1 : 0, sc, tot_kept);
+    carry_eq += tot_eq;
+    carry_kept += tot_kept;
+    if (i < nt) {
+      slot[i] = kept ? pos : -1;
+      if (kept) {
+        const Sector s = P.sectors[P.sv_sector[i]];
+        charge_c[2 * pos] = s.t0;
+        charge_c[2 * pos + 1] = s.t1;
+        lambda_c[pos] = S[i];
+      }
+    }
+  }
+  for (int j = K + threadIdx.x; j < h.chi; j += blockDim.x) {
+    charge_c[2 * j] = h.d;
+    charge_c[2 * j + 1] = h.d;
+    lambda_c[j] = 0.0;
+  }
+  if (threadIdx.x == 0) *num_lambda = K;
+}
+
+// zero the stored part of the outputs (Full.py:130-135 edge storage)
+__global__ void k_zero(double *__restrict__ G0, double *__restrict__ G1, int chi, int w0,
+                       int w1, int c1) {
+  const int64_t tot0 = (int64_t)chi * w0, tot1 = (int64_t)w1 * c1;
+  double2 *a = reinterpret_cast<double2 *>(G0);
+  double2 *b = reinterpret_cast<double2 *>(G1);
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot0 + tot1;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    if (e < tot0) {
+      const int64_t r = e / w0, c = e % w0;
+      a[r * chi + c] = make_double2(0.0, 0.0);
+    } else {
+      const int64_t f = e - tot0;
+      const int64_t r = f / c1, c = f % c1;
+      b[r * chi + c] = make_double2(0.0, 0.0);
+    }
+  }
+}
+
+// one block per singular-value slot: write its V column and W row
+__global__ __launch_bounds__(256) void k_scatter(PlanHeader h, const char *work,
+                                                 const int32_t *__restrict__ slot,
+                                                 const double *__restrict__ Vbuf,
+                                                 const double *__restrict__ Wbuf,
+                                                 double *__restrict__ G0,
+                                                 double *__restrict__ G1, int w0, int w1,
+                                                 int c1) {
+  const DevPlan P = dev_plan(h, work);
+  const int chi = h.chi;
+  for (int64_t i = blockIdx.x; i < h.sv_total; i += gridDim.x) {
+    const int j = slot[i];
+    if (j < 0) continue;
+    const int sid = P.sv_sector[i];
+    const Sector s = P.sectors[sid];
+    const int p = (int)(i - s.s_off);
+    const double2 *V = reinterpret_cast<const double2 *>(Vbuf) + s.v_off + (int64_t)p * s.L;
+    const double2 *W = reinterpret_cast<const double2 *>(Wbuf) + s.w_off + (int64_t)p * s.R;
+    double2 *a = reinterpret_cast<double2 *>(G0);
+    double2 *b = reinterpret_cast<double2 *>(G1);
+    if (j < w0)
+      for (int r = threadIdx.x; r < s.L; r += blockDim.x)
+        a[(int64_t)P.rows[s.row_off + r] * chi + j] = V[r];
+    if (j < w1)
+      for (int c = threadIdx.x; c < s.R; c += blockDim.x) {
+        const int gc = P.cols[s.col_off + c];
+        if (gc < c1) b[(int64_t)j * chi + gc] = W[c];
+      }
+  }
+}
+
+int launch_truncate(const PlanHeader &h, void *d_work, double *d_Glc, double *d_Gcr,
+                    int32_t *d_charge_c, double *d_lambda_c, int32_t *d_num_lambda, int edge,
+                    hipStream_t stream) {
+  char *w = reinterpret_cast<char *>(d_work);
+  const double *V = reinterpret_cast<const double *>(w + h.w_V);
+  const double *W = reinterpret_cast<const double *>(w + h.w_W);
+  const double *S = reinterpret_cast<const double *>(w + h.w_S);
+  const int32_t *F = reinterpret_cast<const int32_t *>(w + h.w_flag);
+  int32_t *slot = reinterpret_cast<int32_t *>(w + h.w_slot);
+  hipLaunchKernelGGL(k_topk, dim3(1), dim3(256), 0, stream, h, (const char *)d_work, S, F, slot,
+                     d_charge_c, d_lambda_c, d_num_lambda);
+  BSMPO_HIP(hipGetLastError());
+  const int chi = h.chi;
+  const int wd = std::min(chi, h.d * h.d);
+  const int w0 = edge ? wd : chi, w1 = edge ? wd : chi, c1 = edge ? 1 : chi;
+  const int64_t tot = (int64_t)chi * w0 + (int64_t)w1 * c1;
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<int64_t>((tot + 255) / 256, 16384)),
+                     dim3(256), 0, stream, d_Glc, d_Gcr, chi, w0, w1, c1);
+  BSMPO_HIP(hipGetLastError());
+  if (h.sv_total > 0) {
+    hipLaunchKernelGGL(k_scatter, dim3((unsigned)std::min<int64_t>(h.sv_total, 65535)),
+                       dim3(256), 0, stream, h, (const char *)d_work, (const int32_t *)slot, V, W,
+                       d_Glc, d_Gcr, w0, w1, c1);
+    BSMPO_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+}  // namespace bsmpo

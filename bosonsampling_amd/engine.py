@@ -1,0 +1,307 @@
+"""Device-resident MPO and the single-GPU simulation driver.
+
+The state lives in HBM as torch tensors used purely as buffers:
+  Gamma  [M, chi, chi] complex128, Lambda [M-1, chi] float64,
+  charge [M+1, chi, 2] int32 (lexicographically sorted, padded with d),
+with a host mirror of the charges for planning.  Every update runs through
+libbsmpo (include/bsmpo.h): host C++ planning from the charges, then the
+gfx950 kernels on the current torch stream.
+
+Reference correspondence:
+  DeviceMPO.update        RankWorker.RankProcess + FullWorker.Check
+                          (src/beam_splitter_parallel/Rank.py:32-206, Full.py:113-137)
+  simulate                FullWorker.LayerUpdate / Simulate (Full.py:149-202)
+  observables             summary.Probability / EntanglementEntropy (src/summary.py:3-36)
+  initialize              MPO.Initialize + Canonicalize (src/MPO.py:21-156)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from . import _capi
+from ._capi import check, lib, ptr
+
+C128, C64, BF16X3 = _capi.C128, _capi.C64, _capi.BF16X3
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        raise _capi.BsmpoError("no GPU visible: the bsmpo engine runs on HIP devices only")
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Workspace:
+    """Grow-only device workspace and pinned plan staging buffer."""
+
+    def __init__(self, device):
+        self.device = device
+        self.work = None
+        self.plan = None
+
+    def device_bytes(self, n):
+        if self.work is None or self.work.numel() < n:
+            self.work = torch.empty(max(n, 1 << 20) * 5 // 4, dtype=torch.uint8, device=self.device)
+        return self.work
+
+    def host_plan(self, n):
+        if self.plan is None or self.plan.numel() < n:
+            self.plan = torch.empty(max(n, 1 << 16) * 5 // 4, dtype=torch.uint8, pin_memory=True)
+        return self.plan
+
+
+class Plan:
+    """Host plan of one update (opaque blob built by bsmpo_plan_build)."""
+
+    def __init__(self, d, chi, CL, CC, CR, ws: Workspace = None, tau=None):
+        CL = np.ascontiguousarray(CL, np.int32)
+        CC = np.ascontiguousarray(CC, np.int32)
+        CR = np.ascontiguousarray(CR, np.int32)
+        t0, t1 = (-1, -1) if tau is None else (int(tau[0]), int(tau[1]))
+        pb, wb = C.c_size_t(), C.c_size_t()
+        check(lib().bsmpo_plan_query(d, chi, ptr(CL), ptr(CC), ptr(CR), t0, t1,
+                                     C.byref(pb), C.byref(wb)), "plan_query")
+        self.plan_bytes, self.work_bytes = pb.value, wb.value
+        if ws is not None:
+            self.buf = ws.host_plan(self.plan_bytes)
+        else:
+            self.buf = np.zeros(self.plan_bytes, np.uint8)
+        check(lib().bsmpo_plan_build(d, chi, ptr(CL), ptr(CC), ptr(CR), t0, t1, ptr(self.buf),
+                                     self.plan_bytes), "plan_build")
+        self.d, self.chi = d, chi
+
+    @property
+    def address(self):
+        return ptr(self.buf)
+
+    def info(self):
+        inf = _capi.PlanInfo()
+        check(lib().bsmpo_plan_info_get(C.c_void_p(self.address), C.byref(inf)), "plan_info")
+        return inf
+
+    def sectors(self):
+        """[(tau0, tau1, L, R)], packed C offsets, rows pool, cols pool."""
+        inf = self.info()
+        out = np.zeros(4 * max(inf.n_sectors, 1), np.int32)
+        coff = np.zeros(max(inf.n_sectors, 1), np.int64)
+        tot_rows = int(sum(0 for _ in ()))
+        check(lib().bsmpo_plan_sectors(C.c_void_p(self.address), ptr(out), ptr(coff), None, None),
+              "plan_sectors")
+        sec = out[:4 * inf.n_sectors].reshape(-1, 4)
+        nr = int(sec[:, 2].sum())
+        nc = int(sec[:, 3].sum())
+        rows = np.zeros(max(nr, 1), np.int32)
+        cols = np.zeros(max(nc, 1), np.int32)
+        check(lib().bsmpo_plan_sectors(C.c_void_p(self.address), None, None, ptr(rows), ptr(cols)),
+              "plan_sectors")
+        return sec, coff[:inf.n_sectors], rows[:nr], cols[:nc]
+
+
+def bs_tables(d: int, Bs: np.ndarray, device) -> torch.Tensor:
+    """U3 tables [nb, d, d, d] complex128 on the device for 2x2 matrices Bs."""
+    Bs = np.ascontiguousarray(np.asarray(Bs, np.complex128).reshape(-1, 2, 2))
+    nb = Bs.shape[0]
+    dB = torch.from_numpy(Bs.view(np.float64).copy()).to(device)
+    U3 = torch.empty((nb, d, d, d), dtype=torch.complex128, device=device)
+    check(lib().bsmpo_bs_table(d, nb, C.c_void_p(dB.data_ptr()), C.c_void_p(U3.data_ptr()),
+                               _stream()), "bs_table")
+    return U3
+
+
+class DeviceMPO:
+    """MPO state in HBM (see module docstring for the layout)."""
+
+    def __init__(self, M: int, d: int, chi: int, device="cuda"):
+        _require_gpu()
+        self.M, self.d, self.chi = M, d, chi
+        self.device = torch.device(device)
+        self.Gamma = torch.zeros((M, chi, chi), dtype=torch.complex128, device=self.device)
+        self.Lambda = torch.zeros((M - 1, chi), dtype=torch.float64, device=self.device)
+        self.charge = torch.full((M + 1, chi, 2), d, dtype=torch.int32, device=self.device)
+        self.edge = torch.ones(chi, dtype=torch.float64, device=self.device)
+        self.charge_host = np.full((M + 1, chi, 2), d, np.int32)
+        self.num_lambda = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self.ws = Workspace(self.device)
+
+    # ------------------------------------------------------------ host I/O
+    @classmethod
+    def from_host(cls, Gamma, Lambda, charge, d, device="cuda"):
+        M, chi = Gamma.shape[0], Gamma.shape[1]
+        st = cls(M, d, chi, device)
+        st.Gamma.copy_(torch.from_numpy(np.ascontiguousarray(Gamma, np.complex128)))
+        st.Lambda.copy_(torch.from_numpy(np.ascontiguousarray(Lambda, np.float64)))
+        st.charge.copy_(torch.from_numpy(np.ascontiguousarray(charge, np.int32)))
+        st.charge_host[:] = charge
+        return st
+
+    def to_host(self):
+        torch.cuda.synchronize(self.device)
+        return (self.Gamma.cpu().numpy(), self.Lambda.cpu().numpy(), self.charge.cpu().numpy())
+
+    def clone(self):
+        st = DeviceMPO(self.M, self.d, self.chi, self.device)
+        st.Gamma.copy_(self.Gamma)
+        st.Lambda.copy_(self.Lambda)
+        st.charge.copy_(self.charge)
+        st.charge_host[:] = self.charge_host
+        return st
+
+    # -------------------------------------------------------------- update
+    def plan(self, mode: int) -> Plan:
+        ch = self.charge_host
+        return Plan(self.d, self.chi, ch[mode], ch[mode + 1], ch[mode + 2], self.ws)
+
+    def update(self, mode: int, U3: torch.Tensor, precision: int = C128, plan: Plan = None,
+               sync_charges: bool = True):
+        """Apply one beam splitter (U3 table on device) to sites (mode, mode+1)."""
+        M = self.M
+        if not 0 <= mode <= M - 2:
+            raise ValueError(f"mode {mode} out of range for {M} sites")
+        if U3.dtype != torch.complex128 or tuple(U3.shape) != (self.d,) * 3 or not U3.is_cuda:
+            raise ValueError("U3 must be a device complex128 [d, d, d] table")
+        plan = plan or self.plan(mode)
+        work = self.ws.device_bytes(plan.work_bytes)
+        LR = self.edge if mode == M - 2 else self.Lambda[mode + 1]
+        check(lib().bsmpo_site_update(
+            self.d, self.chi, C.c_void_p(plan.address),
+            C.c_void_p(self.Gamma[mode].data_ptr()), C.c_void_p(self.Gamma[mode + 1].data_ptr()),
+            C.c_void_p(LR.data_ptr()), C.c_void_p(U3.data_ptr()),
+            C.c_void_p(self.charge[mode + 1].data_ptr()), C.c_void_p(self.Lambda[mode].data_ptr()),
+            C.c_void_p(self.num_lambda.data_ptr()), 1 if mode == M - 2 else 0, int(precision),
+            C.c_void_p(work.data_ptr()), work.numel(), _stream()), "site_update")
+        if sync_charges:
+            self.charge_host[mode + 1] = self.charge[mode + 1].cpu().numpy()
+        return plan
+
+    # --------------------------------------------------------- observables
+    def observables(self):
+        """(trace, marginals[M, d]) over the ket = bra diagonal."""
+        M, d, chi = self.M, self.d, self.chi
+        R = torch.empty((M + 1, chi), dtype=torch.complex128, device=self.device)
+        L = torch.empty((M + 1, chi), dtype=torch.complex128, device=self.device)
+        marg = torch.empty((M, d), dtype=torch.float64, device=self.device)
+        tr = torch.empty(1, dtype=torch.float64, device=self.device)
+        check(lib().bsmpo_observables(M, d, chi, C.c_void_p(self.Gamma.data_ptr()),
+                                      C.c_void_p(self.Lambda.data_ptr()),
+                                      C.c_void_p(self.charge.data_ptr()), C.c_void_p(R.data_ptr()),
+                                      C.c_void_p(L.data_ptr()), C.c_void_p(marg.data_ptr()),
+                                      C.c_void_p(tr.data_ptr()), _stream()), "observables")
+        return float(tr.item()), marg.cpu().numpy()
+
+    def entanglement_entropy(self):
+        """src/summary.py:3-8: S = -sum p log2 p, p = Lambda^2 / sum Lambda^2."""
+        lam = self.Lambda.cpu().numpy()
+        sq = lam ** 2
+        tot = sq.sum(axis=1, keepdims=True)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            p = sq / tot
+            return -np.nansum(p * np.log2(p), axis=1)
+
+
+# ------------------------------------------------------------ initial state
+
+def input_density(d: int, eta: float) -> np.ndarray:
+    """Single photon through a pure-loss channel of transmission eta:
+    (1 - eta)|0><0| + eta|1><1| (src/MPO.py:41-44 with loss = 1 - eta)."""
+    rho = np.zeros((d, d), np.float64)
+    rho[0, 0] = 1.0 - eta
+    rho[1, 1] = eta
+    return rho
+
+
+def product_mpo(M, d, chi, input_modes, eta, post_selected=None):
+    """Product-state MPO (host arrays) before canonicalisation; generalises
+    src/MPO.py:50-109 to any set of occupied modes and chi >= N + 1.
+    Bond i carries the diagonal charges (c, c), c = 0..(photons in sites >= i);
+    Gamma_i[(a,a), (b,b)] = rho_i[a-b, a-b]; Lambda = 1 on used slots."""
+    inputs = sorted(set(int(m) for m in input_modes))
+    rho_in = input_density(d, eta)
+    rho_vac = np.zeros((d, d))
+    rho_vac[0, 0] = 1.0
+    n_right = [sum(1 for m in inputs if m >= i) for i in range(M + 1)]
+    charge = np.full((M + 1, chi, 2), d, np.int32)
+    slots = []
+    for i in range(M + 1):
+        if i == 0:
+            cs = [post_selected] if post_selected is not None else list(range(min(n_right[0], d - 1) + 1))
+        elif i == M:
+            cs = [0]
+        else:
+            cs = list(range(min(n_right[i], d - 1) + 1))
+        if len(cs) > chi:
+            raise ValueError("chi too small for the product state")
+        for k, c in enumerate(cs):
+            charge[i, k] = (c, c)
+        slots.append(cs)
+    Gamma = np.zeros((M, chi, chi), np.complex128)
+    for i in range(M):
+        rho = rho_in if i in inputs else rho_vac
+        for a_i, a in enumerate(slots[i]):
+            for b_i, b in enumerate(slots[i + 1]):
+                n = a - b
+                if 0 <= n < d:
+                    Gamma[i, a_i, b_i] = rho[n, n]
+    Lambda = np.zeros((M - 1, chi), np.float64)
+    for i in range(1, M):
+        Lambda[i - 1, :len(slots[i])] = 1.0
+    return Gamma, Lambda, charge
+
+
+def canonicalisation_bs(d: int) -> np.ndarray:
+    """Single-particle matrix of the reference canonicalisation sweep,
+    Rand_BS_MPS(0, d, 0) (src/MPO.py:6-10,151): t = exp(2 pi i u), r = 0."""
+    rs = np.random.RandomState(0)
+    t = np.exp(1j * rs.rand() * 2 * np.pi)
+    return np.array([[t, 0.0], [0.0, np.conj(t)]], dtype=np.complex128)
+
+
+def initialize(M, d, chi, input_modes, eta, post_selected=None, device="cuda") -> DeviceMPO:
+    """Initial MPO on the device: product state, then the right-to-left
+    canonicalisation sweep of src/MPO.py:137-156, run by the GPU update."""
+    G, L, ch = product_mpo(M, d, chi, input_modes, eta, post_selected)
+    st = DeviceMPO.from_host(G, L, ch, d, device)
+    U3 = bs_tables(d, canonicalisation_bs(d), st.device)[0]
+    for mode in range(M - 2, -1, -1):
+        st.update(mode, U3)
+    return st
+
+
+# ---------------------------------------------------------------- schedule
+
+def layer_modes(M: int, layer: int, max_input: int):
+    """(position, mode) of the non-skipped BSs of a brick-wall layer
+    (Full.py:152-153; skip mode > last occupied input + layer)."""
+    return [(i, mode) for i, mode in enumerate(range(layer % 2, M - 1, 2))
+            if mode <= max_input + layer]
+
+
+def simulate(st: DeviceMPO, mesh: np.ndarray, input_modes, layers=None, precision=C128,
+             observe=True, callback=None):
+    """Apply the mesh layer by layer (Full.py:169-202).  Returns per-layer
+    trace prob[M+1] and EE[M-1, M+1] (NaN where not observed)."""
+    M, d = st.M, st.d
+    max_input = max(input_modes)
+    prob = np.full(M + 1, np.nan)
+    EE = np.full((M - 1, M + 1), np.nan)
+    if observe:
+        prob[0] = st.observables()[0]
+        EE[:, 0] = st.entanglement_entropy()
+    for layer in (range(M) if layers is None else layers):
+        todo = layer_modes(M, layer, max_input)
+        if not todo:
+            continue
+        U3 = bs_tables(d, np.stack([mesh[layer, i] for i, _ in todo]), st.device)
+        for k, (i, mode) in enumerate(todo):
+            st.update(mode, U3[k], precision)
+            if callback is not None:
+                callback(layer, i, mode, st)
+        if observe:
+            prob[layer + 1] = st.observables()[0]
+            EE[:, layer + 1] = st.entanglement_entropy()
+    return prob, EE

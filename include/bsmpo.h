@@ -1,0 +1,166 @@
+/*
+ * bsmpo.h -- C ABI of the MI355X (gfx950) beam-splitter site update for the
+ * lossy boson-sampling MPO.  Plain pointers and sizes only.
+ *
+ * Conventions
+ *  - complex numbers are complex128 (interleaved re, im doubles) unless a
+ *    function says complex64 (interleaved floats);
+ *  - matrices are row-major; Gamma tensors are chi x chi per site;
+ *  - charges are int32 pairs (ket, bra) per bond slot, lexicographically
+ *    sorted, unused slots padded with d;
+ *  - every pointer named d_* is device memory owned by the caller; the
+ *    library never allocates or frees caller memory.  Functions that need
+ *    scratch take a caller-provided workspace (d_work, work_bytes) whose
+ *    size is obtained from the matching *_query call;
+ *  - all device work is enqueued on `stream` (a hipStream_t passed as
+ *    void*; NULL = the default stream) and nothing synchronises the host
+ *    unless the function says so;
+ *  - return value: 0 ok, > 0 a hipError_t, < 0 invalid argument; the text
+ *    of the last error of the calling thread is bsmpo_last_error().
+ */
+#ifndef BSMPO_H
+#define BSMPO_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- misc -- */
+
+/* Text of the last error raised on this thread ("" if none). */
+const char *bsmpo_last_error(void);
+
+/* ABI version (increments on any signature change). */
+int bsmpo_abi_version(void);
+
+/* Contraction arithmetic of the sector GEMMs. */
+enum bsmpo_precision {
+  BSMPO_C128 = 0,    /* complex128 on FP64 MFMA                             */
+  BSMPO_C64 = 1,     /* complex64 on FP32 MFMA, FP32 accumulation            */
+  BSMPO_BF16X3 = 2   /* split-BF16x3 compensated products, FP32 accumulation */
+};
+
+/* ------------------------------------------------- beam-splitter table -- */
+
+/*
+ * Fock-space beam-splitter elements for nb beam splitters.
+ *   d_B   : device, nb x 2 x 2 complex128, single-particle matrix, new = B old
+ *   d_U3  : device, nb x d x d x d complex128,
+ *           U3[n1][n2][p] = <n1, n2 | U(B) | p, n1 + n2 - p>  (0 outside)
+ * Replaces src/cuda_kernels.py:18-50 (Rand_U) + src/u_kernel.cu:37-70
+ * (`unitary`); the reference's B = [[t, r], [-r*, t*]] is one such matrix.
+ * Evaluated in FP64 (the reference stores float).
+ */
+int bsmpo_bs_table(int d, int nb, const double *d_B, double *d_U3, void *stream);
+
+/* ------------------------------------------------------ update planning -- */
+
+/*
+ * Host-side plan of one site update from the three bonds' charges.
+ *   h_CL, h_CC, h_CR : host int32 [chi][2] (left, centre, right bond)
+ * The plan is written into an opaque host buffer of plan_bytes bytes
+ * (size from bsmpo_plan_query with the same arguments) and contains the
+ * sector list (tau, rows cl >= tau, cols cr <= tau), the charge-block tables
+ * and the contraction work list.  tau0 < 0 plans every sector of the update;
+ * tau0, tau1 >= 0 plans that one sector only (the update_MPO drop-in).
+ * Replaces the Aligner of src/mpo_sort.py:39-418 and the tau loop bounds of
+ * src/beam_splitter_parallel/Rank.py:60-105.
+ */
+int bsmpo_plan_query(int d, int chi, const int32_t *h_CL, const int32_t *h_CC,
+                     const int32_t *h_CR, int tau0, int tau1, size_t *plan_bytes,
+                     size_t *work_bytes);
+int bsmpo_plan_build(int d, int chi, const int32_t *h_CL, const int32_t *h_CC,
+                     const int32_t *h_CR, int tau0, int tau1, void *h_plan, size_t plan_bytes);
+
+/* Summary numbers of a built plan (host, no device work). */
+typedef struct {
+  int n_sectors;        /* non-empty tau sectors                                */
+  int n_items;          /* contraction work items                               */
+  int64_t c_elems;      /* sum_tau L_tau R_tau                                  */
+  int64_t g_elems;      /* sum_tau n_tau^2, n = min(L, R)                      */
+  int64_t sv_total;     /* sum_tau n_tau                                        */
+  double flops_alg;     /* 8 sum_{cl>=ch>=cr} nL nC nR (complex MAC = 8 flops) */
+  double flops_tau;     /* 8 sum_tau sum_{cl>=tau>=cr} sum_{cl>=ch>=cr} nL nC nR */
+  double bytes_alg16;   /* B_alg with s = 16 bytes                               */
+} bsmpo_plan_info;
+int bsmpo_plan_info_get(const void *h_plan, bsmpo_plan_info *info);
+
+/* ------------------------------------------------------- the site update -- */
+
+/*
+ * One beam-splitter site update on the pair (mode, mode+1), RankProcess
+ * semantics (src/beam_splitter_parallel/Rank.py:32-206) with the master's
+ * merge (src/beam_splitter_parallel/Full.py:113-137):
+ *   C_tau = sum_c Glc[l, c] Gcr[c, r] u(cl, cc, cr; tau),  T = C diag(LR),
+ *   T = V S W' (FP64), W = V^H C, global top-chi of S over all sectors,
+ *   stable sort by charge, scatter.
+ * In place on the caller's MPO buffers:
+ *   d_Glc, d_Gcr : device chi x chi complex128, Gamma[mode], Gamma[mode+1]
+ *                  (read, then overwritten with Gamma0Out / Gamma1Out)
+ *   d_LR         : device chi float64 (Lambda[mode+1], or ones at the edge)
+ *   d_U3         : device d^3 complex128 (bsmpo_bs_table)
+ *   d_charge_c   : device int32 [chi][2], new centre charges (padded d)
+ *   d_lambda_c   : device float64 [chi], new centre Lambda (padded 0)
+ *   edge         : nonzero for the last pair (mode == M-2): only columns /
+ *                  rows < min(chi, d*d) of the outputs are stored
+ *                  (Full.py:130-132)
+ *   precision    : enum bsmpo_precision for the sector contraction; the
+ *                  SVD, truncation, BS elements stay FP64
+ *   d_work       : device workspace of work_bytes (bsmpo_plan_query)
+ *   d_num_lambda : device int32[1], number of kept singular values
+ * The host plan must come from bsmpo_plan_build on the same charges.
+ */
+int bsmpo_site_update(int d, int chi, const void *h_plan, double *d_Glc, double *d_Gcr,
+                      const double *d_LR, const double *d_U3, int32_t *d_charge_c,
+                      double *d_lambda_c, int32_t *d_num_lambda, int edge, int precision,
+                      void *d_work, size_t work_bytes, void *stream);
+
+/*
+ * Stage entry point of the same update (used by the update_MPO / update_MPS
+ * drop-ins and by tests): the sector contraction alone.  Writes the packed
+ * C_tau and T_tau = diag(LL[rows]) C_tau diag(LR[cols]) (row-major
+ * L_tau x R_tau each, sectors in plan order) into d_C and d_T, complex128.
+ * Optional d_LL (chi float64, row weights of T) and d_LC (chi float64,
+ * centre weights inside the sum) may be NULL; they give the pure-state
+ * variant of src/update_kernel.cu:594-595,706-715.
+ * Replaces src/cuda_kernels.py:101-127 (update_MPO) + :72-90 (update_MPS),
+ * src/update_MPO_kernel.cu:193-734 and src/update_kernel.cu:193-769
+ * (`kernel`), and the LR weight and compaction of
+ * src/beam_splitter_parallel/Rank.py:115-118.
+ */
+int bsmpo_contract(int d, int chi, const void *h_plan, const double *d_Glc,
+                   const double *d_Gcr, const double *d_LR, const double *d_U3,
+                   double *d_C, double *d_T, const double *d_LL, const double *d_LC,
+                   int precision, void *d_work, size_t work_bytes, void *stream);
+
+/* Copies the device offsets of the packed per-sector buffers into h_out:
+ * for sector s: h_out[4 s + 0..3] = tau0, tau1, L, R, and the packed C/T
+ * element offset is returned in h_coff[s].  Host only. */
+int bsmpo_plan_sectors(const void *h_plan, int32_t *h_out, int64_t *h_coff,
+                       int32_t *h_rows, int32_t *h_cols);
+
+/* --------------------------------------------------------- observables -- */
+
+/*
+ * Right environments over the ket = bra diagonal of an M-site MPO
+ * (src/summary.py:11-26 chain):  R[M] = e_0,
+ * R[b] = Gamma[b] (R[b+1] * mask[b+1]), mask = (c0 == c1 < d) and, on inner
+ * bonds, Lambda > 0.  Left environments the same from the left.
+ *   d_Gamma  : device M x chi x chi complex128
+ *   d_Lambda : device (M-1) x chi float64
+ *   d_charge : device (M+1) x chi x 2 int32
+ *   d_R, d_L : device (M+1) x chi complex128
+ *   d_marg   : device M x d float64, P(n photons in mode j) (unnormalised)
+ *   d_trace  : device float64[1], sum over bond 0 of R[0] (the "norm")
+ */
+int bsmpo_observables(int M, int d, int chi, const double *d_Gamma, const double *d_Lambda,
+                      const int32_t *d_charge, double *d_R, double *d_L, double *d_marg,
+                      double *d_trace, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BSMPO_H */

@@ -1,0 +1,111 @@
+"""CPU checks of the C ABI: the library loads, exports every declared symbol,
+and the host planner reproduces the oracle's sector selection exactly."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import fock, mpo
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "bsmpo.h")).read()
+    return sorted(set(re.findall(r"\b(bsmpo_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from bosonsampling_amd import _capi
+    L = _capi.lib()
+    missing = [n for n in _declared() if not hasattr(L, n)]
+    assert not missing
+    assert set(_declared()) == set(_capi.SIGNATURES)
+    assert L.bsmpo_abi_version() >= 1
+
+
+def _state(M=10, d=6, chi=48, inputs=(0, 2, 4, 6, 8), eta=0.7, seed=1, layers=5):
+    from bosonsampling_amd import mesh
+    mm, _ = mesh.haar_clements_mesh(M, seed)
+    st = mpo.initialize(M, d, chi, list(inputs), eta)
+    mpo.simulate(st, mm, list(inputs), layers=range(layers))
+    return st, mm
+
+
+@pytest.mark.parametrize("mode", [0, 3, 4, 8])
+def test_planner_matches_oracle_sectors(mode):
+    from bosonsampling_amd.engine import Plan
+    st, mm = _state()
+    d, chi = st.d, st.chi
+    U3 = fock.bs_table(d, mm[0, 0])
+    ref = mpo.contract_sectors(d, U3, st.charge[mode], st.charge[mode + 1], st.charge[mode + 2],
+                               st.Gamma[mode], st.Gamma[mode + 1])
+    p = Plan(d, chi, st.charge[mode], st.charge[mode + 1], st.charge[mode + 2])
+    sec, coff, rows, cols = p.sectors()
+    assert sec.shape[0] == len(ref)
+    ro = co = 0
+    for k, s in enumerate(ref):
+        assert tuple(sec[k, :2]) == s.tau
+        L, R = sec[k, 2], sec[k, 3]
+        assert np.array_equal(rows[ro:ro + L], s.rows) and np.array_equal(cols[co:co + R], s.cols)
+        ro, co = ro + L, co + R
+    inf = p.info()
+    assert inf.c_elems == sum(s.rows.size * s.cols.size for s in ref)
+    assert inf.sv_total == sum(min(s.rows.size, s.cols.size) for s in ref)
+
+
+def _alg_counts(d, CL, CC, CR):
+    bl, bc, br = (mpo.charge_blocks(c, d) for c in (CL, CC, CR))
+    n = lambda b, q: b[q][1] - b[q][0]
+    f = sum(8 * n(bl, a) * n(bc, c) * n(br, b) for a in bl for c in bc for b in br
+            if a[0] >= c[0] >= b[0] and a[1] >= c[1] >= b[1])
+    return f
+
+
+def test_planner_algorithmic_flops():
+    from bosonsampling_amd.engine import Plan
+    st, _ = _state()
+    mode = 4
+    p = Plan(st.d, st.chi, st.charge[mode], st.charge[mode + 1], st.charge[mode + 2])
+    assert p.info().flops_alg == _alg_counts(st.d, st.charge[mode], st.charge[mode + 1],
+                                             st.charge[mode + 2])
+    assert p.info().flops_tau >= p.info().flops_alg
+
+
+def test_single_sector_plan():
+    from bosonsampling_amd.engine import Plan
+    st, _ = _state()
+    mode = 4
+    full = Plan(st.d, st.chi, st.charge[mode], st.charge[mode + 1], st.charge[mode + 2]).sectors()[0]
+    t = tuple(full[len(full) // 2, :2])
+    one = Plan(st.d, st.chi, st.charge[mode], st.charge[mode + 1], st.charge[mode + 2], tau=t)
+    sec = one.sectors()[0]
+    assert sec.shape[0] == 1 and tuple(sec[0]) == tuple(full[len(full) // 2])
+
+
+def test_planner_rejects_bad_charges():
+    from bosonsampling_amd._capi import BsmpoError
+    from bosonsampling_amd.engine import Plan
+    d, chi = 4, 8
+    ok = np.full((chi, 2), d, np.int32)
+    ok[:3] = [[0, 0], [1, 1], [2, 2]]
+    bad = ok.copy()
+    bad[[0, 2]] = bad[[2, 0]]          # unsorted
+    with pytest.raises(BsmpoError, match="sorted"):
+        Plan(d, chi, bad, ok, ok)
+    bad = ok.copy()
+    bad[0] = [7, 0]                    # out of range
+    with pytest.raises(BsmpoError, match="range"):
+        Plan(d, chi, ok, bad, ok)
+
+
+def test_site_update_rejects_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    from bosonsampling_amd import engine
+    from bosonsampling_amd._capi import BsmpoError
+    with pytest.raises(BsmpoError):
+        engine.DeviceMPO(4, 3, 8)
