@@ -32,7 +32,27 @@ SIGNATURES = {
     "bsmpo_contract": (C.c_int, [_i, _i, _vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i, _vp,
                                  _sz, _vp]),
     "bsmpo_observables": (C.c_int, [_i, _i, _i, _dp, _dp, _i32p, _dp, _dp, _dp, _dp, _vp]),
+    "bsmpo_timing_enable": (C.c_int, [_i]),
+    "bsmpo_timing_read": (C.c_int, [_dp, _dp, _dp, _i64p, _i]),
 }
+
+STAGES = ("bs_table", "contract", "gram", "jacobi", "vectors", "wmat", "topk", "scatter",
+          "observables")
+
+
+def timing_enable(on: bool = True):
+    check(lib().bsmpo_timing_enable(1 if on else 0), "timing_enable")
+
+
+def timing_read(reset: bool = True):
+    """{stage: (ms, flops, bytes, launches)} accumulated since the last reset."""
+    import numpy as np
+    n = len(STAGES)
+    ms, fl, by = np.zeros(n), np.zeros(n), np.zeros(n)
+    cnt = np.zeros(n, np.int64)
+    check(lib().bsmpo_timing_read(ms.ctypes.data, fl.ctypes.data, by.ctypes.data,
+                                  cnt.ctypes.data, 1 if reset else 0), "timing_read")
+    return {s: (float(ms[i]), float(fl[i]), float(by[i]), int(cnt[i])) for i, s in enumerate(STAGES)}
 
 
 class PlanInfo(C.Structure):

@@ -1,6 +1,9 @@
 // extern "C" entry points of libbsmpo (declared in include/bsmpo.h).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <vector>
+
 #include "internal.h"
 
 namespace bsmpo {
@@ -8,6 +11,45 @@ int launch_observables(int M, int d, int chi, const double *Gamma, const double 
                        const int32_t *charge, double *R, double *L, double *marg, double *trace,
                        hipStream_t stream);
 }
+
+namespace bsmpo {
+namespace {
+struct TimingRec {
+  int stage;
+  hipEvent_t a, b;
+  double flops, bytes;
+  bool closed;
+};
+std::mutex g_tmu;
+bool g_timing = false;
+std::vector<TimingRec> g_recs;
+double g_ms[ST_COUNT], g_flops[ST_COUNT], g_bytes[ST_COUNT];
+int64_t g_cnt[ST_COUNT];
+}  // namespace
+
+bool timing_on() { return g_timing; }
+
+int timing_begin(int stage, hipStream_t stream) {
+  if (!g_timing) return -1;
+  TimingRec r{stage, nullptr, nullptr, 0.0, 0.0, false};
+  if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return -1;
+  hipEventRecord(r.a, stream);
+  std::lock_guard<std::mutex> lk(g_tmu);
+  g_recs.push_back(r);
+  return (int)g_recs.size() - 1;
+}
+
+void timing_end(int token, hipStream_t stream, double flops, double bytes) {
+  if (token < 0) return;
+  std::lock_guard<std::mutex> lk(g_tmu);
+  if (token >= (int)g_recs.size()) return;
+  TimingRec &r = g_recs[token];
+  hipEventRecord(r.b, stream);
+  r.flops = flops;
+  r.bytes = bytes;
+  r.closed = true;
+}
+}  // namespace bsmpo
 
 using namespace bsmpo;
 
@@ -26,6 +68,41 @@ int upload_plan(const void *h_plan, void *d_work, size_t work_bytes, hipStream_t
 }  // namespace
 
 extern "C" {
+
+int bsmpo_timing_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_tmu);
+  g_timing = on != 0;
+  return 0;
+}
+
+int bsmpo_timing_read(double *ms, double *flops, double *bytes, int64_t *count, int reset) {
+  std::vector<TimingRec> recs;
+  {
+    std::lock_guard<std::mutex> lk(g_tmu);
+    recs.swap(g_recs);
+  }
+  for (TimingRec &r : recs) {
+    if (r.closed) {
+      BSMPO_HIP(hipEventSynchronize(r.b));
+      float t = 0.f;
+      BSMPO_HIP(hipEventElapsedTime(&t, r.a, r.b));
+      g_ms[r.stage] += t;
+      g_flops[r.stage] += r.flops;
+      g_bytes[r.stage] += r.bytes;
+      g_cnt[r.stage] += 1;
+    }
+    hipEventDestroy(r.a);
+    hipEventDestroy(r.b);
+  }
+  for (int i = 0; i < ST_COUNT; ++i) {
+    if (ms) ms[i] = g_ms[i];
+    if (flops) flops[i] = g_flops[i];
+    if (bytes) bytes[i] = g_bytes[i];
+    if (count) count[i] = g_cnt[i];
+    if (reset) g_ms[i] = g_flops[i] = g_bytes[i] = 0.0, g_cnt[i] = 0;
+  }
+  return 0;
+}
 
 int bsmpo_bs_table(int d, int nb, const double *d_B, double *d_U3, void *stream) {
   if (nb > 0 && (!d_B || !d_U3)) return fail(-1, "bs_table: NULL pointer");

@@ -31,9 +31,10 @@ __device__ __forceinline__ void cstore(double *p, int64_t i, cplx v) {
 }
 
 // One K=4 step of a complex 16x16 tile on FP64 MFMA.
-// Lane layout of v_mfma_f64_16x16x4f64 (64 lanes):
+// Lane layout of v_mfma_f64_16x16x4f64 (64 lanes), measured on gfx950:
 //   A[i][k] from lane i + 16 k,  B[k][j] from lane j + 16 k,
-//   D[4 (l / 16) + r][l % 16] in element r of lane l's accumulator.
+//   D[l / 16 + 4 r][l % 16] in element r of lane l's accumulator.
+__device__ __forceinline__ int mfma64_row(int kq, int r) { return kq + 4 * r; }
 __device__ __forceinline__ void cmfma_f64(dbl4 &accr, dbl4 &acci, cplx a, cplx b) {
   accr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, accr, 0, 0, 0);
   accr = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, accr, 0, 0, 0);

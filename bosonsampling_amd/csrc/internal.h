@@ -38,6 +38,9 @@ struct PlanHeader {
   int64_t c_elems, g_elems, sv_total, v_elems, w_elems;
   int64_t rows_total, cols_total;
   double flops_alg, flops_tau, bytes_alg16;
+  // flops of the SVD stages (complex MAC = 8): Gram 8 m n^2, W 8 n L R,
+  // vectors 8 L R n, Jacobi one sweep of 8 n^3 (reported per sweep-equivalent)
+  double flops_gram, flops_wmat, flops_vectors, flops_jacobi;
   // host byte offsets of the device-copied region and total blob size
   int64_t dev_begin, blob_bytes;
   // byte offsets RELATIVE TO dev_begin (i.e. inside the device copy)
@@ -93,6 +96,18 @@ inline T *dev_array(void *work, int64_t dev_off) {
 }
 
 int validate_plan(const void *plan, int d, int chi);
+
+// ------------------------------------------------------------- timing --
+// Optional per-stage timing with HIP events recorded on the launch stream
+// (bsmpo_timing_enable / bsmpo_timing_read).  Off by default: no events.
+enum Stage {
+  ST_BSTABLE = 0, ST_CONTRACT, ST_GRAM, ST_JACOBI, ST_VECTORS, ST_WMAT, ST_TOPK,
+  ST_SCATTER, ST_OBSERVABLES, ST_COUNT
+};
+bool timing_on();
+// begin returns a token (or -1 when timing is off); end closes it.
+int timing_begin(int stage, hipStream_t stream);
+void timing_end(int token, hipStream_t stream, double flops, double bytes);
 
 // -------------------------------------------------------------- kernels --
 // launchers implemented in the .hip files; all enqueue on `stream`.

@@ -133,6 +133,7 @@ int launch_observables(int M, int d, int chi, const double *Gamma, const double 
                        hipStream_t stream) {
   if (M < 2 || d < 1 || d > 32 || chi < 1) return fail(-1, "observables: bad sizes");
   const int gx = (chi + 255) / 256;
+  const int tk = timing_begin(ST_OBSERVABLES, stream);
   hipLaunchKernelGGL(k_env_init, dim3(gx), dim3(256), 0, stream, M, d, chi, charge, Lambda, R, L);
   BSMPO_HIP(hipGetLastError());
   for (int b = M - 1; b >= 0; --b) {
@@ -148,6 +149,7 @@ int launch_observables(int M, int d, int chi, const double *Gamma, const double 
   hipLaunchKernelGGL(k_marginals, dim3(M), dim3(256), 0, stream, M, d, chi, Gamma, Lambda, charge,
                      (const double *)R, (const double *)L, marg, trace);
   BSMPO_HIP(hipGetLastError());
+  timing_end(tk, stream, 0.0, 3.0 * 16.0 * M * (double)chi * chi);
   return 0;
 }
 

@@ -135,6 +135,13 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
       w_off += (int64_t)s.n * R;
       s_off += s.n;
       theta_elems += (double)L * R;
+      {
+        const double n = s.n, m = std::max(L, R);
+        b.h.flops_gram += 8.0 * m * n * n;
+        b.h.flops_wmat += 8.0 * n * L * (double)R;
+        if (!s.left_small) b.h.flops_vectors += 8.0 * L * (double)R * n;
+        b.h.flops_jacobi += 8.0 * n * n * n;
+      }
       int sid = (int)b.sectors.size();
       for (auto &p : lsel) {
         const Block &bl = b.lblk[p.first];

@@ -75,8 +75,10 @@ int launch_bs_table(int d, int nb, const double *d_B, double *d_U3, hipStream_t 
     if (n > 0) f *= (long double)n;
     a.sqf[n] = (double)sqrtl(f);
   }
+  const int tk = timing_begin(ST_BSTABLE, stream);
   hipLaunchKernelGGL(k_bs_table, dim3(nb), dim3(256), 0, stream, d, d_B, d_U3, a);
   BSMPO_HIP(hipGetLastError());
+  timing_end(tk, stream, 0.0, 16.0 * nb * d * d * d);
   return 0;
 }
 
@@ -144,7 +146,7 @@ __global__ __launch_bounds__(256) void k_contract_c128(
       const double lr = LR[gcol];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = 4 * kq + r;
+        const int row = mfma64_row(kq, r);
         if (row < item.nr) {
           const int64_t idx = s.c_off + (int64_t)(item.r0 + row) * s.R + item.c0 + i16;
           const double w = LL ? lr * LL[P.rows[s.row_off + item.r0 + row]] : lr;
@@ -161,9 +163,11 @@ int launch_contract(const PlanHeader &h, void *d_work, const double *d_Glc, cons
                     const double *d_LL, const double *d_LC, int precision, hipStream_t stream) {
   if (precision != BSMPO_C128) return fail(-1, "contract: only BSMPO_C128 in this build");
   if (h.n_items == 0) return 0;
+  const int tk = timing_begin(ST_CONTRACT, stream);
   hipLaunchKernelGGL(k_contract_c128, dim3(grid_for(h.n_items)), dim3(256), 0, stream, h,
                      (const char *)d_work, d_Glc, d_Gcr, d_LR, d_U3, d_C, d_T, d_LL, d_LC);
   BSMPO_HIP(hipGetLastError());
+  timing_end(tk, stream, h.flops_alg, h.bytes_alg16);
   return 0;
 }
 
@@ -208,7 +212,7 @@ __global__ __launch_bounds__(256) void k_gram(PlanHeader h, const char *work,
     if (qv) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = 4 * kq + r;
+        const int row = mfma64_row(kq, r);
         if (row < item.nr)
           cstore(G, s.g_off + (int64_t)q * s.n + item.r0 + row, cplx{accr[r], acci[r]});
       }
@@ -394,7 +398,7 @@ __global__ __launch_bounds__(256) void k_vectors(PlanHeader h, const char *work,
       const double inv = sg > 0.0 ? 1.0 / sg : 0.0;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = 4 * kq + r;
+        const int row = mfma64_row(kq, r);
         if (row < item.nr)
           cstore(Vbuf, s.v_off + (int64_t)p * s.L + item.r0 + row,
                  cplx{accr[r] * inv, acci[r] * inv});
@@ -432,7 +436,7 @@ __global__ __launch_bounds__(256) void k_wmat(PlanHeader h, const char *work,
     if (jv) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = 4 * kq + r;
+        const int row = mfma64_row(kq, r);
         if (row < item.nr)
           cstore(Wbuf, s.w_off + (int64_t)(item.r0 + row) * s.R + j, cplx{accr[r], acci[r]});
       }
@@ -452,15 +456,18 @@ int launch_svd(const PlanHeader &h, void *d_work, hipStream_t stream) {
   double *LAM = reinterpret_cast<double *>(w + h.w_lam);
   int32_t *RK = reinterpret_cast<int32_t *>(w + h.w_slot);
   if (h.n_sectors == 0) return 0;
+  int tk = timing_begin(ST_GRAM, stream);
   hipLaunchKernelGGL(k_gram, dim3(grid_for(h.n_gram_items)), dim3(256), 0, stream, h,
                      (const char *)d_work, (const double *)T, G);
   BSMPO_HIP(hipGetLastError());
+  timing_end(tk, stream, h.flops_gram, 16.0 * (h.c_elems + h.g_elems));
   static bool attr_set = false;
   if (!attr_set) {
     BSMPO_HIP(hipFuncSetAttribute((const void *)k_jacobi<true>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 96 * 16));
     attr_set = true;
   }
+  tk = timing_begin(ST_JACOBI, stream);
   for (int c = 0; c < kJacClasses; ++c) {
     const int nsec = h.jac_off[c + 1] - h.jac_off[c];
     if (nsec == 0) continue;
@@ -474,15 +481,20 @@ int launch_svd(const PlanHeader &h, void *d_work, hipStream_t stream) {
     }
     BSMPO_HIP(hipGetLastError());
   }
+  timing_end(tk, stream, h.flops_jacobi, 16.0 * 2.0 * h.g_elems);
   if (h.n_v_items) {
+    tk = timing_begin(ST_VECTORS, stream);
     hipLaunchKernelGGL(k_vectors, dim3(grid_for(h.n_v_items)), dim3(256), 0, stream, h,
                        (const char *)d_work, (const double *)T, (const double *)W,
                        (const double *)S, V);
     BSMPO_HIP(hipGetLastError());
+    timing_end(tk, stream, h.flops_vectors, 0.0);
   }
+  tk = timing_begin(ST_WMAT, stream);
   hipLaunchKernelGGL(k_wmat, dim3(grid_for(h.n_w_items)), dim3(256), 0, stream, h,
                      (const char *)d_work, (const double *)V, (const double *)C, W);
   BSMPO_HIP(hipGetLastError());
+  timing_end(tk, stream, h.flops_wmat, 16.0 * (h.v_elems + h.c_elems + h.w_elems));
   return 0;
 }
 
@@ -640,9 +652,12 @@ int launch_truncate(const PlanHeader &h, void *d_work, double *d_Glc, double *d_
   const double *S = reinterpret_cast<const double *>(w + h.w_S);
   const int32_t *F = reinterpret_cast<const int32_t *>(w + h.w_flag);
   int32_t *slot = reinterpret_cast<int32_t *>(w + h.w_slot);
+  int tk = timing_begin(ST_TOPK, stream);
   hipLaunchKernelGGL(k_topk, dim3(1), dim3(256), 0, stream, h, (const char *)d_work, S, F, slot,
                      d_charge_c, d_lambda_c, d_num_lambda);
   BSMPO_HIP(hipGetLastError());
+  timing_end(tk, stream, 0.0, 12.0 * h.sv_total);
+  tk = timing_begin(ST_SCATTER, stream);
   const int chi = h.chi;
   const int wd = std::min(chi, h.d * h.d);
   const int w0 = edge ? wd : chi, w1 = edge ? wd : chi, c1 = edge ? 1 : chi;
@@ -656,6 +671,7 @@ int launch_truncate(const PlanHeader &h, void *d_work, double *d_Glc, double *d_
                        d_Glc, d_Gcr, w0, w1, c1);
     BSMPO_HIP(hipGetLastError());
   }
+  timing_end(tk, stream, 0.0, 16.0 * ((double)chi * w0 + (double)w1 * c1));
   return 0;
 }
 

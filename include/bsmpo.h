@@ -43,6 +43,22 @@ enum bsmpo_precision {
   BSMPO_BF16X3 = 2   /* split-BF16x3 compensated products, FP32 accumulation */
 };
 
+/* ------------------------------------------------------------- timing -- */
+
+/*
+ * Per-stage device timing (HIP events on the launch stream), for the live
+ * roofline of bench.py.  Stages: 0 bs_table, 1 contract, 2 gram, 3 jacobi,
+ * 4 vectors, 5 wmat, 6 topk, 7 scatter, 8 observables (BSMPO_N_STAGES).
+ * bsmpo_timing_read synchronises on the recorded events and returns the
+ * accumulated milliseconds, algorithmic flops and bytes and launch counts
+ * per stage (arrays of BSMPO_N_STAGES, any may be NULL); reset != 0 zeroes
+ * the accumulators.  No counterpart in the reference (which times with
+ * time.time(), src/beam_splitter_parallel/Rank.py:93-127).
+ */
+#define BSMPO_N_STAGES 9
+int bsmpo_timing_enable(int on);
+int bsmpo_timing_read(double *ms, double *flops, double *bytes, int64_t *count, int reset);
+
 /* ------------------------------------------------- beam-splitter table -- */
 
 /*
