@@ -21,6 +21,7 @@ _vp, _i, _sz, _i32p, _i64p, _dp = C.c_void_p, C.c_int, C.c_size_t, C.c_void_p, C
 SIGNATURES = {
     "bsmpo_last_error": (C.c_char_p, []),
     "bsmpo_abi_version": (C.c_int, []),
+    "bsmpo_set_eig_threshold": (C.c_int, [_i]),
     "bsmpo_bs_table": (C.c_int, [_i, _i, _dp, _dp, _vp]),
     "bsmpo_plan_query": (C.c_int, [_i, _i, _i32p, _i32p, _i32p, _i, _i, C.POINTER(C.c_size_t),
                                    C.POINTER(C.c_size_t)]),

@@ -15,7 +15,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbsmpo.so")
-SOURCES = ["plan.cpp", "capi.cpp", "update.hip", "observables.hip"]
+SOURCES = ["plan.cpp", "capi.cpp", "update.hip", "eig.hip", "observables.hip"]
 ARCH = os.environ.get("BSMPO_ARCH", "gfx950")
 
 

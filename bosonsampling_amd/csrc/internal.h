@@ -26,10 +26,14 @@ int fail(int code, const std::string &msg);
 // is copied verbatim to the front of the device workspace, so every offset
 // used by kernels is relative to that copy.
 constexpr int32_t kPlanMagic = 0x4253504c;  // "BSPL"
-// Jacobi size classes: n <= 16, 32, 64, 96 run with the Gram matrix in LDS,
-// larger n with it in global memory.
+// Eigensolver size classes of the sector Gram matrices: n <= 16, 32, 64, 96
+// run one-sided Jacobi with the matrix in LDS; the last class (n above the
+// runtime threshold, default 96) runs the tridiagonal route (Householder,
+// bisection, inverse iteration for the kept pairs only).
 constexpr int kJacClasses = 5;
 constexpr int kJacLimit[kJacClasses] = {16, 32, 64, 96, 1 << 30};
+constexpr int kTriClass = kJacClasses - 1;
+int eig_threshold();  // sectors with n > threshold take the tridiagonal route
 
 struct PlanHeader {
   int32_t magic, d, chi, n_sectors;
@@ -51,7 +55,9 @@ struct PlanHeader {
   int32_t jac_off[kJacClasses + 1];
   int32_t jac_max_n[kJacClasses];
   // workspace layout (byte offsets from the start of d_work)
-  int64_t w_C, w_T, w_G, w_V, w_W, w_S, w_lam, w_flag, w_slot, w_total;
+  int64_t w_C, w_T, w_G, w_V, w_W, w_S, w_lam, w_flag, w_slot;
+  int64_t w_td, w_te, w_tau, w_kcount, w_triwork, w_total;
+  int32_t eig_threshold;
 };
 
 struct Sector {
@@ -65,6 +71,7 @@ struct Sector {
   int64_t v_off;       // element offset of V (L x n column-major)
   int64_t w_off;       // element offset of W (n x R row-major)
   int64_t s_off;       // element offset of S / flags / slots (n)
+  int64_t tw_off;      // element offset (doubles) of the tridiagonal route's work arrays
 };
 
 struct Block {

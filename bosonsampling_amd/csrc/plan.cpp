@@ -18,6 +18,8 @@
 namespace bsmpo {
 
 static thread_local std::string g_last_error;
+static int g_eig_threshold = 96;
+int eig_threshold() { return g_eig_threshold; }
 
 void set_error(const std::string &msg) { g_last_error = msg; }
 int fail(int code, const std::string &msg) {
@@ -177,15 +179,25 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
 
   for (int sid = 0; sid < (int)b.sectors.size(); ++sid)
     for (int i = 0; i < b.sectors[sid].n; ++i) b.sv_sector.push_back(sid);
+  const int thr = eig_threshold();
+  b.h.eig_threshold = thr;
+  int64_t tw = 0;
   for (int c = 0; c < kJacClasses; ++c) {
     b.h.jac_off[c] = (int32_t)b.jac_order.size();
     b.h.jac_max_n[c] = 0;
     int lo = c == 0 ? 0 : kJacLimit[c - 1];
     for (int sid = 0; sid < (int)b.sectors.size(); ++sid) {
-      int n = b.sectors[sid].n;
-      if (n > lo && n <= kJacLimit[c]) {
+      Sector &sc = b.sectors[sid];
+      const int n = sc.n;
+      const bool tri = n > thr;
+      const bool in = c == kTriClass ? tri : (!tri && n > lo && n <= kJacLimit[c]);
+      if (in) {
         b.jac_order.push_back(sid);
         b.h.jac_max_n[c] = std::max(b.h.jac_max_n[c], n);
+        if (c == kTriClass) {
+          sc.tw_off = tw;
+          tw += (int64_t)4 * n * std::min(n, 256);  // per-thread inverse-iteration arrays
+        }
       }
     }
   }
@@ -257,6 +269,11 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
   h.w_lam = wplace(8 * h.sv_total);
   h.w_flag = wplace(4 * h.sv_total);
   h.w_slot = wplace(4 * h.sv_total);
+  h.w_td = wplace(8 * h.sv_total);
+  h.w_te = wplace(8 * h.sv_total);
+  h.w_tau = wplace(16 * h.sv_total);
+  h.w_kcount = wplace(4 * (int64_t)b.sectors.size());
+  h.w_triwork = wplace(8 * tw);
   h.w_total = w;
   return 0;
 }
@@ -285,6 +302,12 @@ extern "C" {
 const char *bsmpo_last_error(void) { return bsmpo::g_last_error.c_str(); }
 
 int bsmpo_abi_version(void) { return 1; }
+
+int bsmpo_set_eig_threshold(int n) {
+  if (n < 0 || n > 96) return fail(-1, "eig threshold must be in [0, 96]");
+  g_eig_threshold = n;
+  return 0;
+}
 
 int bsmpo_plan_query(int d, int chi, const int32_t *h_CL, const int32_t *h_CC,
                      const int32_t *h_CR, int tau0, int tau1, size_t *plan_bytes,

@@ -371,7 +371,8 @@ __global__ __launch_bounds__(256) void k_vectors(PlanHeader h, const char *work,
                                                  const double *__restrict__ T,
                                                  const double *__restrict__ Wbuf,
                                                  const double *__restrict__ Sbuf,
-                                                 double *__restrict__ Vbuf) {
+                                                 double *__restrict__ Vbuf,
+                                                 const int32_t *__restrict__ kcount) {
   const DevPlan P = dev_plan(h, work);
   const int lane = threadIdx.x & 63;
   const int i16 = lane & 15, kq = lane >> 4;
@@ -380,6 +381,7 @@ __global__ __launch_bounds__(256) void k_vectors(PlanHeader h, const char *work,
   const double2 *Yc = reinterpret_cast<const double2 *>(Wbuf);
   for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < h.n_v_items; it += nwaves) {
     const Item item = P.v_items[it];
+    if (item.c0 >= kcount[item.sector]) continue;  // only kept vectors are needed
     const Sector s = P.sectors[item.sector];
     const int i = item.r0 + i16, p = item.c0 + i16;
     const bool iv = i16 < item.nr, pv = i16 < item.nc;
@@ -411,7 +413,8 @@ __global__ __launch_bounds__(256) void k_vectors(PlanHeader h, const char *work,
 __global__ __launch_bounds__(256) void k_wmat(PlanHeader h, const char *work,
                                               const double *__restrict__ Vbuf,
                                               const double *__restrict__ C,
-                                              double *__restrict__ Wbuf) {
+                                              double *__restrict__ Wbuf,
+                                              const int32_t *__restrict__ kcount) {
   const DevPlan P = dev_plan(h, work);
   const int lane = threadIdx.x & 63;
   const int i16 = lane & 15, kq = lane >> 4;
@@ -420,6 +423,7 @@ __global__ __launch_bounds__(256) void k_wmat(PlanHeader h, const char *work,
   const double2 *Cc = reinterpret_cast<const double2 *>(C);
   for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < h.n_w_items; it += nwaves) {
     const Item item = P.w_items[it];
+    if (item.r0 >= kcount[item.sector]) continue;  // only kept rows of W are needed
     const Sector s = P.sectors[item.sector];
     const int p = item.r0 + i16, j = item.c0 + i16;
     const bool pv = i16 < item.nr, jv = i16 < item.nc;
@@ -443,6 +447,10 @@ __global__ __launch_bounds__(256) void k_wmat(PlanHeader h, const char *work,
     }
   }
 }
+
+int launch_tri_values(const PlanHeader &h, void *d_work, hipStream_t stream);
+int launch_kcount(const PlanHeader &h, void *d_work, hipStream_t stream);
+int launch_tri_vectors(const PlanHeader &h, void *d_work, hipStream_t stream);
 
 int launch_svd(const PlanHeader &h, void *d_work, hipStream_t stream) {
   char *w = reinterpret_cast<char *>(d_work);
@@ -468,33 +476,17 @@ int launch_svd(const PlanHeader &h, void *d_work, hipStream_t stream) {
     attr_set = true;
   }
   tk = timing_begin(ST_JACOBI, stream);
-  for (int c = 0; c < kJacClasses; ++c) {
+  for (int c = 0; c < kTriClass; ++c) {
     const int nsec = h.jac_off[c + 1] - h.jac_off[c];
     if (nsec == 0) continue;
-    if (kJacLimit[c] <= 96) {
-      const size_t lds = (size_t)h.jac_max_n[c] * h.jac_max_n[c] * 16;
-      hipLaunchKernelGGL(k_jacobi<true>, dim3(nsec), dim3(256), lds, stream, h,
-                         (const char *)d_work, h.jac_off[c], G, V, W, S, F, LAM, RK);
-    } else {
-      hipLaunchKernelGGL(k_jacobi<false>, dim3(nsec), dim3(256), 0, stream, h,
-                         (const char *)d_work, h.jac_off[c], G, V, W, S, F, LAM, RK);
-    }
+    const size_t lds = (size_t)h.jac_max_n[c] * h.jac_max_n[c] * 16;
+    hipLaunchKernelGGL(k_jacobi<true>, dim3(nsec), dim3(256), lds, stream, h,
+                       (const char *)d_work, h.jac_off[c], G, V, W, S, F, LAM, RK);
     BSMPO_HIP(hipGetLastError());
   }
+  int rc = launch_tri_values(h, d_work, stream);
+  if (rc) return rc;
   timing_end(tk, stream, h.flops_jacobi, 16.0 * 2.0 * h.g_elems);
-  if (h.n_v_items) {
-    tk = timing_begin(ST_VECTORS, stream);
-    hipLaunchKernelGGL(k_vectors, dim3(grid_for(h.n_v_items)), dim3(256), 0, stream, h,
-                       (const char *)d_work, (const double *)T, (const double *)W,
-                       (const double *)S, V);
-    BSMPO_HIP(hipGetLastError());
-    timing_end(tk, stream, h.flops_vectors, 0.0);
-  }
-  tk = timing_begin(ST_WMAT, stream);
-  hipLaunchKernelGGL(k_wmat, dim3(grid_for(h.n_w_items)), dim3(256), 0, stream, h,
-                     (const char *)d_work, (const double *)V, (const double *)C, W);
-  BSMPO_HIP(hipGetLastError());
-  timing_end(tk, stream, h.flops_wmat, 16.0 * (h.v_elems + h.c_elems + h.w_elems));
   return 0;
 }
 
@@ -657,6 +649,32 @@ int launch_truncate(const PlanHeader &h, void *d_work, double *d_Glc, double *d_
                      d_charge_c, d_lambda_c, d_num_lambda);
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, 0.0, 12.0 * h.sv_total);
+  int rc = launch_kcount(h, d_work, stream);
+  if (rc) return rc;
+  tk = timing_begin(ST_JACOBI, stream);
+  if ((rc = launch_tri_vectors(h, d_work, stream))) return rc;
+  timing_end(tk, stream, 0.0, 0.0);
+  {
+    double *T = reinterpret_cast<double *>(w + h.w_T);
+    double *C = reinterpret_cast<double *>(w + h.w_C);
+    double *Vm = reinterpret_cast<double *>(w + h.w_V);
+    double *Wm = reinterpret_cast<double *>(w + h.w_W);
+    double *Sm = reinterpret_cast<double *>(w + h.w_S);
+    const int32_t *KC = reinterpret_cast<const int32_t *>(w + h.w_kcount);
+  if (h.n_v_items) {
+    tk = timing_begin(ST_VECTORS, stream);
+    hipLaunchKernelGGL(k_vectors, dim3(grid_for(h.n_v_items)), dim3(256), 0, stream, h,
+                       (const char *)d_work, (const double *)T, (const double *)Wm,
+                       (const double *)Sm, Vm, KC);
+    BSMPO_HIP(hipGetLastError());
+    timing_end(tk, stream, h.flops_vectors, 0.0);
+  }
+  tk = timing_begin(ST_WMAT, stream);
+  hipLaunchKernelGGL(k_wmat, dim3(grid_for(h.n_w_items)), dim3(256), 0, stream, h,
+                     (const char *)d_work, (const double *)Vm, (const double *)C, Wm, KC);
+  BSMPO_HIP(hipGetLastError());
+  timing_end(tk, stream, h.flops_wmat, 16.0 * (h.v_elems + h.c_elems + h.w_elems));
+  }
   tk = timing_begin(ST_SCATTER, stream);
   const int chi = h.chi;
   const int wd = std::min(chi, h.d * h.d);

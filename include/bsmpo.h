@@ -36,6 +36,16 @@ const char *bsmpo_last_error(void);
 /* ABI version (increments on any signature change). */
 int bsmpo_abi_version(void);
 
+/*
+ * Sector-SVD route selection (process-wide, affects plans built afterwards):
+ * Gram matrices with n <= threshold are diagonalised by one-sided Jacobi in
+ * LDS, larger ones by the tridiagonal route (Householder reduction, Sturm
+ * bisection for all eigenvalues, inverse iteration + back-transformation for
+ * the kept pairs only).  Default 96, allowed [0, 96].  No counterpart in the
+ * reference (which calls cuSOLVER gesvd per sector, Rank.py:123).
+ */
+int bsmpo_set_eig_threshold(int n);
+
 /* Contraction arithmetic of the sector GEMMs. */
 enum bsmpo_precision {
   BSMPO_C128 = 0,    /* complex128 on FP64 MFMA                             */

@@ -15,6 +15,16 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-10
 
 
+@pytest.fixture(params=[96, 0], ids=["jacobi", "tridiag"])
+def eig_route(request):
+    """Run the test with the default eigensolver routing (n <= 96 Jacobi in
+    LDS, larger n tridiagonal) and with every sector on the tridiagonal route."""
+    from bosonsampling_amd._capi import check, lib
+    check(lib().bsmpo_set_eig_threshold(request.param), "set_eig_threshold")
+    yield request.param
+    check(lib().bsmpo_set_eig_threshold(96), "set_eig_threshold")
+
+
 def _oracle_state(M, d, chi, inputs, eta, seed, layers):
     from bosonsampling_amd import mesh
     mm, _ = mesh.haar_clements_mesh(M, seed)
@@ -87,7 +97,7 @@ def test_contract_matches_oracle(gpu, case):
 @pytest.mark.parametrize("case", [(8, 5, 16, [0, 1, 2, 3], 0.5, 0, 3, 2),
                                   (8, 5, 16, [0, 1, 2, 3], 0.5, 0, 5, 6),
                                   (10, 6, 48, [0, 2, 4, 6, 8], 0.7, 1, 5, 4)])
-def test_site_update_matches_oracle(gpu, case):
+def test_site_update_matches_oracle(gpu, eig_route, case):
     import torch
     from bosonsampling_amd import engine
     M, d, chi, inputs, eta, seed, layers, mode = case
@@ -121,7 +131,7 @@ def test_observables_match_oracle(gpu):
 
 @pytest.mark.parametrize("case", [(8, 5, 16, [0, 1, 2, 3], 0.5, 0),      # reference CPU size
                                   (10, 6, 48, [0, 2, 4, 6, 8], 0.7, 1)])
-def test_simulate_matches_oracle(gpu, case):
+def test_simulate_matches_oracle(gpu, eig_route, case):
     from bosonsampling_amd import engine, mesh
     M, d, chi, inputs, eta, seed = case
     print(f"simulate case M={M} d={d} chi={chi}", flush=True)
@@ -136,7 +146,7 @@ def test_simulate_matches_oracle(gpu, case):
     assert np.abs(marg - ref).max() <= TOL * np.abs(ref).max()
 
 
-def test_gpu_initialize_matches_oracle(gpu):
+def test_gpu_initialize_matches_oracle(gpu, eig_route):
     from bosonsampling_amd import engine
     M, d, chi, inputs, eta = 10, 6, 48, [0, 2, 4, 6, 8], 0.7
     st = mpo.initialize(M, d, chi, inputs, eta)
