@@ -38,7 +38,7 @@ SIGNATURES = {
 }
 
 STAGES = ("bs_table", "contract", "gram", "jacobi", "vectors", "wmat", "topk", "scatter",
-          "observables")
+          "observables", "tridiag", "bisect", "trivec", "backtransform")
 
 
 def timing_enable(on: bool = True):

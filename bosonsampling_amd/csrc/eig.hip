@@ -17,8 +17,8 @@
 
 namespace bsmpo {
 
-constexpr int kTriRows = 8;   // rows per thread in the reduction passes (n <= 2048)
 constexpr double kDropRel2Tri = 1e-12;
+constexpr double kClusterRel = 1e-6;
 
 __device__ double block_sum_d(double v, double *red) {
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
@@ -31,126 +31,176 @@ __device__ double block_sum_d(double v, double *red) {
   return t;
 }
 
-__global__ __launch_bounds__(256) void k_tridiag(PlanHeader h, const char *work, int cls_begin,
-                                                 double *__restrict__ Gbuf,
-                                                 double *__restrict__ TD,
-                                                 double *__restrict__ TE,
-                                                 double *__restrict__ TAU) {
+// Householder generator (LAPACK zlarfg form): H^H (alpha; x) = (beta; 0) with
+// beta real, H = I - tau v v^H, v = (1; x / (alpha - beta)).
+struct Reflector {
+  double2 tau, scale;
+  double beta;
+};
+__device__ __forceinline__ Reflector make_reflector(double2 alpha, double xn2) {
+  Reflector R;
+  R.tau = make_double2(0.0, 0.0);
+  R.scale = make_double2(0.0, 0.0);
+  R.beta = alpha.x;
+  if (!(xn2 == 0.0 && alpha.y == 0.0)) {
+    R.beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2), alpha.x);
+    R.tau = make_double2((R.beta - alpha.x) / R.beta, -alpha.y / R.beta);
+    const double2 den = make_double2(alpha.x - R.beta, alpha.y);
+    const double dd = den.x * den.x + den.y * den.y;
+    R.scale = make_double2(den.x / dd, -den.y / dd);
+  }
+  return R;
+}
+
+__device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Householder tridiagonalisation G = Q T Q^H, one block of 1024 threads per
+// sector.  Step j reduces column j; its rank-2 update
+// A22 -= v_j w_j^H + w_j v_j^H is fused with the matrix-vector product of
+// step j+1 into one read-modify-write pass over the trailing matrix.
+constexpr int kTriThreads = 1024;
+constexpr int kTriRowsPT = 2;  // rows per thread: n <= 2048
+constexpr int kTriUnroll = 8;
+
+__device__ double block_sum_big(double v, double *red) {
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
+  __syncthreads();
+  return t;
+}
+
+__global__ __launch_bounds__(kTriThreads) void k_tridiag(PlanHeader h, const char *work,
+                                                         int cls_begin, double *__restrict__ Gbuf,
+                                                         double *__restrict__ TD,
+                                                         double *__restrict__ TE,
+                                                         double *__restrict__ TAU) {
   extern __shared__ double2 tsm[];
-  __shared__ double red[8];
+  __shared__ double red[16];
   const DevPlan P = dev_plan(h, work);
   const Sector s = P.sectors[P.jac_order[cls_begin + blockIdx.x]];
   const int n = s.n;
   double2 *A = reinterpret_cast<double2 *>(Gbuf) + s.g_off;  // A(r, c) = A[c n + r]
-  double2 *v = tsm, *p = tsm + n;
+  double2 *vo = tsm, *wo = tsm + n, *vn = tsm + 2 * n;        // v_{j-1}, w_{j-1}, v_j
   double *d = TD + s.s_off, *e = TE + s.s_off;
   double2 *tau = reinterpret_cast<double2 *>(TAU) + s.s_off;
   const int t = threadIdx.x, nt = blockDim.x;
-  for (int i = 0; i + 1 < n; ++i) {
-    // Householder of x = A(i+1:n, i): H^H (alpha; x2) = (beta; 0), beta real
+  if (n == 1) {
+    if (t == 0) { d[0] = A[0].x; tau[0] = make_double2(0.0, 0.0); }
+    return;
+  }
+  for (int r = t; r < n; r += nt) { vo[r] = make_double2(0.0, 0.0); wo[r] = make_double2(0.0, 0.0); }
+  __syncthreads();
+  // j runs over columns to reduce; U_{j-1} = (vo, wo) is pending on A[j:, j:]
+  for (int j = 0; j + 1 < n; ++j) {
+    // (a) column j gets U_{j-1}: A(r, j) -= vo_r conj(wo_j) + wo_r conj(vo_j), r >= j
+    const double2 voj = vo[j], woj = wo[j];
     double xn2 = 0.0;
-    for (int r = i + 2 + t; r < n; r += nt) {
-      const double2 a = A[(int64_t)i * n + r];
-      xn2 += a.x * a.x + a.y * a.y;
+    for (int r = j + t; r < n; r += nt) {
+      double2 a = A[(int64_t)j * n + r];
+      const double2 vr = vo[r], wr = wo[r];
+      a.x -= vr.x * woj.x + vr.y * woj.y + wr.x * voj.x + wr.y * voj.y;
+      a.y -= vr.y * woj.x - vr.x * woj.y + wr.y * voj.x - wr.x * voj.y;
+      A[(int64_t)j * n + r] = a;
+      if (r >= j + 2) xn2 += a.x * a.x + a.y * a.y;
     }
-    xn2 = block_sum_d(xn2, red);
-    const double2 alpha = A[(int64_t)i * n + i + 1];
-    double2 tu = make_double2(0.0, 0.0), scale = make_double2(0.0, 0.0);
-    double beta = alpha.x;
-    if (!(xn2 == 0.0 && alpha.y == 0.0)) {
-      beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2), alpha.x);
-      tu = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
-      const double2 den = make_double2(alpha.x - beta, alpha.y);  // 1 / (alpha - beta)
-      const double dd = den.x * den.x + den.y * den.y;
-      scale = make_double2(den.x / dd, -den.y / dd);
-    }
+    xn2 = block_sum_big(xn2, red);  // also orders the column writes
+    // (b) reflector of column j
+    const double2 alpha = A[(int64_t)j * n + j + 1];
+    const Reflector R = make_reflector(alpha, xn2);
     if (t == 0) {
-      d[i] = A[(int64_t)i * n + i].x;
-      e[i] = beta;
-      tau[i] = tu;
+      d[j] = A[(int64_t)j * n + j].x;
+      e[j] = R.beta;
+      tau[j] = R.tau;
     }
-    for (int r = i + 1 + t; r < n; r += nt) {
+    for (int r = j + 1 + t; r < n; r += nt) {
       double2 vr;
-      if (r == i + 1) {
+      if (r == j + 1) {
         vr = make_double2(1.0, 0.0);
       } else {
-        const double2 a = A[(int64_t)i * n + r];
-        vr = make_double2(a.x * scale.x - a.y * scale.y, a.x * scale.y + a.y * scale.x);
-        A[(int64_t)i * n + r] = vr;  // reflector kept below the subdiagonal
+        vr = cmul2(A[(int64_t)j * n + r], R.scale);
+        A[(int64_t)j * n + r] = vr;  // reflector kept below the subdiagonal
       }
-      v[r] = vr;
+      vn[r] = vr;
     }
     __syncthreads();
-    if (tu.x == 0.0 && tu.y == 0.0) continue;  // H = I (uniform)
-    // p = tau A22 v over rows r in [i+1, n)
-    double2 acc[kTriRows];
+    // (c) fused pass over r, c >= j+1: apply U_{j-1}, accumulate p = A v_j
+    double2 acc[kTriRowsPT], vr_[kTriRowsPT], wr_[kTriRowsPT];
 #pragma unroll
-    for (int k = 0; k < kTriRows; ++k) acc[k] = make_double2(0.0, 0.0);
-    for (int c = i + 1; c < n; ++c) {
-      const double2 vc = v[c];
+    for (int k = 0; k < kTriRowsPT; ++k) {
+      const int r = j + 1 + t + k * nt;
+      acc[k] = make_double2(0.0, 0.0);
+      vr_[k] = r < n ? vo[r] : make_double2(0.0, 0.0);
+      wr_[k] = r < n ? wo[r] : make_double2(0.0, 0.0);
+    }
+    for (int c0 = j + 1; c0 < n; c0 += kTriUnroll) {
 #pragma unroll
-      for (int k = 0; k < kTriRows; ++k) {
-        const int r = i + 1 + t + k * nt;
-        if (r < n) {
-          const double2 a = A[(int64_t)c * n + r];
-          acc[k].x += a.x * vc.x - a.y * vc.y;
-          acc[k].y += a.x * vc.y + a.y * vc.x;
+      for (int k = 0; k < kTriRowsPT; ++k) {
+        const int r = j + 1 + t + k * nt;
+        if (r >= n) continue;
+        double2 a[kTriUnroll];
+#pragma unroll
+        for (int u = 0; u < kTriUnroll; ++u) {
+          const int c = c0 + u;
+          a[u] = c < n ? A[(int64_t)c * n + r] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < kTriUnroll; ++u) {
+          const int c = c0 + u;
+          if (c < n) {
+            const double2 vc = vo[c], wc = wo[c], nv = vn[c];
+            double2 x = a[u];
+            x.x -= vr_[k].x * wc.x + vr_[k].y * wc.y + wr_[k].x * vc.x + wr_[k].y * vc.y;
+            x.y -= vr_[k].y * wc.x - vr_[k].x * wc.y + wr_[k].y * vc.x - wr_[k].x * vc.y;
+            A[(int64_t)c * n + r] = x;
+            acc[k].x += x.x * nv.x - x.y * nv.y;
+            acc[k].y += x.x * nv.y + x.y * nv.x;
+          }
         }
       }
     }
-    double pr = 0.0, pi = 0.0;  // p^H v, with p = tau A v
+    // (d) p = tau A v ; w = p - 1/2 tau (p^H v) v
+    double pr = 0.0, pi = 0.0;
 #pragma unroll
-    for (int k = 0; k < kTriRows; ++k) {
-      const int r = i + 1 + t + k * nt;
+    for (int k = 0; k < kTriRowsPT; ++k) {
+      const int r = j + 1 + t + k * nt;
       if (r < n) {
-        const double2 q = make_double2(tu.x * acc[k].x - tu.y * acc[k].y,
-                                       tu.x * acc[k].y + tu.y * acc[k].x);
-        acc[k] = q;
-        const double2 vr = v[r];
-        pr += q.x * vr.x + q.y * vr.y;   // conj(q) v
-        pi += q.x * vr.y - q.y * vr.x;
+        acc[k] = cmul2(R.tau, acc[k]);
+        const double2 v = vn[r];
+        pr += acc[k].x * v.x + acc[k].y * v.y;
+        pi += acc[k].x * v.y - acc[k].y * v.x;
       }
     }
-    pr = block_sum_d(pr, red);
-    pi = block_sum_d(pi, red);
-    // alpha2 = -1/2 tau (p^H v);  w = p + alpha2 v  (stored in p)
-    const double2 a2 = make_double2(-0.5 * (tu.x * pr - tu.y * pi), -0.5 * (tu.x * pi + tu.y * pr));
+    pr = block_sum_big(pr, red);
+    pi = block_sum_big(pi, red);
+    const double2 a2 = make_double2(-0.5 * (R.tau.x * pr - R.tau.y * pi),
+                                    -0.5 * (R.tau.x * pi + R.tau.y * pr));
 #pragma unroll
-    for (int k = 0; k < kTriRows; ++k) {
-      const int r = i + 1 + t + k * nt;
+    for (int k = 0; k < kTriRowsPT; ++k) {
+      const int r = j + 1 + t + k * nt;
       if (r < n) {
-        const double2 vr = v[r];
-        p[r] = make_double2(acc[k].x + a2.x * vr.x - a2.y * vr.y, acc[k].y + a2.x * vr.y + a2.y * vr.x);
+        const double2 v = vn[r];
+        const double2 q = cmul2(a2, v);
+        wo[r] = make_double2(acc[k].x + q.x, acc[k].y + q.y);
+        vo[r] = v;
       }
     }
-    __syncthreads();
-    // A22 -= v w^H + w v^H
-    double2 vr_[kTriRows], wr_[kTriRows];
-#pragma unroll
-    for (int k = 0; k < kTriRows; ++k) {
-      const int r = i + 1 + t + k * nt;
-      vr_[k] = r < n ? v[r] : make_double2(0.0, 0.0);
-      wr_[k] = r < n ? p[r] : make_double2(0.0, 0.0);
-    }
-    for (int c = i + 1; c < n; ++c) {
-      const double2 vc = v[c], wc = p[c];
-#pragma unroll
-      for (int k = 0; k < kTriRows; ++k) {
-        const int r = i + 1 + t + k * nt;
-        if (r < n) {
-          double2 a = A[(int64_t)c * n + r];
-          // v_r conj(w_c) + w_r conj(v_c)
-          a.x -= vr_[k].x * wc.x + vr_[k].y * wc.y + wr_[k].x * vc.x + wr_[k].y * vc.y;
-          a.y -= vr_[k].y * wc.x - vr_[k].x * wc.y + wr_[k].y * vc.x - wr_[k].x * vc.y;
-          A[(int64_t)c * n + r] = a;
-        }
-      }
-    }
+    if (t == 0) { vo[j] = make_double2(0.0, 0.0); wo[j] = make_double2(0.0, 0.0); }
     __syncthreads();
   }
   if (t == 0) {
-    d[n - 1] = A[(int64_t)(n - 1) * n + n - 1].x;
-    tau[n - 1] = make_double2(0.0, 0.0);
+    // last diagonal element gets the final pending update
+    const int r = n - 1;
+    double2 a = A[(int64_t)r * n + r];
+    const double2 vr = vo[r], wr = wo[r];
+    a.x -= 2.0 * (vr.x * wr.x + vr.y * wr.y);
+    d[r] = a.x;
+    tau[r] = make_double2(0.0, 0.0);
   }
 }
 
@@ -233,40 +283,41 @@ __global__ void k_kcount(PlanHeader h, const char *work, const int32_t *__restri
   }
 }
 
-// Solve (T - lam I) x = b in place (b -> x), partial pivoting (as LAPACK dgtsv).
-// dd, du, dl: work arrays of length n (dl also holds the fill-in).
-__device__ void tri_solve(const double *d, const double *e, int n, double lam, double tiny,
-                          double *dd, double *du, double *dl, double *b) {
-  for (int i = 0; i < n; ++i) dd[i] = d[i] - lam;
-  for (int i = 0; i + 1 < n; ++i) { du[i] = e[i]; dl[i] = e[i]; }
+// Solve (T - lam I) x = b in place (b -> x), partial pivoting (as LAPACK
+// dgtsv).  The elimination carries the running row in registers and only
+// stores what the back-substitution needs (pivot row: dd, du, fill-in dl).
+__device__ void tri_solve(const double *__restrict__ d, const double *__restrict__ e, int n,
+                          double lam, double tiny, double *__restrict__ dd,
+                          double *__restrict__ du, double *__restrict__ dl,
+                          double *__restrict__ b) {
+  // running (not yet eliminated) row i: [.., cd, cu] with rhs cb
+  double cd = d[0] - lam, cu = n > 1 ? e[0] : 0.0, cb = b[0];
   for (int i = 0; i + 1 < n; ++i) {
-    if (fabs(dd[i]) >= fabs(dl[i])) {
-      if (dd[i] == 0.0) dd[i] = tiny;
-      const double f = dl[i] / dd[i];
-      dd[i + 1] -= f * du[i];
-      b[i + 1] -= f * b[i];
-      dl[i] = 0.0;
+    const double sub = e[i];                          // T(i+1, i)
+    const double nd = d[i + 1] - lam;                 // T(i+1, i+1)
+    const double nu = i + 2 < n ? e[i + 1] : 0.0;     // T(i+1, i+2)
+    const double nb = b[i + 1];
+    if (fabs(cd) >= fabs(sub)) {
+      if (cd == 0.0) cd = tiny;
+      const double f = sub / cd;
+      dd[i] = cd; du[i] = cu; dl[i] = 0.0; b[i] = cb;
+      cd = nd - f * cu; cu = nu; cb = nb - f * cb;
     } else {
-      const double f = dd[i] / dl[i];
-      dd[i] = dl[i];
-      const double tmp = dd[i + 1];
-      dd[i + 1] = du[i] - f * tmp;
-      if (i + 2 < n) {
-        dl[i] = du[i + 1];
-        du[i + 1] = -f * dl[i];
-      } else {
-        dl[i] = 0.0;
-      }
-      du[i] = tmp;
-      const double bt = b[i];
-      b[i] = b[i + 1];
-      b[i + 1] = bt - f * b[i + 1];
+      const double f = cd / sub;
+      dd[i] = sub; du[i] = nd; dl[i] = nu; b[i] = nb;   // swapped: row i+1 becomes pivot
+      cd = cu - f * nd; cu = -f * nu; cb = cb - f * nb;
     }
   }
-  if (dd[n - 1] == 0.0) dd[n - 1] = tiny;
-  b[n - 1] /= dd[n - 1];
-  if (n >= 2) b[n - 2] = (b[n - 2] - du[n - 2] * b[n - 1]) / dd[n - 2];
-  for (int i = n - 3; i >= 0; --i) b[i] = (b[i] - du[i] * b[i + 1] - dl[i] * b[i + 2]) / dd[i];
+  if (cd == 0.0) cd = tiny;
+  dd[n - 1] = cd;
+  double x1 = cb / cd, x2 = 0.0;
+  b[n - 1] = x1;
+  for (int i = n - 2; i >= 0; --i) {
+    const double x = (b[i] - du[i] * x1 - dl[i] * x2) / dd[i];
+    b[i] = x;
+    x2 = x1;
+    x1 = x;
+  }
 }
 
 // One block per large sector: eigenvectors of the kept eigenvalues j < k,
@@ -277,7 +328,6 @@ __global__ __launch_bounds__(256) void k_tri_vectors(
     const double *__restrict__ TD, const double *__restrict__ TE, const double *__restrict__ TAU,
     const double *__restrict__ lam, const int32_t *__restrict__ kcount,
     double *__restrict__ Vbuf, double *__restrict__ Wbuf, double *__restrict__ triwork) {
-  extern __shared__ double2 vsm[];  // reflector cache (n)
   __shared__ int cstart[2048 + 1];
   __shared__ int ncl;
   const DevPlan P = dev_plan(h, work);
@@ -288,12 +338,15 @@ __global__ __launch_bounds__(256) void k_tri_vectors(
   const double *d = TD + s.s_off, *e = TE + s.s_off, *l = lam + s.s_off;
   double2 *X = s.left_small ? reinterpret_cast<double2 *>(Vbuf) + s.v_off
                             : reinterpret_cast<double2 *>(Wbuf) + s.w_off;  // column j: X[j n + r]
-  // clusters of close eigenvalues (gap <= 1e-3 |T|), as LAPACK's stein
+  // clusters of close eigenvalues (gap <= kClusterRel |T|) are orthogonalised
+  // explicitly; outside them inverse iteration loses at most
+  // eps |T| / gap <= 2e-10 orthogonality (LAPACK's stein uses 1e-3 |T|, which
+  // would chain most kept eigenvalues of a sector into one serial cluster)
   if (threadIdx.x == 0) {
     double tn = 0.0;
     for (int i = 0; i < n; ++i)
       tn = fmax(tn, fabs(d[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0));
-    const double ortol = 1e-3 * tn;
+    const double ortol = kClusterRel * tn;
     int c = 0;
     for (int j = 0; j < k; ++j)
       if (j == 0 || fabs(l[j - 1] - l[j]) > ortol) cstart[c++] = j;
@@ -304,7 +357,7 @@ __global__ __launch_bounds__(256) void k_tri_vectors(
   double tn1 = 0.0;
   for (int i = 0; i < n; ++i) tn1 = fmax(tn1, fabs(d[i]) + (i + 1 < n ? 2.0 * fabs(e[i]) : 0.0));
   const double tiny = DBL_EPSILON * fmax(tn1, DBL_MIN);
-  double *wk = triwork + s.tw_off + (int64_t)threadIdx.x * 4 * n;
+  double *wk = triwork + s.tw_off + (int64_t)threadIdx.x * 4 * n;  // < min(n, 1024) threads used
   double *dd = wk, *du = wk + n, *dl = wk + 2 * n, *b = wk + 3 * n;
   for (int c = threadIdx.x; c < ncl; c += blockDim.x) {
     for (int j = cstart[c]; j < cstart[c + 1]; ++j) {
@@ -324,39 +377,85 @@ __global__ __launch_bounds__(256) void k_tri_vectors(
       for (int i = 0; i < n; ++i) X[(int64_t)j * n + i] = make_double2(b[i], 0.0);
     }
   }
-  __syncthreads();
-  // x = H_0 H_1 ... H_{n-2} y : apply reflectors from the last to the first
+}
+
+// x = H_0 H_1 ... H_{n-2} y for the kept eigenvectors of every large sector
+template <int kBtRows>
+__global__ __launch_bounds__(kBtRows == 16 ? 1024 : 256) void k_tri_backtransform(
+    PlanHeader h, const char *work, int cls_begin, const double *__restrict__ Gbuf,
+    const double *__restrict__ TAU, const int32_t *__restrict__ kcount,
+    double *__restrict__ Vbuf, double *__restrict__ Wbuf) {
+  extern __shared__ double2 vsm[];  // reflector cache (n)
+  const DevPlan P = dev_plan(h, work);
+  const int sid = P.jac_order[cls_begin + blockIdx.x];
+  const Sector s = P.sectors[sid];
+  const int n = s.n, k = kcount[sid];
+  if (k == 0) return;
+  double2 *X = s.left_small ? reinterpret_cast<double2 *>(Vbuf) + s.v_off
+                            : reinterpret_cast<double2 *>(Wbuf) + s.w_off;
   const double2 *A = reinterpret_cast<const double2 *>(Gbuf) + s.g_off;
   const double2 *tau = reinterpret_cast<const double2 *>(TAU) + s.s_off;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int i = n - 2; i >= 0; --i) {
-    const double2 ti = tau[i];
-    if (ti.x == 0.0 && ti.y == 0.0) continue;
-    for (int r = i + 1 + threadIdx.x; r < n; r += blockDim.x)
-      vsm[r] = r == i + 1 ? make_double2(1.0, 0.0) : A[(int64_t)i * n + r];
-    __syncthreads();
-    for (int j = wave; j < k; j += nw) {
-      double2 *x = X + (int64_t)j * n;
-      double sr = 0.0, si = 0.0;  // v^H x
-      for (int r = i + 1 + lane; r < n; r += 64) {
-        const double2 a = vsm[r], b2 = x[r];
-        sr += a.x * b2.x + a.y * b2.y;
-        si += a.x * b2.y - a.y * b2.x;
+  constexpr int kBtVec = 1;  // n <= 64 * kBtRows
+  for (int j0 = 0; j0 < k; j0 += nw * kBtVec) {
+    double2 x[kBtVec][kBtRows];
+    const int jb = j0 + wave * kBtVec;
+#pragma unroll
+    for (int u = 0; u < kBtVec; ++u)
+#pragma unroll
+      for (int q = 0; q < kBtRows; ++q) {
+        const int r = lane + 64 * q;
+        x[u][q] = (jb + u < k && r < n) ? X[(int64_t)(jb + u) * n + r] : make_double2(0.0, 0.0);
       }
-      for (int o = 32; o >= 1; o >>= 1) {
-        sr += __shfl_xor(sr, o);
-        si += __shfl_xor(si, o);
+    for (int i = n - 2; i >= 0; --i) {
+      const double2 ti = tau[i];
+      if (ti.x == 0.0 && ti.y == 0.0) continue;  // uniform
+      __syncthreads();
+      for (int r = i + 1 + threadIdx.x; r < n; r += blockDim.x)
+        vsm[r] = r == i + 1 ? make_double2(1.0, 0.0) : A[(int64_t)i * n + r];
+      __syncthreads();
+      double sr[kBtVec], si[kBtVec];
+#pragma unroll
+      for (int u = 0; u < kBtVec; ++u) { sr[u] = 0.0; si[u] = 0.0; }
+#pragma unroll
+      for (int q = 0; q < kBtRows; ++q) {
+        const int r = lane + 64 * q;
+        if (r > i && r < n) {
+          const double2 a = vsm[r];
+#pragma unroll
+          for (int u = 0; u < kBtVec; ++u) {
+            sr[u] += a.x * x[u][q].x + a.y * x[u][q].y;
+            si[u] += a.x * x[u][q].y - a.y * x[u][q].x;
+          }
+        }
       }
-      const double2 f = make_double2(ti.x * sr - ti.y * si, ti.x * si + ti.y * sr);  // tau (v^H x)
-      for (int r = i + 1 + lane; r < n; r += 64) {
-        const double2 a = vsm[r];
-        double2 b2 = x[r];
-        b2.x -= a.x * f.x - a.y * f.y;
-        b2.y -= a.x * f.y + a.y * f.x;
-        x[r] = b2;
+#pragma unroll
+      for (int u = 0; u < kBtVec; ++u)
+        for (int o = 32; o >= 1; o >>= 1) {
+          sr[u] += __shfl_xor(sr[u], o);
+          si[u] += __shfl_xor(si[u], o);
+        }
+#pragma unroll
+      for (int q = 0; q < kBtRows; ++q) {
+        const int r = lane + 64 * q;
+        if (r > i && r < n) {
+          const double2 a = vsm[r];
+#pragma unroll
+          for (int u = 0; u < kBtVec; ++u) {
+            const double2 f = make_double2(ti.x * sr[u] - ti.y * si[u], ti.x * si[u] + ti.y * sr[u]);
+            x[u][q].x -= a.x * f.x - a.y * f.y;
+            x[u][q].y -= a.x * f.y + a.y * f.x;
+          }
+        }
       }
     }
-    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kBtVec; ++u)
+#pragma unroll
+      for (int q = 0; q < kBtRows; ++q) {
+        const int r = lane + 64 * q;
+        if (jb + u < k && r < n) X[(int64_t)(jb + u) * n + r] = x[u][q];
+      }
   }
 }
 
@@ -364,16 +463,26 @@ int launch_tri_values(const PlanHeader &h, void *d_work, hipStream_t stream) {
   const int c = kTriClass;
   const int nsec = h.jac_off[c + 1] - h.jac_off[c];
   if (nsec == 0) return 0;
-  if (h.jac_max_n[c] > 256 * kTriRows) return fail(-1, "sector too large for the tridiagonal route");
+  if (h.jac_max_n[c] > kTriThreads * kTriRowsPT)
+    return fail(-1, "sector too large for the tridiagonal route (n > 2048)");
   char *w = reinterpret_cast<char *>(d_work);
   double *G = reinterpret_cast<double *>(w + h.w_G);
   double *TD = reinterpret_cast<double *>(w + h.w_td);
   double *TE = reinterpret_cast<double *>(w + h.w_te);
   double *TAU = reinterpret_cast<double *>(w + h.w_tau);
-  const size_t lds = (size_t)2 * h.jac_max_n[c] * 16;
-  hipLaunchKernelGGL(k_tridiag, dim3(nsec), dim3(256), lds, stream, h, (const char *)d_work,
-                     h.jac_off[c], G, TD, TE, TAU);
+  const size_t lds = (size_t)3 * h.jac_max_n[c] * 16;
+  static bool attr = false;
+  if (!attr) {
+    BSMPO_HIP(hipFuncSetAttribute((const void *)k_tridiag,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 3 * 2048 * 16));
+    attr = true;
+  }
+  int tk = timing_begin(ST_TRIDIAG, stream);
+  hipLaunchKernelGGL(k_tridiag, dim3(nsec), dim3(kTriThreads), lds, stream, h,
+                     (const char *)d_work, h.jac_off[c], G, TD, TE, TAU);
   BSMPO_HIP(hipGetLastError());
+  timing_end(tk, stream, 0.0, 0.0);
+  tk = timing_begin(ST_BISECT, stream);
   double *LAM = reinterpret_cast<double *>(w + h.w_lam);
   double *S = reinterpret_cast<double *>(w + h.w_S);
   int32_t *F = reinterpret_cast<int32_t *>(w + h.w_flag);
@@ -381,6 +490,7 @@ int launch_tri_values(const PlanHeader &h, void *d_work, hipStream_t stream) {
                      (const char *)d_work, h.jac_off[c], (const double *)TD, (const double *)TE,
                      LAM, S, F);
   BSMPO_HIP(hipGetLastError());
+  timing_end(tk, stream, 0.0, 0.0);
   return 0;
 }
 
@@ -398,13 +508,28 @@ int launch_tri_vectors(const PlanHeader &h, void *d_work, hipStream_t stream) {
   const int nsec = h.jac_off[c + 1] - h.jac_off[c];
   if (nsec == 0) return 0;
   char *w = reinterpret_cast<char *>(d_work);
-  const size_t lds = (size_t)h.jac_max_n[c] * 16;
-  hipLaunchKernelGGL(k_tri_vectors, dim3(nsec), dim3(256), lds, stream, h, (const char *)d_work,
+  int tk = timing_begin(ST_TRIVEC, stream);
+  hipLaunchKernelGGL(k_tri_vectors, dim3(nsec), dim3(256), 0, stream, h, (const char *)d_work,
                      h.jac_off[c], (const double *)(w + h.w_G), (const double *)(w + h.w_td),
                      (const double *)(w + h.w_te), (const double *)(w + h.w_tau),
                      (const double *)(w + h.w_lam), (const int32_t *)(w + h.w_kcount),
                      (double *)(w + h.w_V), (double *)(w + h.w_W), (double *)(w + h.w_triwork));
   BSMPO_HIP(hipGetLastError());
+  timing_end(tk, stream, 0.0, 0.0);
+  tk = timing_begin(ST_BACKTRANS, stream);
+  const size_t lds = (size_t)h.jac_max_n[c] * 16;
+  if (h.jac_max_n[c] <= 1024)
+    hipLaunchKernelGGL(k_tri_backtransform<16>, dim3(nsec), dim3(1024), lds, stream, h,
+                       (const char *)d_work, h.jac_off[c], (const double *)(w + h.w_G),
+                       (const double *)(w + h.w_tau), (const int32_t *)(w + h.w_kcount),
+                       (double *)(w + h.w_V), (double *)(w + h.w_W));
+  else
+    hipLaunchKernelGGL(k_tri_backtransform<32>, dim3(nsec), dim3(256), lds, stream, h,
+                       (const char *)d_work, h.jac_off[c], (const double *)(w + h.w_G),
+                       (const double *)(w + h.w_tau), (const int32_t *)(w + h.w_kcount),
+                       (double *)(w + h.w_V), (double *)(w + h.w_W));
+  BSMPO_HIP(hipGetLastError());
+  timing_end(tk, stream, 0.0, 0.0);
   return 0;
 }
 

@@ -109,7 +109,7 @@ int validate_plan(const void *plan, int d, int chi);
 // (bsmpo_timing_enable / bsmpo_timing_read).  Off by default: no events.
 enum Stage {
   ST_BSTABLE = 0, ST_CONTRACT, ST_GRAM, ST_JACOBI, ST_VECTORS, ST_WMAT, ST_TOPK,
-  ST_SCATTER, ST_OBSERVABLES, ST_COUNT
+  ST_SCATTER, ST_OBSERVABLES, ST_TRIDIAG, ST_BISECT, ST_TRIVEC, ST_BACKTRANS, ST_COUNT
 };
 bool timing_on();
 // begin returns a token (or -1 when timing is off); end closes it.

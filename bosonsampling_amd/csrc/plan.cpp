@@ -196,7 +196,7 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
         b.h.jac_max_n[c] = std::max(b.h.jac_max_n[c], n);
         if (c == kTriClass) {
           sc.tw_off = tw;
-          tw += (int64_t)4 * n * std::min(n, 256);  // per-thread inverse-iteration arrays
+          tw += (int64_t)4 * n * std::min(n, 1024);  // per-thread inverse-iteration arrays
         }
       }
     }

@@ -269,6 +269,7 @@ __global__ __launch_bounds__(256) void k_jacobi(PlanHeader h, const char *work, 
   const int group = threadIdx.x >> 3, gl = threadIdx.x & 7;
   const int ngroups = blockDim.x >> 3;
   const int n2 = n + (n & 1);
+  const double tol = fmax(kJacTol, (double)n * 2.220446049250313e-16);
   for (int sweep = 0; sweep < kJacSweeps && n > 1; ++sweep) {
     if (threadIdx.x == 0) any_rot = 0;
     __syncthreads();
@@ -293,7 +294,7 @@ __global__ __launch_bounds__(256) void k_jacobi(PlanHeader h, const char *work, 
           gi += __shfl_xor(gi, o);
         }
         const double ga = sqrt(gr * gr + gi * gi);
-        if (ga > kJacTol * sqrt(al * be) && ga > 0.0) {
+        if (ga > tol * sqrt(al * be) && ga > 0.0) {
           const double zeta = (be - al) / (2.0 * ga);
           const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
           const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
@@ -476,17 +477,21 @@ int launch_svd(const PlanHeader &h, void *d_work, hipStream_t stream) {
     attr_set = true;
   }
   tk = timing_begin(ST_JACOBI, stream);
-  for (int c = 0; c < kTriClass; ++c) {
-    const int nsec = h.jac_off[c + 1] - h.jac_off[c];
-    if (nsec == 0) continue;
-    const size_t lds = (size_t)h.jac_max_n[c] * h.jac_max_n[c] * 16;
-    hipLaunchKernelGGL(k_jacobi<true>, dim3(nsec), dim3(256), lds, stream, h,
-                       (const char *)d_work, h.jac_off[c], G, V, W, S, F, LAM, RK);
-    BSMPO_HIP(hipGetLastError());
+  {
+    // all LDS classes in one launch (the sectors are contiguous in jac_order)
+    const int nsec = h.jac_off[kTriClass] - h.jac_off[0];
+    int nmax = 0;
+    for (int c = 0; c < kTriClass; ++c) nmax = std::max(nmax, h.jac_max_n[c]);
+    if (nsec > 0) {
+      const size_t lds = (size_t)nmax * nmax * 16;
+      hipLaunchKernelGGL(k_jacobi<true>, dim3(nsec), dim3(256), lds, stream, h,
+                         (const char *)d_work, h.jac_off[0], G, V, W, S, F, LAM, RK);
+      BSMPO_HIP(hipGetLastError());
+    }
   }
+  timing_end(tk, stream, h.flops_jacobi, 16.0 * 2.0 * h.g_elems);
   int rc = launch_tri_values(h, d_work, stream);
   if (rc) return rc;
-  timing_end(tk, stream, h.flops_jacobi, 16.0 * 2.0 * h.g_elems);
   return 0;
 }
 
@@ -504,6 +509,9 @@ __device__ int block_sum_int(int v, int *scratch) {
   __syncthreads();
   return t;
 }
+
+// relative width of the tie band around the chi-th singular value
+constexpr double kTieRel = 1e-10;
 
 // exclusive prefix sum over a 256-thread block (Hillis-Steele in LDS)
 __device__ int block_scan_excl(int v, int *sc, int &total) {
@@ -548,18 +556,22 @@ __global__ __launch_bounds__(256) void k_topk(PlanHeader h, const char *work,
       if (cnt >= K) t = cand;
     }
   }
+  // ties: values within kTieRel of the K-th largest t, broken by the lower
+  // concatenated index (degenerate twins of Hermitian-conjugate sectors)
+  const double tv = __longlong_as_double((long long)t);
+  const double hi = tv * (1.0 + kTieRel), lo = tv * (1.0 - kTieRel);
   int gt = 0;
-  for (int64_t i = threadIdx.x; i < nt; i += blockDim.x) gt += (flags[i] && key[i] > t) ? 1 : 0;
+  for (int64_t i = threadIdx.x; i < nt; i += blockDim.x) gt += (flags[i] && S[i] > hi) ? 1 : 0;
   gt = block_sum_int(gt, scratch);
   const int need_eq = K - gt;
-  // scan in index order: kept = key > t, or key == t among the first need_eq
+  // scan in index order: kept = value > hi, or lo <= value <= hi among the first need_eq
   __shared__ int sc[256];
   int carry_eq = 0, carry_kept = 0;
   for (int64_t base = 0; base < nt; base += blockDim.x) {
     const int64_t i = base + threadIdx.x;
     const bool v = i < nt && flags[i] && K > 0;
-    const bool gtf = v && key[i] > t;
-    const bool eqf = v && key[i] == t;
+    const bool gtf = v && S[i] > hi;
+    const bool eqf = v && !gtf && S[i] >= lo;
     int tot_eq;
     const int eq_rank = carry_eq + block_scan_excl(eqf ? 1 : 0, sc, tot_eq);
     const bool kept = gtf || (eqf && eq_rank < need_eq);
@@ -651,9 +663,7 @@ int launch_truncate(const PlanHeader &h, void *d_work, double *d_Glc, double *d_
   timing_end(tk, stream, 0.0, 12.0 * h.sv_total);
   int rc = launch_kcount(h, d_work, stream);
   if (rc) return rc;
-  tk = timing_begin(ST_JACOBI, stream);
   if ((rc = launch_tri_vectors(h, d_work, stream))) return rc;
-  timing_end(tk, stream, 0.0, 0.0);
   {
     double *T = reinterpret_cast<double *>(w + h.w_T);
     double *C = reinterpret_cast<double *>(w + h.w_C);

@@ -58,14 +58,15 @@ enum bsmpo_precision {
 /*
  * Per-stage device timing (HIP events on the launch stream), for the live
  * roofline of bench.py.  Stages: 0 bs_table, 1 contract, 2 gram, 3 jacobi,
- * 4 vectors, 5 wmat, 6 topk, 7 scatter, 8 observables (BSMPO_N_STAGES).
+ * 4 vectors, 5 wmat, 6 topk, 7 scatter, 8 observables, 9 tridiag,
+ * 10 bisect, 11 trivec, 12 backtransform (BSMPO_N_STAGES).
  * bsmpo_timing_read synchronises on the recorded events and returns the
  * accumulated milliseconds, algorithmic flops and bytes and launch counts
  * per stage (arrays of BSMPO_N_STAGES, any may be NULL); reset != 0 zeroes
  * the accumulators.  No counterpart in the reference (which times with
  * time.time(), src/beam_splitter_parallel/Rank.py:93-127).
  */
-#define BSMPO_N_STAGES 9
+#define BSMPO_N_STAGES 13
 int bsmpo_timing_enable(int on);
 int bsmpo_timing_read(double *ms, double *flops, double *bytes, int64_t *count, int reset);
 

@@ -176,22 +176,39 @@ def contract_sector_literal(d, U3, CL, CC, CR, Glc, Gcr, tau):
     return np.array(rows), np.array(cols), C
 
 
-def select_top(svals_per_sector, chi):
+# Values within TIE_REL (relative) of the chi-th largest singular value are
+# ties: exactly degenerate twins are the rule here, because the density
+# matrix is Hermitian and sectors (a, b) and (b, a) have equal singular
+# values; which twin survives a cut between them is decided by rounding
+# noise in any implementation (argpartition on float32 in the reference,
+# Rank.py:142).  Ties are broken by the lower concatenated index, on the CPU
+# and on the GPU alike.
+TIE_REL = 1e-10
+
+
+def select_top(svals_per_sector, chi, tie_rel=TIE_REL):
     """Global top-chi over all sectors' singular values (Rank.py:139-145).
 
-    Deterministic stand-in for argpartition: sort by (-sigma, concat index),
-    keep num = min(total, chi), then order the kept ones by (sector order,
-    position in sector) -- the stable lexsort by charge of Rank.py:182-196
-    with a defined within-sector order.  Returns list of (sector, position).
+    Deterministic stand-in for argpartition: keep num = min(total, chi)
+    values: every value above the chi-th largest t by more than tie_rel * t,
+    then the ties (|value - t| <= tie_rel * t) in concatenated order.  The
+    kept ones are ordered by (sector order, position in sector) -- the stable
+    lexsort by charge of Rank.py:182-196 with a defined within-sector order.
+    Returns a list of (sector, position).
     """
     flat = []
     for s, S in enumerate(svals_per_sector):
         for p, v in enumerate(S):
             flat.append((float(v), s, p))
     num = min(len(flat), chi)
-    order = sorted(range(len(flat)), key=lambda i: (-flat[i][0], i))[:num]
-    kept = sorted((flat[i][1], flat[i][2]) for i in order)
-    return kept
+    if num == 0:
+        return []
+    t = sorted((f[0] for f in flat), reverse=True)[num - 1]
+    hi, lo = t * (1 + tie_rel), t * (1 - tie_rel)
+    above = [i for i, f in enumerate(flat) if f[0] > hi]
+    band = [i for i, f in enumerate(flat) if lo <= f[0] <= hi]
+    chosen = above + band[:num - len(above)]
+    return sorted((flat[i][1], flat[i][2]) for i in chosen)
 
 
 def _svd_left(T):

@@ -221,5 +221,12 @@ def test_truncation_keeps_top_chi_without_renormalisation():
 def test_select_top_ties_and_order():
     kept = mpo.select_top([np.array([3.0, 1.0]), np.array([2.0, 1.0, 0.5])], 3)
     assert kept == [(0, 0), (0, 1), (1, 0)]   # tie at 1.0: lower concat index wins
+    # twins split by rounding noise are still ties: the lower index wins
+    for eps in (1e-15, -1e-15, 3e-13):
+        kept = mpo.select_top([np.array([3.0]), np.array([1.0]), np.array([1.0 * (1 + eps)])], 2)
+        assert kept == [(0, 0), (1, 0)]
+    # genuinely different values are not ties
+    kept = mpo.select_top([np.array([3.0]), np.array([1.0]), np.array([1.0 + 1e-6])], 2)
+    assert kept == [(0, 0), (2, 0)]
     kept = mpo.select_top([np.array([3.0]), np.array([5.0])], 10)
     assert kept == [(0, 0), (1, 0)]
