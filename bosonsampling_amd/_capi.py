@@ -22,6 +22,7 @@ SIGNATURES = {
     "bsmpo_last_error": (C.c_char_p, []),
     "bsmpo_abi_version": (C.c_int, []),
     "bsmpo_set_eig_threshold": (C.c_int, [_i]),
+    "bsmpo_set_tie_rel": (C.c_int, [C.c_double]),
     "bsmpo_bs_table": (C.c_int, [_i, _i, _dp, _dp, _vp]),
     "bsmpo_plan_query": (C.c_int, [_i, _i, _i32p, _i32p, _i32p, _i, _i, C.POINTER(C.c_size_t),
                                    C.POINTER(C.c_size_t)]),
@@ -33,6 +34,9 @@ SIGNATURES = {
     "bsmpo_contract": (C.c_int, [_i, _i, _vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i, _vp,
                                  _sz, _vp]),
     "bsmpo_observables": (C.c_int, [_i, _i, _i, _dp, _dp, _i32p, _dp, _dp, _dp, _dp, _vp]),
+    "bsmpo_group_query": (C.c_int, [_i, _vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "bsmpo_group_build": (C.c_int, [_i, _vp, _vp, _vp, _vp, _sz]),
+    "bsmpo_group_run": (C.c_int, [_vp, _i, _vp, _sz, _vp]),
     "bsmpo_timing_enable": (C.c_int, [_i]),
     "bsmpo_timing_read": (C.c_int, [_dp, _dp, _dp, _i64p, _i]),
 }
@@ -60,6 +64,12 @@ class PlanInfo(C.Structure):
     _fields_ = [("n_sectors", C.c_int), ("n_items", C.c_int), ("c_elems", C.c_int64),
                 ("g_elems", C.c_int64), ("sv_total", C.c_int64), ("flops_alg", C.c_double),
                 ("flops_tau", C.c_double), ("bytes_alg16", C.c_double)]
+
+
+class UpdateIO(C.Structure):
+    _fields_ = [("Glc", C.c_void_p), ("Gcr", C.c_void_p), ("LR", C.c_void_p), ("U3", C.c_void_p),
+                ("charge_c", C.c_void_p), ("lambda_c", C.c_void_p), ("num_lambda", C.c_void_p),
+                ("edge", C.c_int32), ("reserved", C.c_int32)]
 
 
 class BsmpoError(RuntimeError):

@@ -15,7 +15,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbsmpo.so")
-SOURCES = ["plan.cpp", "capi.cpp", "update.hip", "eig.hip", "observables.hip"]
+SOURCES = ["plan.cpp", "group.cpp", "capi.cpp", "update.hip", "contract_lowp.hip", "eig.hip",
+           "observables.hip"]
 ARCH = os.environ.get("BSMPO_ARCH", "gfx950")
 
 

@@ -1,7 +1,9 @@
 // extern "C" entry points of libbsmpo (declared in include/bsmpo.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "internal.h"
@@ -55,14 +57,37 @@ using namespace bsmpo;
 
 namespace {
 
-int upload_plan(const void *h_plan, void *d_work, size_t work_bytes, hipStream_t stream) {
-  const PlanHeader *h = header(h_plan);
+int run_group(const void *blob, int precision, void *d_work, size_t work_bytes, hipStream_t st,
+              bool contract_only) {
+  const GroupHeader *g = group_header(blob);
+  if (!g || g->magic != kGroupMagic) return fail(-1, "not a bsmpo group");
   if (!d_work) return fail(-1, "workspace is NULL");
-  if ((int64_t)work_bytes < h->w_total) return fail(-1, "workspace too small for this plan");
-  if (h->dev_bytes > 0)
-    BSMPO_HIP(hipMemcpyAsync(d_work, reinterpret_cast<const char *>(h_plan) + h->dev_begin,
-                             (size_t)h->dev_bytes, hipMemcpyHostToDevice, stream));
-  return 0;
+  if ((int64_t)work_bytes < g->work_bytes) return fail(-1, "workspace too small for this group");
+  if (precision != BSMPO_C128 && precision != BSMPO_C64 && precision != BSMPO_BF16X3)
+    return fail(-1, "unknown precision");
+  BSMPO_HIP(hipMemcpyAsync(d_work, reinterpret_cast<const char *>(blob) + kGroupHostHead,
+                           (size_t)g->staging_bytes, hipMemcpyHostToDevice, st));
+  const char *w = reinterpret_cast<const char *>(d_work);
+  int rc = launch_contract(*g, w, precision, st);
+  if (rc || contract_only) return rc;
+  if ((rc = launch_svd(*g, w, st))) return rc;
+  return launch_truncate(*g, w, st);
+}
+
+// single-update entry points run as a group of one (host image reused per thread)
+thread_local std::vector<char> t_blob;
+
+int single(const void *h_plan, const UpdateIO &io, void *d_work, size_t work_bytes, int precision,
+           hipStream_t st, bool contract_only) {
+  const void *plans[1] = {h_plan};
+  int64_t hb = 0, wb = 0;
+  int rc = group_query(1, plans, &hb, &wb);
+  if (rc) return rc;
+  if ((int64_t)work_bytes < wb) return fail(-1, "workspace too small for this plan");
+  if ((int64_t)t_blob.size() < hb) t_blob.resize((size_t)hb);
+  if ((rc = group_build(1, plans, &io, reinterpret_cast<char *>(d_work), t_blob.data(), hb)))
+    return rc;
+  return run_group(t_blob.data(), precision, d_work, work_bytes, st, contract_only);
 }
 
 }  // namespace
@@ -116,10 +141,16 @@ int bsmpo_contract(int d, int chi, const void *h_plan, const double *d_Glc, cons
   int rc = validate_plan(h_plan, d, chi);
   if (rc) return rc;
   if (!d_Glc || !d_Gcr || !d_LR || !d_U3 || !d_C || !d_T) return fail(-1, "contract: NULL pointer");
-  hipStream_t st = (hipStream_t)stream;
-  if ((rc = upload_plan(h_plan, d_work, work_bytes, st))) return rc;
-  return launch_contract(*header(h_plan), d_work, d_Glc, d_Gcr, d_LR, d_U3, d_C, d_T, d_LL, d_LC,
-                         precision, st);
+  UpdateIO io{};
+  io.Glc = const_cast<double *>(d_Glc);
+  io.Gcr = const_cast<double *>(d_Gcr);
+  io.LR = d_LR;
+  io.U3 = d_U3;
+  io.LL = d_LL;
+  io.LC = d_LC;
+  io.C_override = d_C;
+  io.T_override = d_T;
+  return single(h_plan, io, d_work, work_bytes, precision, (hipStream_t)stream, true);
 }
 
 int bsmpo_site_update(int d, int chi, const void *h_plan, double *d_Glc, double *d_Gcr,
@@ -130,17 +161,54 @@ int bsmpo_site_update(int d, int chi, const void *h_plan, double *d_Glc, double 
   if (rc) return rc;
   if (!d_Glc || !d_Gcr || !d_LR || !d_U3 || !d_charge_c || !d_lambda_c || !d_num_lambda)
     return fail(-1, "site_update: NULL pointer");
-  const PlanHeader &h = *header(h_plan);
-  hipStream_t st = (hipStream_t)stream;
-  if ((rc = upload_plan(h_plan, d_work, work_bytes, st))) return rc;
-  char *w = reinterpret_cast<char *>(d_work);
-  if ((rc = launch_contract(h, d_work, d_Glc, d_Gcr, d_LR, d_U3,
-                            reinterpret_cast<double *>(w + h.w_C),
-                            reinterpret_cast<double *>(w + h.w_T), nullptr, nullptr, precision,
-                            st)))
-    return rc;
-  if ((rc = launch_svd(h, d_work, st))) return rc;
-  return launch_truncate(h, d_work, d_Glc, d_Gcr, d_charge_c, d_lambda_c, d_num_lambda, edge, st);
+  UpdateIO io{};
+  io.Glc = d_Glc;
+  io.Gcr = d_Gcr;
+  io.LR = d_LR;
+  io.U3 = d_U3;
+  io.charge_c = d_charge_c;
+  io.lambda_c = d_lambda_c;
+  io.num_lambda = d_num_lambda;
+  io.edge = edge;
+  return single(h_plan, io, d_work, work_bytes, precision, (hipStream_t)stream, false);
+}
+
+int bsmpo_group_query(int n_upd, const void *const *h_plans, size_t *host_bytes,
+                      size_t *work_bytes) {
+  if (!h_plans || !host_bytes || !work_bytes) return fail(-1, "group_query: NULL argument");
+  int64_t hb = 0, wb = 0;
+  int rc = group_query(n_upd, h_plans, &hb, &wb);
+  if (rc) return rc;
+  *host_bytes = (size_t)hb;
+  *work_bytes = (size_t)wb;
+  return 0;
+}
+
+int bsmpo_group_build(int n_upd, const void *const *h_plans, const bsmpo_update_io *io,
+                      void *d_work, void *h_group, size_t host_bytes) {
+  if (!h_plans || !io || !d_work || !h_group) return fail(-1, "group_build: NULL argument");
+  std::vector<UpdateIO> v((size_t)std::max(n_upd, 0));
+  for (int u = 0; u < n_upd; ++u) {
+    if (!io[u].Glc || !io[u].Gcr || !io[u].LR || !io[u].U3 || !io[u].charge_c ||
+        !io[u].lambda_c || !io[u].num_lambda)
+      return fail(-1, "group_build: NULL pointer in update " + std::to_string(u));
+    v[u] = UpdateIO{};
+    v[u].Glc = io[u].Glc;
+    v[u].Gcr = io[u].Gcr;
+    v[u].LR = io[u].LR;
+    v[u].U3 = io[u].U3;
+    v[u].charge_c = io[u].charge_c;
+    v[u].lambda_c = io[u].lambda_c;
+    v[u].num_lambda = io[u].num_lambda;
+    v[u].edge = io[u].edge;
+  }
+  return group_build(n_upd, h_plans, v.data(), reinterpret_cast<char *>(d_work),
+                     reinterpret_cast<char *>(h_group), (int64_t)host_bytes);
+}
+
+int bsmpo_group_run(const void *h_group, int precision, void *d_work, size_t work_bytes,
+                    void *stream) {
+  return run_group(h_group, precision, d_work, work_bytes, (hipStream_t)stream, false);
 }
 
 int bsmpo_observables(int M, int d, int chi, const double *d_Gamma, const double *d_Lambda,

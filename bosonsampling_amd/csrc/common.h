@@ -42,6 +42,29 @@ __device__ __forceinline__ void cmfma_f64(dbl4 &accr, dbl4 &acci, cplx a, cplx b
   acci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, acci, 0, 0, 0);
 }
 
+// Everything a kernel needs about one update of a group: the update's plan
+// arrays (device copies), its scratch arrays and the MPO buffers it reads
+// and writes.  Kernels take an array of these plus a work list of
+// (update, local index) pairs, so one launch covers any number of updates.
+struct UpdDesc {
+  const Sector *sectors;
+  const int32_t *rows, *cols;
+  const Block *lblk, *cblk, *rblk;
+  const Item *items, *gram_items, *w_items, *v_items;
+  const int32_t *charges, *sv_sector;
+  double *C, *T, *G, *V, *W, *S, *lam, *td, *te, *tau, *triwork;
+  float *Cf;  // complex64 staging of the FP32 / BF16x3 contraction (unused for c128)
+  int32_t *flag, *slot, *kcount;
+  double *Glc, *Gcr;
+  const double *LR, *U3, *LL, *LC;
+  int32_t *charge_c;
+  double *lambda_c;
+  int32_t *num_lambda;
+  int32_t d, chi, edge, n_sectors, n_cblk, pad0;
+  int64_t sv_total;
+  double tie_rel;
+};
+
 // Plan arrays as seen by kernels (device copy at the front of the workspace).
 struct DevPlan {
   const Sector *sectors;

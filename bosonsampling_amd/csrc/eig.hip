@@ -75,20 +75,18 @@ __device__ double block_sum_big(double v, double *red) {
   return t;
 }
 
-__global__ __launch_bounds__(kTriThreads) void k_tridiag(PlanHeader h, const char *work,
-                                                         int cls_begin, double *__restrict__ Gbuf,
-                                                         double *__restrict__ TD,
-                                                         double *__restrict__ TE,
-                                                         double *__restrict__ TAU) {
+__global__ __launch_bounds__(kTriThreads) void k_tridiag(const UpdDesc *__restrict__ Ud,
+                                                         const int2 *__restrict__ list) {
   extern __shared__ double2 tsm[];
   __shared__ double red[16];
-  const DevPlan P = dev_plan(h, work);
-  const Sector s = P.sectors[P.jac_order[cls_begin + blockIdx.x]];
+  const int2 ref = list[blockIdx.x];
+  const UpdDesc &u = Ud[ref.x];
+  const Sector s = u.sectors[ref.y];
   const int n = s.n;
-  double2 *A = reinterpret_cast<double2 *>(Gbuf) + s.g_off;  // A(r, c) = A[c n + r]
+  double2 *A = reinterpret_cast<double2 *>(u.G) + s.g_off;  // A(r, c) = A[c n + r]
   double2 *vo = tsm, *wo = tsm + n, *vn = tsm + 2 * n;        // v_{j-1}, w_{j-1}, v_j
-  double *d = TD + s.s_off, *e = TE + s.s_off;
-  double2 *tau = reinterpret_cast<double2 *>(TAU) + s.s_off;
+  double *d = u.td + s.s_off, *e = u.te + s.s_off;
+  double2 *tau = reinterpret_cast<double2 *>(u.tau) + s.s_off;
   const int t = threadIdx.x, nt = blockDim.x;
   if (n == 1) {
     if (t == 0) { d[0] = A[0].x; tau[0] = make_double2(0.0, 0.0); }
@@ -233,16 +231,15 @@ __device__ double bisect_kth(const double *d, const double *e, int n, int j, dou
 }
 
 // grid (large sectors, ceil(n_max / 256)); thread -> eigenvalue index j
-__global__ __launch_bounds__(256) void k_bisect(PlanHeader h, const char *work, int cls_begin,
-                                                const double *__restrict__ TD,
-                                                const double *__restrict__ TE,
-                                                double *__restrict__ lam,
-                                                double *__restrict__ S,
-                                                int32_t *__restrict__ flags) {
-  const DevPlan P = dev_plan(h, work);
-  const Sector s = P.sectors[P.jac_order[cls_begin + blockIdx.x]];
+__global__ __launch_bounds__(256) void k_bisect(const UpdDesc *__restrict__ Ud,
+                                                const int2 *__restrict__ list) {
+  const int2 ref = list[blockIdx.x];
+  const UpdDesc &u = Ud[ref.x];
+  const Sector s = u.sectors[ref.y];
   const int n = s.n;
-  const double *d = TD + s.s_off, *e = TE + s.s_off;
+  const double *d = u.td + s.s_off, *e = u.te + s.s_off;
+  double *lam = u.lam, *S = u.S;
+  int32_t *flags = u.flag;
   // Gershgorin interval and pivot guard (redundant per thread; O(n))
   double lo = d[0], hi = d[0], emax2 = 0.0;
   for (int i = 0; i < n; ++i) {
@@ -266,20 +263,20 @@ __global__ __launch_bounds__(256) void k_bisect(PlanHeader h, const char *work, 
 }
 
 // per-sector count of kept singular values (a prefix of the sorted list)
-__global__ void k_kcount(PlanHeader h, const char *work, const int32_t *__restrict__ slot,
-                         int32_t *__restrict__ kcount) {
+__global__ void k_kcount(const UpdDesc *__restrict__ Ud, const int2 *__restrict__ list) {
   __shared__ int red[8];
-  const DevPlan P = dev_plan(h, work);
-  const Sector s = P.sectors[blockIdx.x];
+  const int2 ref = list[blockIdx.x];
+  const UpdDesc &u = Ud[ref.x];
+  const Sector s = u.sectors[ref.y];
   int c = 0;
-  for (int p = threadIdx.x; p < s.n; p += blockDim.x) c += slot[s.s_off + p] >= 0 ? 1 : 0;
+  for (int p = threadIdx.x; p < s.n; p += blockDim.x) c += u.slot[s.s_off + p] >= 0 ? 1 : 0;
   for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
   __syncthreads();
   if (threadIdx.x == 0) {
     int tot = 0;
     for (int i = 0; i < (int)(blockDim.x >> 6); ++i) tot += red[i];
-    kcount[blockIdx.x] = tot;
+    u.kcount[ref.y] = tot;
   }
 }
 
@@ -323,21 +320,19 @@ __device__ void tri_solve(const double *__restrict__ d, const double *__restrict
 // One block per large sector: eigenvectors of the kept eigenvalues j < k,
 // written as complex columns (n long) into the sector's Y slot, then
 // back-transformed by the Householder reflectors stored in G.
-__global__ __launch_bounds__(256) void k_tri_vectors(
-    PlanHeader h, const char *work, int cls_begin, const double *__restrict__ Gbuf,
-    const double *__restrict__ TD, const double *__restrict__ TE, const double *__restrict__ TAU,
-    const double *__restrict__ lam, const int32_t *__restrict__ kcount,
-    double *__restrict__ Vbuf, double *__restrict__ Wbuf, double *__restrict__ triwork) {
+__global__ __launch_bounds__(256) void k_tri_vectors(const UpdDesc *__restrict__ Ud,
+                                                     const int2 *__restrict__ list) {
   __shared__ int cstart[2048 + 1];
   __shared__ int ncl;
-  const DevPlan P = dev_plan(h, work);
-  const int sid = P.jac_order[cls_begin + blockIdx.x];
-  const Sector s = P.sectors[sid];
-  const int n = s.n, k = kcount[sid];
+  const int2 ref = list[blockIdx.x];
+  const UpdDesc &u = Ud[ref.x];
+  const Sector s = u.sectors[ref.y];
+  const int n = s.n, k = u.kcount[ref.y];
   if (k == 0) return;
-  const double *d = TD + s.s_off, *e = TE + s.s_off, *l = lam + s.s_off;
-  double2 *X = s.left_small ? reinterpret_cast<double2 *>(Vbuf) + s.v_off
-                            : reinterpret_cast<double2 *>(Wbuf) + s.w_off;  // column j: X[j n + r]
+  const double *d = u.td + s.s_off, *e = u.te + s.s_off, *l = u.lam + s.s_off;
+  double *triwork = u.triwork;
+  double2 *X = s.left_small ? reinterpret_cast<double2 *>(u.V) + s.v_off
+                            : reinterpret_cast<double2 *>(u.W) + s.w_off;  // column j: X[j n + r]
   // clusters of close eigenvalues (gap <= kClusterRel |T|) are orthogonalised
   // explicitly; outside them inverse iteration loses at most
   // eps |T| / gap <= 2e-10 orthogonality (LAPACK's stein uses 1e-3 |T|, which
@@ -382,19 +377,17 @@ __global__ __launch_bounds__(256) void k_tri_vectors(
 // x = H_0 H_1 ... H_{n-2} y for the kept eigenvectors of every large sector
 template <int kBtRows>
 __global__ __launch_bounds__(kBtRows == 16 ? 1024 : 256) void k_tri_backtransform(
-    PlanHeader h, const char *work, int cls_begin, const double *__restrict__ Gbuf,
-    const double *__restrict__ TAU, const int32_t *__restrict__ kcount,
-    double *__restrict__ Vbuf, double *__restrict__ Wbuf) {
+    const UpdDesc *__restrict__ Ud, const int2 *__restrict__ list) {
   extern __shared__ double2 vsm[];  // reflector cache (n)
-  const DevPlan P = dev_plan(h, work);
-  const int sid = P.jac_order[cls_begin + blockIdx.x];
-  const Sector s = P.sectors[sid];
-  const int n = s.n, k = kcount[sid];
+  const int2 ref = list[blockIdx.x];
+  const UpdDesc &u = Ud[ref.x];
+  const Sector s = u.sectors[ref.y];
+  const int n = s.n, k = u.kcount[ref.y];
   if (k == 0) return;
-  double2 *X = s.left_small ? reinterpret_cast<double2 *>(Vbuf) + s.v_off
-                            : reinterpret_cast<double2 *>(Wbuf) + s.w_off;
-  const double2 *A = reinterpret_cast<const double2 *>(Gbuf) + s.g_off;
-  const double2 *tau = reinterpret_cast<const double2 *>(TAU) + s.s_off;
+  double2 *X = s.left_small ? reinterpret_cast<double2 *>(u.V) + s.v_off
+                            : reinterpret_cast<double2 *>(u.W) + s.w_off;
+  const double2 *A = reinterpret_cast<const double2 *>(u.G) + s.g_off;
+  const double2 *tau = reinterpret_cast<const double2 *>(u.tau) + s.s_off;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   constexpr int kBtVec = 1;  // n <= 64 * kBtRows
   for (int j0 = 0; j0 < k; j0 += nw * kBtVec) {
@@ -459,18 +452,17 @@ __global__ __launch_bounds__(kBtRows == 16 ? 1024 : 256) void k_tri_backtransfor
   }
 }
 
-int launch_tri_values(const PlanHeader &h, void *d_work, hipStream_t stream) {
-  const int c = kTriClass;
-  const int nsec = h.jac_off[c + 1] - h.jac_off[c];
-  if (nsec == 0) return 0;
-  if (h.jac_max_n[c] > kTriThreads * kTriRowsPT)
+static inline const UpdDesc *gdesc(const GroupHeader &g, const char *w) {
+  return reinterpret_cast<const UpdDesc *>(w + g.o_desc);
+}
+static inline const int2 *glist(const char *w, int64_t off) {
+  return reinterpret_cast<const int2 *>(w + off);
+}
+
+int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
+  if (g.n_tri == 0) return 0;
+  if (g.tri_max_n > kTriThreads * kTriRowsPT)
     return fail(-1, "sector too large for the tridiagonal route (n > 2048)");
-  char *w = reinterpret_cast<char *>(d_work);
-  double *G = reinterpret_cast<double *>(w + h.w_G);
-  double *TD = reinterpret_cast<double *>(w + h.w_td);
-  double *TE = reinterpret_cast<double *>(w + h.w_te);
-  double *TAU = reinterpret_cast<double *>(w + h.w_tau);
-  const size_t lds = (size_t)3 * h.jac_max_n[c] * 16;
   static bool attr = false;
   if (!attr) {
     BSMPO_HIP(hipFuncSetAttribute((const void *)k_tridiag,
@@ -478,56 +470,41 @@ int launch_tri_values(const PlanHeader &h, void *d_work, hipStream_t stream) {
     attr = true;
   }
   int tk = timing_begin(ST_TRIDIAG, stream);
-  hipLaunchKernelGGL(k_tridiag, dim3(nsec), dim3(kTriThreads), lds, stream, h,
-                     (const char *)d_work, h.jac_off[c], G, TD, TE, TAU);
+  hipLaunchKernelGGL(k_tridiag, dim3(g.n_tri), dim3(kTriThreads), (size_t)3 * g.tri_max_n * 16,
+                     stream, gdesc(g, w), glist(w, g.o_tri));
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, 0.0, 0.0);
   tk = timing_begin(ST_BISECT, stream);
-  double *LAM = reinterpret_cast<double *>(w + h.w_lam);
-  double *S = reinterpret_cast<double *>(w + h.w_S);
-  int32_t *F = reinterpret_cast<int32_t *>(w + h.w_flag);
-  hipLaunchKernelGGL(k_bisect, dim3(nsec, (h.jac_max_n[c] + 255) / 256), dim3(256), 0, stream, h,
-                     (const char *)d_work, h.jac_off[c], (const double *)TD, (const double *)TE,
-                     LAM, S, F);
+  hipLaunchKernelGGL(k_bisect, dim3(g.n_tri, (g.tri_max_n + 255) / 256), dim3(256), 0, stream,
+                     gdesc(g, w), glist(w, g.o_tri));
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, 0.0, 0.0);
   return 0;
 }
 
-int launch_kcount(const PlanHeader &h, void *d_work, hipStream_t stream) {
-  if (h.n_sectors == 0) return 0;
-  char *w = reinterpret_cast<char *>(d_work);
-  hipLaunchKernelGGL(k_kcount, dim3(h.n_sectors), dim3(256), 0, stream, h, (const char *)d_work,
-                     (const int32_t *)(w + h.w_slot), (int32_t *)(w + h.w_kcount));
+int launch_kcount(const GroupHeader &g, const char *w, hipStream_t stream) {
+  if (g.n_sec == 0) return 0;
+  hipLaunchKernelGGL(k_kcount, dim3(g.n_sec), dim3(256), 0, stream, gdesc(g, w),
+                     glist(w, g.o_sec));
   BSMPO_HIP(hipGetLastError());
   return 0;
 }
 
-int launch_tri_vectors(const PlanHeader &h, void *d_work, hipStream_t stream) {
-  const int c = kTriClass;
-  const int nsec = h.jac_off[c + 1] - h.jac_off[c];
-  if (nsec == 0) return 0;
-  char *w = reinterpret_cast<char *>(d_work);
+int launch_tri_vectors(const GroupHeader &g, const char *w, hipStream_t stream) {
+  if (g.n_tri == 0) return 0;
   int tk = timing_begin(ST_TRIVEC, stream);
-  hipLaunchKernelGGL(k_tri_vectors, dim3(nsec), dim3(256), 0, stream, h, (const char *)d_work,
-                     h.jac_off[c], (const double *)(w + h.w_G), (const double *)(w + h.w_td),
-                     (const double *)(w + h.w_te), (const double *)(w + h.w_tau),
-                     (const double *)(w + h.w_lam), (const int32_t *)(w + h.w_kcount),
-                     (double *)(w + h.w_V), (double *)(w + h.w_W), (double *)(w + h.w_triwork));
+  hipLaunchKernelGGL(k_tri_vectors, dim3(g.n_tri), dim3(256), 0, stream, gdesc(g, w),
+                     glist(w, g.o_tri));
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, 0.0, 0.0);
   tk = timing_begin(ST_BACKTRANS, stream);
-  const size_t lds = (size_t)h.jac_max_n[c] * 16;
-  if (h.jac_max_n[c] <= 1024)
-    hipLaunchKernelGGL(k_tri_backtransform<16>, dim3(nsec), dim3(1024), lds, stream, h,
-                       (const char *)d_work, h.jac_off[c], (const double *)(w + h.w_G),
-                       (const double *)(w + h.w_tau), (const int32_t *)(w + h.w_kcount),
-                       (double *)(w + h.w_V), (double *)(w + h.w_W));
+  const size_t lds = (size_t)g.tri_max_n * 16;
+  if (g.tri_max_n <= 1024)
+    hipLaunchKernelGGL(k_tri_backtransform<16>, dim3(g.n_tri), dim3(1024), lds, stream,
+                       gdesc(g, w), glist(w, g.o_tri));
   else
-    hipLaunchKernelGGL(k_tri_backtransform<32>, dim3(nsec), dim3(256), lds, stream, h,
-                       (const char *)d_work, h.jac_off[c], (const double *)(w + h.w_G),
-                       (const double *)(w + h.w_tau), (const int32_t *)(w + h.w_kcount),
-                       (double *)(w + h.w_V), (double *)(w + h.w_W));
+    hipLaunchKernelGGL(k_tri_backtransform<32>, dim3(g.n_tri), dim3(256), lds, stream,
+                       gdesc(g, w), glist(w, g.o_tri));
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, 0.0, 0.0);
   return 0;

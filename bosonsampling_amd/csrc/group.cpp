@@ -1,0 +1,191 @@
+// Host construction of a group of planned updates: per-update descriptors,
+// flattened (update, item) work lists and the workspace layout.  One launch
+// per stage then covers every update of the group (all beam splitters of a
+// brick-wall layer, across any number of independent interferometers).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+
+namespace bsmpo {
+namespace {
+
+int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+struct Layout {
+  GroupHeader g;
+  std::vector<int64_t> plan_off, ws_off;
+};
+
+int layout(int n, const void *const *plans, Layout &L) {
+  if (n <= 0) return fail(-1, "group: no updates");
+  GroupHeader &g = L.g;
+  std::memset(&g, 0, sizeof(g));
+  g.magic = kGroupMagic;
+  g.n_upd = n;
+  int64_t n_items = 0, n_gram = 0, n_tri = 0, n_sec = 0, n_v = 0, n_w = 0, n_jac = 0;
+  for (int u = 0; u < n; ++u) {
+    const PlanHeader *h = header(plans[u]);
+    if (!h || h->magic != kPlanMagic) return fail(-1, "group: entry is not a bsmpo plan");
+    n_items += h->n_items;
+    n_gram += h->n_gram_items;
+    n_v += h->n_v_items;
+    n_w += h->n_w_items;
+    n_sec += h->n_sectors;
+    n_tri += h->jac_off[kTriClass + 1] - h->jac_off[kTriClass];
+    n_jac += h->jac_off[kTriClass] - h->jac_off[0];
+    for (int c = 0; c < kTriClass; ++c) g.jac_max_n[c] = std::max(g.jac_max_n[c], h->jac_max_n[c]);
+    g.tri_max_n = std::max(g.tri_max_n, h->jac_max_n[kTriClass]);
+    g.chi = std::max(g.chi, h->chi);
+    g.max_sv = std::max<int64_t>(g.max_sv, h->sv_total);
+    g.flops_alg += h->flops_alg;
+    g.flops_gram += h->flops_gram;
+    g.flops_jacobi += h->flops_jacobi;
+    g.flops_wmat += h->flops_wmat;
+    g.flops_vectors += h->flops_vectors;
+    g.bytes_alg16 += h->bytes_alg16;
+    g.c_elems += h->c_elems;
+    g.g_elems += h->g_elems;
+    g.v_elems += h->v_elems;
+    g.w_elems += h->w_elems;
+    g.sv_total += h->sv_total;
+  }
+  if (n_items > INT32_MAX || n_gram > INT32_MAX || n_w > INT32_MAX || n_v > INT32_MAX)
+    return fail(-1, "group: too many work items");
+  g.n_items = (int32_t)n_items;
+  g.n_gram = (int32_t)n_gram;
+  g.n_tri = (int32_t)n_tri;
+  g.n_sec = (int32_t)n_sec;
+  g.n_v = (int32_t)n_v;
+  g.n_w = (int32_t)n_w;
+  int64_t o = 0;
+  auto place = [&](int64_t bytes) {
+    const int64_t at = o;
+    o = align_up(o + std::max<int64_t>(bytes, 8), 256);
+    return at;
+  };
+  g.o_desc = place((int64_t)sizeof(UpdDesc) * n);
+  g.o_items = place(8 * n_items);
+  g.o_gram = place(8 * n_gram);
+  g.o_jac = place(8 * n_jac);
+  g.o_tri = place(8 * n_tri);
+  g.o_sec = place(8 * n_sec);
+  g.o_v = place(8 * n_v);
+  g.o_w = place(8 * n_w);
+  L.plan_off.resize(n);
+  for (int u = 0; u < n; ++u) L.plan_off[u] = place(header(plans[u])->dev_bytes);
+  g.staging_bytes = o;
+  L.ws_off.resize(n);
+  for (int u = 0; u < n; ++u) {
+    const PlanHeader *h = header(plans[u]);
+    L.ws_off[u] = place(h->w_total - h->w_C);
+  }
+  g.work_bytes = o;
+  g.host_bytes = kGroupHostHead + g.staging_bytes;
+  return 0;
+}
+
+}  // namespace
+
+int group_query(int n, const void *const *plans, int64_t *host_bytes, int64_t *work_bytes) {
+  Layout L;
+  int rc = layout(n, plans, L);
+  if (rc) return rc;
+  *host_bytes = L.g.host_bytes;
+  *work_bytes = L.g.work_bytes;
+  return 0;
+}
+
+int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_work,
+                char *host_blob, int64_t host_bytes) {
+  Layout L;
+  int rc = layout(n, plans, L);
+  if (rc) return rc;
+  GroupHeader &g = L.g;
+  if (host_bytes < g.host_bytes) return fail(-1, "group: host buffer too small");
+  char *img = host_blob + kGroupHostHead;  // image of d_work[0, staging_bytes)
+  std::memset(img, 0, (size_t)g.staging_bytes);
+  UpdDesc *desc = reinterpret_cast<UpdDesc *>(img + g.o_desc);
+  int2 *items = reinterpret_cast<int2 *>(img + g.o_items);
+  int2 *gram = reinterpret_cast<int2 *>(img + g.o_gram);
+  int2 *jac = reinterpret_cast<int2 *>(img + g.o_jac);
+  int2 *tri = reinterpret_cast<int2 *>(img + g.o_tri);
+  int2 *sec = reinterpret_cast<int2 *>(img + g.o_sec);
+  int2 *vl = reinterpret_cast<int2 *>(img + g.o_v);
+  int2 *wl = reinterpret_cast<int2 *>(img + g.o_w);
+  int64_t ni = 0, ng = 0, nt = 0, ns = 0, nv = 0, nw = 0;
+  for (int u = 0; u < n; ++u) {
+    const PlanHeader *h = header(plans[u]);
+    std::memcpy(img + L.plan_off[u], reinterpret_cast<const char *>(plans[u]) + h->dev_begin,
+                (size_t)h->dev_bytes);
+    char *pb = d_work + L.plan_off[u];           // device address of the plan copy
+    char *wb = d_work + L.ws_off[u] - h->w_C;     // so that wb + h->w_X is the scratch X
+    UpdDesc &D = desc[u];
+    D.sectors = reinterpret_cast<const Sector *>(pb + h->o_sectors);
+    D.rows = reinterpret_cast<const int32_t *>(pb + h->o_rows);
+    D.cols = reinterpret_cast<const int32_t *>(pb + h->o_cols);
+    D.lblk = reinterpret_cast<const Block *>(pb + h->o_lblk);
+    D.cblk = reinterpret_cast<const Block *>(pb + h->o_cblk);
+    D.rblk = reinterpret_cast<const Block *>(pb + h->o_rblk);
+    D.items = reinterpret_cast<const Item *>(pb + h->o_items);
+    D.gram_items = reinterpret_cast<const Item *>(pb + h->o_gram_items);
+    D.w_items = reinterpret_cast<const Item *>(pb + h->o_w_items);
+    D.v_items = reinterpret_cast<const Item *>(pb + h->o_v_items);
+    D.charges = reinterpret_cast<const int32_t *>(pb + h->o_charges);
+    D.sv_sector = reinterpret_cast<const int32_t *>(pb + h->o_sv_sector);
+    D.C = io[u].C_override ? io[u].C_override : reinterpret_cast<double *>(wb + h->w_C);
+    D.T = io[u].T_override ? io[u].T_override : reinterpret_cast<double *>(wb + h->w_T);
+    D.G = reinterpret_cast<double *>(wb + h->w_G);
+    D.V = reinterpret_cast<double *>(wb + h->w_V);
+    D.W = reinterpret_cast<double *>(wb + h->w_W);
+    D.S = reinterpret_cast<double *>(wb + h->w_S);
+    D.lam = reinterpret_cast<double *>(wb + h->w_lam);
+    D.td = reinterpret_cast<double *>(wb + h->w_td);
+    D.te = reinterpret_cast<double *>(wb + h->w_te);
+    D.tau = reinterpret_cast<double *>(wb + h->w_tau);
+    D.triwork = reinterpret_cast<double *>(wb + h->w_triwork);
+    D.Cf = nullptr;
+    D.flag = reinterpret_cast<int32_t *>(wb + h->w_flag);
+    D.slot = reinterpret_cast<int32_t *>(wb + h->w_slot);
+    D.kcount = reinterpret_cast<int32_t *>(wb + h->w_kcount);
+    D.Glc = io[u].Glc;
+    D.Gcr = io[u].Gcr;
+    D.LR = io[u].LR;
+    D.U3 = io[u].U3;
+    D.LL = io[u].LL;
+    D.LC = io[u].LC;
+    D.charge_c = io[u].charge_c;
+    D.lambda_c = io[u].lambda_c;
+    D.num_lambda = io[u].num_lambda;
+    D.d = h->d;
+    D.chi = h->chi;
+    D.edge = io[u].edge;
+    D.n_sectors = h->n_sectors;
+    D.n_cblk = h->n_cblk;
+    D.sv_total = h->sv_total;
+    D.tie_rel = h->tie_rel;
+    for (int i = 0; i < h->n_items; ++i) items[ni++] = make_int2(u, i);
+    for (int i = 0; i < h->n_gram_items; ++i) gram[ng++] = make_int2(u, i);
+    for (int i = 0; i < h->n_v_items; ++i) vl[nv++] = make_int2(u, i);
+    for (int i = 0; i < h->n_w_items; ++i) wl[nw++] = make_int2(u, i);
+    for (int i = 0; i < h->n_sectors; ++i) sec[ns++] = make_int2(u, i);
+    const int32_t *jo = host_array<int32_t>(plans[u], h->o_jac_order);
+    for (int i = h->jac_off[kTriClass]; i < h->jac_off[kTriClass + 1]; ++i) tri[nt++] = make_int2(u, jo[i]);
+  }
+  // LDS Jacobi list grouped by size class across updates
+  int64_t nj = 0;
+  for (int c = 0; c < kTriClass; ++c) {
+    g.jac_off[c] = (int32_t)nj;
+    for (int u = 0; u < n; ++u) {
+      const PlanHeader *h = header(plans[u]);
+      const int32_t *jo = host_array<int32_t>(plans[u], h->o_jac_order);
+      for (int i = h->jac_off[c]; i < h->jac_off[c + 1]; ++i) jac[nj++] = make_int2(u, jo[i]);
+    }
+  }
+  g.jac_off[kTriClass] = (int32_t)nj;
+  std::memcpy(host_blob, &g, sizeof(GroupHeader));
+  return 0;
+}
+
+}  // namespace bsmpo

@@ -34,6 +34,7 @@ constexpr int kJacClasses = 5;
 constexpr int kJacLimit[kJacClasses] = {16, 32, 64, 96, 1 << 30};
 constexpr int kTriClass = kJacClasses - 1;
 int eig_threshold();  // sectors with n > threshold take the tridiagonal route
+double tie_rel();     // relative width of the tie band at the chi-th singular value
 
 struct PlanHeader {
   int32_t magic, d, chi, n_sectors;
@@ -57,7 +58,8 @@ struct PlanHeader {
   // workspace layout (byte offsets from the start of d_work)
   int64_t w_C, w_T, w_G, w_V, w_W, w_S, w_lam, w_flag, w_slot;
   int64_t w_td, w_te, w_tau, w_kcount, w_triwork, w_total;
-  int32_t eig_threshold;
+  int32_t eig_threshold, pad_t;
+  double tie_rel;
 };
 
 struct Sector {
@@ -89,6 +91,25 @@ struct Item {
   int32_t pad0, pad1;
 };
 
+// A group: any number of planned updates run by one launch per stage.
+// Device layout of the group workspace (byte offsets from d_work):
+//   [UpdDesc x n_upd][work lists (int2 pairs)][plan device parts] = staging,
+//   uploaded from the host image in one copy, then each update's scratch.
+constexpr int32_t kGroupMagic = 0x4253475a;  // "BSGZ"
+struct GroupHeader {
+  int32_t magic, n_upd, chi, pad0;
+  int32_t n_items, n_gram, n_tri, n_sec, n_v, n_w;
+  int32_t jac_off[kTriClass + 1];
+  int32_t jac_max_n[kTriClass];
+  int32_t tri_max_n, pad1;
+  int64_t o_desc, o_items, o_gram, o_jac, o_tri, o_sec, o_v, o_w;
+  int64_t staging_bytes, work_bytes, max_sv, host_bytes;
+  double flops_alg, flops_gram, flops_jacobi, flops_wmat, flops_vectors, bytes_alg16;
+  double c_elems, g_elems, v_elems, w_elems, sv_total;
+};
+constexpr int64_t kGroupHostHead = 1024;  // host blob: header, then the staging image
+static_assert(sizeof(GroupHeader) <= kGroupHostHead, "group header too large");
+
 inline const PlanHeader *header(const void *plan) {
   return reinterpret_cast<const PlanHeader *>(plan);
 }
@@ -119,13 +140,27 @@ void timing_end(int token, hipStream_t stream, double flops, double bytes);
 // -------------------------------------------------------------- kernels --
 // launchers implemented in the .hip files; all enqueue on `stream`.
 int launch_bs_table(int d, int nb, const double *d_B, double *d_U3, hipStream_t stream);
-int launch_contract(const PlanHeader &h, void *d_work, const double *d_Glc,
-                    const double *d_Gcr, const double *d_LR, const double *d_U3, double *d_C,
-                    double *d_T, const double *d_LL, const double *d_LC, int precision,
-                    hipStream_t stream);
-int launch_svd(const PlanHeader &h, void *d_work, hipStream_t stream);
-int launch_truncate(const PlanHeader &h, void *d_work, double *d_Glc, double *d_Gcr,
-                    int32_t *d_charge_c, double *d_lambda_c, int32_t *d_num_lambda, int edge,
-                    hipStream_t stream);
+int launch_contract(const GroupHeader &g, const char *d_work, int precision, hipStream_t stream);
+int launch_contract_lowp(const GroupHeader &g, const char *d_work, int precision,
+                         hipStream_t stream);
+int launch_svd(const GroupHeader &g, const char *d_work, hipStream_t stream);
+int launch_truncate(const GroupHeader &g, const char *d_work, hipStream_t stream);
+
+// group construction (group.cpp)
+struct UpdateIO {
+  double *Glc, *Gcr;
+  const double *LR, *U3, *LL, *LC;
+  int32_t *charge_c;
+  double *lambda_c;
+  int32_t *num_lambda;
+  double *C_override, *T_override;  // bsmpo_contract writes to caller buffers
+  int edge;
+};
+int group_query(int n, const void *const *plans, int64_t *host_bytes, int64_t *work_bytes);
+int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_work,
+                char *host_blob, int64_t host_bytes);
+inline const GroupHeader *group_header(const void *blob) {
+  return reinterpret_cast<const GroupHeader *>(blob);
+}
 
 }  // namespace bsmpo

@@ -19,7 +19,9 @@ namespace bsmpo {
 
 static thread_local std::string g_last_error;
 static int g_eig_threshold = 96;
+static double g_tie_rel = 1e-6;
 int eig_threshold() { return g_eig_threshold; }
+double tie_rel() { return g_tie_rel; }
 
 void set_error(const std::string &msg) { g_last_error = msg; }
 int fail(int code, const std::string &msg) {
@@ -181,6 +183,7 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
     for (int i = 0; i < b.sectors[sid].n; ++i) b.sv_sector.push_back(sid);
   const int thr = eig_threshold();
   b.h.eig_threshold = thr;
+  b.h.tie_rel = tie_rel();
   int64_t tw = 0;
   for (int c = 0; c < kJacClasses; ++c) {
     b.h.jac_off[c] = (int32_t)b.jac_order.size();
@@ -303,6 +306,12 @@ const char *bsmpo_last_error(void) { return bsmpo::g_last_error.c_str(); }
 
 int bsmpo_abi_version(void) { return 1; }
 
+int bsmpo_set_tie_rel(double rel) {
+  if (!(rel >= 0.0 && rel < 1e-2)) return fail(-1, "tie band must be in [0, 1e-2)");
+  g_tie_rel = rel;
+  return 0;
+}
+
 int bsmpo_set_eig_threshold(int n) {
   if (n < 0 || n > 96) return fail(-1, "eig threshold must be in [0, 96]");
   g_eig_threshold = n;
@@ -317,7 +326,14 @@ int bsmpo_plan_query(int d, int chi, const int32_t *h_CL, const int32_t *h_CC,
   int rc = build(d, chi, h_CL, h_CC, h_CR, tau0, tau1, b);
   if (rc) return rc;
   *plan_bytes = (size_t)b.h.blob_bytes;
-  *work_bytes = (size_t)b.h.w_total;
+  // workspace of a single-update run (a group of one)
+  std::vector<char> blob((size_t)b.h.blob_bytes);
+  if ((rc = bsmpo_plan_build(d, chi, h_CL, h_CC, h_CR, tau0, tau1, blob.data(), blob.size())))
+    return rc;
+  const void *plans[1] = {blob.data()};
+  int64_t hb = 0, wb = 0;
+  if ((rc = group_query(1, plans, &hb, &wb))) return rc;
+  *work_bytes = (size_t)wb;
   return 0;
 }
 

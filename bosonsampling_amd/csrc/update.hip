@@ -87,44 +87,43 @@ int launch_bs_table(int d, int nb, const double *d_B, double *d_U3, hipStream_t 
 // left charge block cl.  The BS weight u(cl, ch, cr; tau) depends on the
 // column's cr and on the centre block ch only, so it is folded into the B
 // operand as it is loaded:  B'[c][j] = Gcr[c][col_j] * u(cl, ch(c), cr_j).
-__global__ __launch_bounds__(256) void k_contract_c128(
-    PlanHeader h, const char *work, const double *__restrict__ Glc,
-    const double *__restrict__ Gcr, const double *__restrict__ LR,
-    const double *__restrict__ U3, double *__restrict__ C, double *__restrict__ T,
-    const double *__restrict__ LL, const double *__restrict__ LC) {
-  const DevPlan P = dev_plan(h, work);
-  const int d = h.d, chi = h.chi;
+__global__ __launch_bounds__(256) void k_contract_c128(const UpdDesc *__restrict__ Ud,
+                                                       const int2 *__restrict__ list, int nlist) {
   const int lane = threadIdx.x & 63;
   const int i16 = lane & 15, kq = lane >> 4;
-  const int32_t *CR = P.charges + 4 * chi;
   const int nwaves = gridDim.x * 4;
-  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < h.n_items; it += nwaves) {
-    const Item item = P.items[it];
-    const Sector s = P.sectors[item.sector];
-    const Block bl = P.lblk[item.blk];
+  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < nlist; it += nwaves) {
+    const int2 ref = list[it];
+    const UpdDesc &u = Ud[ref.x];
+    const int d = u.d, chi = u.chi;
+    const int32_t *CR = u.charges + 4 * chi;
+    const Item item = u.items[ref.y];
+    const Sector s = u.sectors[item.sector];
+    const Block bl = u.lblk[item.blk];
     const bool rv = i16 < item.nr;
     const bool cv = i16 < item.nc;
-    const int grow = rv ? P.rows[s.row_off + item.r0 + i16] : 0;
-    const int gcol = cv ? P.cols[s.col_off + item.c0 + i16] : 0;
+    const int grow = rv ? u.rows[s.row_off + item.r0 + i16] : 0;
+    const int gcol = cv ? u.cols[s.col_off + item.c0 + i16] : 0;
     const int cr0 = cv ? CR[2 * gcol] : (1 << 20);
     const int cr1 = cv ? CR[2 * gcol + 1] : (1 << 20);
-    // smallest column charge in the tile (columns are sorted: lane 0 has it)
-    const int mincr0 = __shfl(cr0, 0), mincr1_all = cr1;
-    int mincr1 = mincr1_all;
+    // smallest column charge of the tile (columns are sorted: lane 0 has cr0)
+    const int mincr0 = __shfl(cr0, 0);
+    int mincr1 = cr1;
     for (int o = 8; o >= 1; o >>= 1) mincr1 = min(mincr1, __shfl_xor(mincr1, o));
-    const double2 *G1 = reinterpret_cast<const double2 *>(Glc);
-    const double2 *G2 = reinterpret_cast<const double2 *>(Gcr);
-    const double2 *U = reinterpret_cast<const double2 *>(U3);
+    const double2 *G1 = reinterpret_cast<const double2 *>(u.Glc);
+    const double2 *G2 = reinterpret_cast<const double2 *>(u.Gcr);
+    const double2 *U = reinterpret_cast<const double2 *>(u.U3);
+    const double *LC = u.LC;
     dbl4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
-    for (int cb = 0; cb < h.n_cblk; ++cb) {
-      const Block bc = P.cblk[cb];
+    for (int cb = 0; cb < u.n_cblk; ++cb) {
+      const Block bc = u.cblk[cb];
       if (bc.c0 > bl.c0 || bc.c1 > bl.c1) continue;
       if (bc.c0 < mincr0 || bc.c1 < mincr1) continue;
-      cplx u{0.0, 0.0};
+      cplx w{0.0, 0.0};
       if (cv && bc.c0 >= cr0 && bc.c1 >= cr1) {
         const double2 a = U[((bl.c0 - s.t0) * d + (s.t0 - cr0)) * d + (bl.c0 - bc.c0)];
         const double2 b = U[((bl.c1 - s.t1) * d + (s.t1 - cr1)) * d + (bl.c1 - bc.c1)];
-        u = cmulc(cplx{a.x, a.y}, cplx{b.x, b.y});
+        w = cmulc(cplx{a.x, a.y}, cplx{b.x, b.y});
       }
       for (int k0 = bc.start; k0 < bc.end; k0 += 4) {
         const int k = k0 + kq;
@@ -139,52 +138,38 @@ __global__ __launch_bounds__(256) void k_contract_c128(
           const double lc = LC ? LC[k] : 1.0;
           g = cplx{v.x * lc, v.y * lc};
         }
-        cmfma_f64(accr, acci, a, cmul(g, u));
+        cmfma_f64(accr, acci, a, cmul(g, w));
       }
     }
     if (cv) {
-      const double lr = LR[gcol];
+      const double lr = u.LR[gcol];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = mfma64_row(kq, r);
         if (row < item.nr) {
           const int64_t idx = s.c_off + (int64_t)(item.r0 + row) * s.R + item.c0 + i16;
-          const double w = LL ? lr * LL[P.rows[s.row_off + item.r0 + row]] : lr;
-          cstore(C, idx, cplx{accr[r], acci[r]});
-          cstore(T, idx, cplx{accr[r] * w, acci[r] * w});
+          const double wt = u.LL ? lr * u.LL[u.rows[s.row_off + item.r0 + row]] : lr;
+          cstore(u.C, idx, cplx{accr[r], acci[r]});
+          cstore(u.T, idx, cplx{accr[r] * wt, acci[r] * wt});
         }
       }
     }
   }
 }
 
-int launch_contract(const PlanHeader &h, void *d_work, const double *d_Glc, const double *d_Gcr,
-                    const double *d_LR, const double *d_U3, double *d_C, double *d_T,
-                    const double *d_LL, const double *d_LC, int precision, hipStream_t stream) {
-  if (precision != BSMPO_C128) return fail(-1, "contract: only BSMPO_C128 in this build");
-  if (h.n_items == 0) return 0;
-  const int tk = timing_begin(ST_CONTRACT, stream);
-  hipLaunchKernelGGL(k_contract_c128, dim3(grid_for(h.n_items)), dim3(256), 0, stream, h,
-                     (const char *)d_work, d_Glc, d_Gcr, d_LR, d_U3, d_C, d_T, d_LL, d_LC);
-  BSMPO_HIP(hipGetLastError());
-  timing_end(tk, stream, h.flops_alg, h.bytes_alg16);
-  return 0;
-}
-
 // ================================================================ Gram ==
 // G (n x n, column-major) = T T^H if L <= R else T^H T, T row-major L x R.
-__global__ __launch_bounds__(256) void k_gram(PlanHeader h, const char *work,
-                                              const double *__restrict__ T,
-                                              double *__restrict__ G) {
-  const DevPlan P = dev_plan(h, work);
+__global__ __launch_bounds__(256) void k_gram(const UpdDesc *__restrict__ Ud,
+                                              const int2 *__restrict__ list, int nlist) {
   const int lane = threadIdx.x & 63;
   const int i16 = lane & 15, kq = lane >> 4;
   const int nwaves = gridDim.x * 4;
-  const double2 *Tc = reinterpret_cast<const double2 *>(T);
-  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < h.n_gram_items; it += nwaves) {
-    const Item item = P.gram_items[it];
-    const Sector s = P.sectors[item.sector];
-    const double2 *Ts = Tc + s.c_off;
+  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < nlist; it += nwaves) {
+    const int2 ref = list[it];
+    const UpdDesc &u = Ud[ref.x];
+    const Item item = u.gram_items[ref.y];
+    const Sector s = u.sectors[item.sector];
+    const double2 *Ts = reinterpret_cast<const double2 *>(u.T) + s.c_off;
     const int p = item.r0 + i16, q = item.c0 + i16;
     const bool pv = i16 < item.nr, qv = i16 < item.nc;
     dbl4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
@@ -214,21 +199,21 @@ __global__ __launch_bounds__(256) void k_gram(PlanHeader h, const char *work,
       for (int r = 0; r < 4; ++r) {
         const int row = mfma64_row(kq, r);
         if (row < item.nr)
-          cstore(G, s.g_off + (int64_t)q * s.n + item.r0 + row, cplx{accr[r], acci[r]});
+          cstore(u.G, s.g_off + (int64_t)q * s.n + item.r0 + row, cplx{accr[r], acci[r]});
       }
     }
   }
 }
 
 // ============================================================== Jacobi ==
-// One-sided (Hestenes) Jacobi on the Hermitian PSD Gram matrix G = X:
-// rotate column pairs until all are orthogonal.  Then X = Y diag(lambda)
-// with Y the eigenvectors of G and lambda = sigma^2 its eigenvalues.
+// One-sided (Hestenes) Jacobi on the Hermitian PSD Gram matrix G = X, held
+// in LDS: rotate column pairs until all are orthogonal.  Then
+// X = Y diag(lambda) with Y the eigenvectors of G and lambda = sigma^2.
 // Column pairs follow the round-robin tournament; 8 lanes per pair.
 constexpr int kJacSweeps = 40;
 constexpr double kJacTol = 1e-15;
-// singular values with sigma <= kDropRel * sigma_max(sector) are the Gram
-// route's noise floor and are dropped (sigma := 0, vectors 0).
+// singular values with sigma <= 1e-6 sigma_max(sector) are the Gram route's
+// noise floor and are dropped (sigma := 0, vectors 0).
 constexpr double kDropRel2 = 1e-12;  // on lambda = sigma^2
 
 __device__ __forceinline__ void rr_pair(int step, int i, int n2, int &p, int &q) {
@@ -243,26 +228,21 @@ __device__ __forceinline__ void rr_pair(int step, int i, int n2, int &p, int &q)
   if (p > q) { const int t = p; p = q; q = t; }
 }
 
-template <bool kLds>
-__global__ __launch_bounds__(256) void k_jacobi(PlanHeader h, const char *work, int cls_begin,
-                                                double *__restrict__ Gbuf,
-                                                double *__restrict__ Vbuf,
-                                                double *__restrict__ Wbuf,
-                                                double *__restrict__ Sbuf,
-                                                int32_t *__restrict__ flags,
-                                                double *__restrict__ lam,
-                                                int32_t *__restrict__ rank) {
+__global__ __launch_bounds__(256) void k_jacobi(const UpdDesc *__restrict__ Ud,
+                                                const int2 *__restrict__ list) {
   extern __shared__ double2 smem[];
-  const DevPlan P = dev_plan(h, work);
-  const int sid = P.jac_order[cls_begin + blockIdx.x];
-  const Sector s = P.sectors[sid];
+  const int2 ref = list[blockIdx.x];
+  const UpdDesc &u = Ud[ref.x];
+  const Sector s = u.sectors[ref.y];
   const int n = s.n;
-  double2 *X = kLds ? smem : reinterpret_cast<double2 *>(Gbuf) + s.g_off;
+  double2 *X = smem;
   __shared__ int any_rot;
   __shared__ double lam_max;
   if (n == 0) return;
-  if (kLds) {
-    const double2 *Gs = reinterpret_cast<const double2 *>(Gbuf) + s.g_off;
+  double *lam = u.lam;
+  int32_t *rank = u.slot;  // scratch until the truncation stage
+  {
+    const double2 *Gs = reinterpret_cast<const double2 *>(u.G) + s.g_off;
     for (int e = threadIdx.x; e < n * n; e += blockDim.x) X[e] = Gs[e];
     __syncthreads();
   }
@@ -320,7 +300,7 @@ __global__ __launch_bounds__(256) void k_jacobi(PlanHeader h, const char *work, 
   }
   // eigenvalues = column norms (X = Y diag(lambda)); order them descending,
   // ties by index, and write sigma = sqrt(lambda), flags and unit vectors to
-  // the sorted positions.  lam / rank scratch: one entry per sv slot.
+  // the sorted positions.
   for (int p = threadIdx.x; p < n; p += blockDim.x) {
     double nr = 0.0;
     for (int i = 0; i < n; ++i) {
@@ -345,15 +325,15 @@ __global__ __launch_bounds__(256) void k_jacobi(PlanHeader h, const char *work, 
     rank[s.s_off + p] = r;
   }
   __syncthreads();
-  double2 *Yout = s.left_small ? reinterpret_cast<double2 *>(Vbuf) + s.v_off
-                               : reinterpret_cast<double2 *>(Wbuf) + s.w_off;
+  double2 *Yout = s.left_small ? reinterpret_cast<double2 *>(u.V) + s.v_off
+                               : reinterpret_cast<double2 *>(u.W) + s.w_off;
   const double thr = kDropRel2 * lam_max;
   for (int p = threadIdx.x; p < n; p += blockDim.x) {
     const double lp = lam[s.s_off + p];
     const bool keep = lam_max > 0.0 && lp > thr;
     const int r = rank[s.s_off + p];
-    Sbuf[s.s_off + r] = keep ? sqrt(lp) : 0.0;
-    flags[s.s_off + r] = keep ? 1 : 0;
+    u.S[s.s_off + r] = keep ? sqrt(lp) : 0.0;
+    u.flag[s.s_off + r] = keep ? 1 : 0;
   }
   for (int p = 0; p < n; ++p) {
     const double lp = lam[s.s_off + p];
@@ -368,22 +348,19 @@ __global__ __launch_bounds__(256) void k_jacobi(PlanHeader h, const char *work, 
 }
 
 // V = T Y diag(1/sigma) (L > R sectors); Y lives in the W area, n x n col-major.
-__global__ __launch_bounds__(256) void k_vectors(PlanHeader h, const char *work,
-                                                 const double *__restrict__ T,
-                                                 const double *__restrict__ Wbuf,
-                                                 const double *__restrict__ Sbuf,
-                                                 double *__restrict__ Vbuf,
-                                                 const int32_t *__restrict__ kcount) {
-  const DevPlan P = dev_plan(h, work);
+__global__ __launch_bounds__(256) void k_vectors(const UpdDesc *__restrict__ Ud,
+                                                 const int2 *__restrict__ list, int nlist) {
   const int lane = threadIdx.x & 63;
   const int i16 = lane & 15, kq = lane >> 4;
   const int nwaves = gridDim.x * 4;
-  const double2 *Tc = reinterpret_cast<const double2 *>(T);
-  const double2 *Yc = reinterpret_cast<const double2 *>(Wbuf);
-  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < h.n_v_items; it += nwaves) {
-    const Item item = P.v_items[it];
-    if (item.c0 >= kcount[item.sector]) continue;  // only kept vectors are needed
-    const Sector s = P.sectors[item.sector];
+  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < nlist; it += nwaves) {
+    const int2 ref = list[it];
+    const UpdDesc &u = Ud[ref.x];
+    const Item item = u.v_items[ref.y];
+    if (item.c0 >= u.kcount[item.sector]) continue;  // only kept vectors are needed
+    const Sector s = u.sectors[item.sector];
+    const double2 *Tc = reinterpret_cast<const double2 *>(u.T);
+    const double2 *Yc = reinterpret_cast<const double2 *>(u.W);
     const int i = item.r0 + i16, p = item.c0 + i16;
     const bool iv = i16 < item.nr, pv = i16 < item.nc;
     dbl4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
@@ -397,13 +374,13 @@ __global__ __launch_bounds__(256) void k_vectors(PlanHeader h, const char *work,
       cmfma_f64(accr, acci, a, b);
     }
     if (pv) {
-      const double sg = Sbuf[s.s_off + p];
+      const double sg = u.S[s.s_off + p];
       const double inv = sg > 0.0 ? 1.0 / sg : 0.0;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = mfma64_row(kq, r);
         if (row < item.nr)
-          cstore(Vbuf, s.v_off + (int64_t)p * s.L + item.r0 + row,
+          cstore(u.V, s.v_off + (int64_t)p * s.L + item.r0 + row,
                  cplx{accr[r] * inv, acci[r] * inv});
       }
     }
@@ -411,21 +388,19 @@ __global__ __launch_bounds__(256) void k_vectors(PlanHeader h, const char *work,
 }
 
 // W (n x R row-major) = V^H C, V L x n column-major, C L x R row-major.
-__global__ __launch_bounds__(256) void k_wmat(PlanHeader h, const char *work,
-                                              const double *__restrict__ Vbuf,
-                                              const double *__restrict__ C,
-                                              double *__restrict__ Wbuf,
-                                              const int32_t *__restrict__ kcount) {
-  const DevPlan P = dev_plan(h, work);
+__global__ __launch_bounds__(256) void k_wmat(const UpdDesc *__restrict__ Ud,
+                                              const int2 *__restrict__ list, int nlist) {
   const int lane = threadIdx.x & 63;
   const int i16 = lane & 15, kq = lane >> 4;
   const int nwaves = gridDim.x * 4;
-  const double2 *Vc = reinterpret_cast<const double2 *>(Vbuf);
-  const double2 *Cc = reinterpret_cast<const double2 *>(C);
-  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < h.n_w_items; it += nwaves) {
-    const Item item = P.w_items[it];
-    if (item.r0 >= kcount[item.sector]) continue;  // only kept rows of W are needed
-    const Sector s = P.sectors[item.sector];
+  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < nlist; it += nwaves) {
+    const int2 ref = list[it];
+    const UpdDesc &u = Ud[ref.x];
+    const Item item = u.w_items[ref.y];
+    if (item.r0 >= u.kcount[item.sector]) continue;  // only kept rows of W are needed
+    const Sector s = u.sectors[item.sector];
+    const double2 *Vc = reinterpret_cast<const double2 *>(u.V);
+    const double2 *Cc = reinterpret_cast<const double2 *>(u.C);
     const int p = item.r0 + i16, j = item.c0 + i16;
     const bool pv = i16 < item.nr, jv = i16 < item.nc;
     dbl4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
@@ -443,62 +418,13 @@ __global__ __launch_bounds__(256) void k_wmat(PlanHeader h, const char *work,
       for (int r = 0; r < 4; ++r) {
         const int row = mfma64_row(kq, r);
         if (row < item.nr)
-          cstore(Wbuf, s.w_off + (int64_t)(item.r0 + row) * s.R + j, cplx{accr[r], acci[r]});
+          cstore(u.W, s.w_off + (int64_t)(item.r0 + row) * s.R + j, cplx{accr[r], acci[r]});
       }
     }
   }
 }
 
-int launch_tri_values(const PlanHeader &h, void *d_work, hipStream_t stream);
-int launch_kcount(const PlanHeader &h, void *d_work, hipStream_t stream);
-int launch_tri_vectors(const PlanHeader &h, void *d_work, hipStream_t stream);
-
-int launch_svd(const PlanHeader &h, void *d_work, hipStream_t stream) {
-  char *w = reinterpret_cast<char *>(d_work);
-  double *T = reinterpret_cast<double *>(w + h.w_T);
-  double *C = reinterpret_cast<double *>(w + h.w_C);
-  double *G = reinterpret_cast<double *>(w + h.w_G);
-  double *V = reinterpret_cast<double *>(w + h.w_V);
-  double *W = reinterpret_cast<double *>(w + h.w_W);
-  double *S = reinterpret_cast<double *>(w + h.w_S);
-  int32_t *F = reinterpret_cast<int32_t *>(w + h.w_flag);
-  double *LAM = reinterpret_cast<double *>(w + h.w_lam);
-  int32_t *RK = reinterpret_cast<int32_t *>(w + h.w_slot);
-  if (h.n_sectors == 0) return 0;
-  int tk = timing_begin(ST_GRAM, stream);
-  hipLaunchKernelGGL(k_gram, dim3(grid_for(h.n_gram_items)), dim3(256), 0, stream, h,
-                     (const char *)d_work, (const double *)T, G);
-  BSMPO_HIP(hipGetLastError());
-  timing_end(tk, stream, h.flops_gram, 16.0 * (h.c_elems + h.g_elems));
-  static bool attr_set = false;
-  if (!attr_set) {
-    BSMPO_HIP(hipFuncSetAttribute((const void *)k_jacobi<true>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 96 * 16));
-    attr_set = true;
-  }
-  tk = timing_begin(ST_JACOBI, stream);
-  {
-    // all LDS classes in one launch (the sectors are contiguous in jac_order)
-    const int nsec = h.jac_off[kTriClass] - h.jac_off[0];
-    int nmax = 0;
-    for (int c = 0; c < kTriClass; ++c) nmax = std::max(nmax, h.jac_max_n[c]);
-    if (nsec > 0) {
-      const size_t lds = (size_t)nmax * nmax * 16;
-      hipLaunchKernelGGL(k_jacobi<true>, dim3(nsec), dim3(256), lds, stream, h,
-                         (const char *)d_work, h.jac_off[0], G, V, W, S, F, LAM, RK);
-      BSMPO_HIP(hipGetLastError());
-    }
-  }
-  timing_end(tk, stream, h.flops_jacobi, 16.0 * 2.0 * h.g_elems);
-  int rc = launch_tri_values(h, d_work, stream);
-  if (rc) return rc;
-  return 0;
-}
-
 // ========================================================= truncation ==
-// One block: global top-chi over all sectors' singular values (ties by the
-// lower concatenated index), then slots in concatenated order, which is the
-// (tau0, tau1, descending sigma) order of the stable charge sort.
 __device__ int block_sum_int(int v, int *scratch) {
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
   __syncthreads();
@@ -509,9 +435,6 @@ __device__ int block_sum_int(int v, int *scratch) {
   __syncthreads();
   return t;
 }
-
-// relative width of the tie band around the chi-th singular value
-constexpr double kTieRel = 1e-10;
 
 // exclusive prefix sum over a 256-thread block (Hillis-Steele in LDS)
 __device__ int block_scan_excl(int v, int *sc, int &total) {
@@ -530,23 +453,24 @@ __device__ int block_scan_excl(int v, int *sc, int &total) {
   return incl - v;
 }
 
-__global__ __launch_bounds__(256) void k_topk(PlanHeader h, const char *work,
-                                              const double *__restrict__ S,
-                                              const int32_t *__restrict__ flags,
-                                              int32_t *__restrict__ slot,
-                                              int32_t *__restrict__ charge_c,
-                                              double *__restrict__ lambda_c,
-                                              int32_t *__restrict__ num_lambda) {
+// One block per update: global top-chi over all sectors' singular values,
+// then slots in concatenated order, which is the (tau0, tau1, descending
+// sigma) order of the stable charge sort.  Values within tie_rel (relative)
+// of the chi-th largest t are ties, broken by the lower concatenated index
+// (degenerate twins of Hermitian-conjugate sectors).
+__global__ __launch_bounds__(256) void k_topk(const UpdDesc *__restrict__ Ud) {
   __shared__ int scratch[16];
-  const DevPlan P = dev_plan(h, work);
-  const int64_t nt = h.sv_total;
+  __shared__ int sc[256];
+  const UpdDesc &u = Ud[blockIdx.x];
+  const int64_t nt = u.sv_total;
+  const double *S = u.S;
+  const int32_t *flags = u.flag;
   const unsigned long long *key = reinterpret_cast<const unsigned long long *>(S);
   int valid = 0;
   for (int64_t i = threadIdx.x; i < nt; i += blockDim.x) valid += flags[i] ? 1 : 0;
   valid = block_sum_int(valid, scratch);
-  const int K = min(valid, h.chi);
-  // K-th largest key among valid entries, by bitwise binary search
-  unsigned long long t = 0ull;
+  const int K = min(valid, u.chi);
+  unsigned long long t = 0ull;  // K-th largest key, by bitwise binary search
   if (K > 0) {
     for (int bit = 62; bit >= 0; --bit) {
       const unsigned long long cand = t | (1ull << bit);
@@ -556,16 +480,12 @@ __global__ __launch_bounds__(256) void k_topk(PlanHeader h, const char *work,
       if (cnt >= K) t = cand;
     }
   }
-  // ties: values within kTieRel of the K-th largest t, broken by the lower
-  // concatenated index (degenerate twins of Hermitian-conjugate sectors)
   const double tv = __longlong_as_double((long long)t);
-  const double hi = tv * (1.0 + kTieRel), lo = tv * (1.0 - kTieRel);
+  const double hi = tv * (1.0 + u.tie_rel), lo = tv * (1.0 - u.tie_rel);
   int gt = 0;
   for (int64_t i = threadIdx.x; i < nt; i += blockDim.x) gt += (flags[i] && S[i] > hi) ? 1 : 0;
   gt = block_sum_int(gt, scratch);
   const int need_eq = K - gt;
-  // scan in index order: kept = value > hi, or lo <= value <= hi among the first need_eq
-  __shared__ int sc[256];
   int carry_eq = 0, carry_kept = 0;
   for (int64_t base = 0; base < nt; base += blockDim.x) {
     const int64_t i = base + threadIdx.x;
@@ -580,29 +500,40 @@ __global__ __launch_bounds__(256) void k_topk(PlanHeader h, const char *work,
     carry_eq += tot_eq;
     carry_kept += tot_kept;
     if (i < nt) {
-      slot[i] = kept ? pos : -1;
+      u.slot[i] = kept ? pos : -1;
       if (kept) {
-        const Sector s = P.sectors[P.sv_sector[i]];
-        charge_c[2 * pos] = s.t0;
-        charge_c[2 * pos + 1] = s.t1;
-        lambda_c[pos] = S[i];
+        const Sector s = u.sectors[u.sv_sector[i]];
+        u.charge_c[2 * pos] = s.t0;
+        u.charge_c[2 * pos + 1] = s.t1;
+        u.lambda_c[pos] = S[i];
       }
     }
   }
-  for (int j = K + threadIdx.x; j < h.chi; j += blockDim.x) {
-    charge_c[2 * j] = h.d;
-    charge_c[2 * j + 1] = h.d;
-    lambda_c[j] = 0.0;
+  for (int j = K + threadIdx.x; j < u.chi; j += blockDim.x) {
+    u.charge_c[2 * j] = u.d;
+    u.charge_c[2 * j + 1] = u.d;
+    u.lambda_c[j] = 0.0;
   }
-  if (threadIdx.x == 0) *num_lambda = K;
+  if (threadIdx.x == 0) *u.num_lambda = K;
 }
 
-// zero the stored part of the outputs (Full.py:130-135 edge storage)
-__global__ void k_zero(double *__restrict__ G0, double *__restrict__ G1, int chi, int w0,
-                       int w1, int c1) {
+// stored part of the outputs (Full.py:130-135 edge storage)
+__device__ __forceinline__ void out_region(const UpdDesc &u, int &w0, int &w1, int &c1) {
+  const int wd = min(u.chi, u.d * u.d);
+  w0 = u.edge ? wd : u.chi;
+  w1 = u.edge ? wd : u.chi;
+  c1 = u.edge ? 1 : u.chi;
+}
+
+// grid (x, n_updates): zero the stored part of Gamma0Out / Gamma1Out
+__global__ void k_zero(const UpdDesc *__restrict__ Ud) {
+  const UpdDesc &u = Ud[blockIdx.y];
+  int w0, w1, c1;
+  out_region(u, w0, w1, c1);
+  const int chi = u.chi;
   const int64_t tot0 = (int64_t)chi * w0, tot1 = (int64_t)w1 * c1;
-  double2 *a = reinterpret_cast<double2 *>(G0);
-  double2 *b = reinterpret_cast<double2 *>(G1);
+  double2 *a = reinterpret_cast<double2 *>(u.Glc);
+  double2 *b = reinterpret_cast<double2 *>(u.Gcr);
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot0 + tot1;
        e += (int64_t)gridDim.x * blockDim.x) {
     if (e < tot0) {
@@ -616,90 +547,116 @@ __global__ void k_zero(double *__restrict__ G0, double *__restrict__ G1, int chi
   }
 }
 
-// one block per singular-value slot: write its V column and W row
-__global__ __launch_bounds__(256) void k_scatter(PlanHeader h, const char *work,
-                                                 const int32_t *__restrict__ slot,
-                                                 const double *__restrict__ Vbuf,
-                                                 const double *__restrict__ Wbuf,
-                                                 double *__restrict__ G0,
-                                                 double *__restrict__ G1, int w0, int w1,
-                                                 int c1) {
-  const DevPlan P = dev_plan(h, work);
-  const int chi = h.chi;
-  for (int64_t i = blockIdx.x; i < h.sv_total; i += gridDim.x) {
-    const int j = slot[i];
+// grid (x, n_updates): one block per kept singular-value slot: V column, W row
+__global__ __launch_bounds__(256) void k_scatter(const UpdDesc *__restrict__ Ud) {
+  const UpdDesc &u = Ud[blockIdx.y];
+  int w0, w1, c1;
+  out_region(u, w0, w1, c1);
+  const int chi = u.chi;
+  for (int64_t i = blockIdx.x; i < u.sv_total; i += gridDim.x) {
+    const int j = u.slot[i];
     if (j < 0) continue;
-    const int sid = P.sv_sector[i];
-    const Sector s = P.sectors[sid];
+    const Sector s = u.sectors[u.sv_sector[i]];
     const int p = (int)(i - s.s_off);
-    const double2 *V = reinterpret_cast<const double2 *>(Vbuf) + s.v_off + (int64_t)p * s.L;
-    const double2 *W = reinterpret_cast<const double2 *>(Wbuf) + s.w_off + (int64_t)p * s.R;
-    double2 *a = reinterpret_cast<double2 *>(G0);
-    double2 *b = reinterpret_cast<double2 *>(G1);
+    const double2 *V = reinterpret_cast<const double2 *>(u.V) + s.v_off + (int64_t)p * s.L;
+    const double2 *W = reinterpret_cast<const double2 *>(u.W) + s.w_off + (int64_t)p * s.R;
+    double2 *a = reinterpret_cast<double2 *>(u.Glc);
+    double2 *b = reinterpret_cast<double2 *>(u.Gcr);
     if (j < w0)
       for (int r = threadIdx.x; r < s.L; r += blockDim.x)
-        a[(int64_t)P.rows[s.row_off + r] * chi + j] = V[r];
+        a[(int64_t)u.rows[s.row_off + r] * chi + j] = V[r];
     if (j < w1)
       for (int c = threadIdx.x; c < s.R; c += blockDim.x) {
-        const int gc = P.cols[s.col_off + c];
+        const int gc = u.cols[s.col_off + c];
         if (gc < c1) b[(int64_t)j * chi + gc] = W[c];
       }
   }
 }
 
-int launch_truncate(const PlanHeader &h, void *d_work, double *d_Glc, double *d_Gcr,
-                    int32_t *d_charge_c, double *d_lambda_c, int32_t *d_num_lambda, int edge,
-                    hipStream_t stream) {
-  char *w = reinterpret_cast<char *>(d_work);
-  const double *V = reinterpret_cast<const double *>(w + h.w_V);
-  const double *W = reinterpret_cast<const double *>(w + h.w_W);
-  const double *S = reinterpret_cast<const double *>(w + h.w_S);
-  const int32_t *F = reinterpret_cast<const int32_t *>(w + h.w_flag);
-  int32_t *slot = reinterpret_cast<int32_t *>(w + h.w_slot);
-  int tk = timing_begin(ST_TOPK, stream);
-  hipLaunchKernelGGL(k_topk, dim3(1), dim3(256), 0, stream, h, (const char *)d_work, S, F, slot,
-                     d_charge_c, d_lambda_c, d_num_lambda);
-  BSMPO_HIP(hipGetLastError());
-  timing_end(tk, stream, 0.0, 12.0 * h.sv_total);
-  int rc = launch_kcount(h, d_work, stream);
-  if (rc) return rc;
-  if ((rc = launch_tri_vectors(h, d_work, stream))) return rc;
-  {
-    double *T = reinterpret_cast<double *>(w + h.w_T);
-    double *C = reinterpret_cast<double *>(w + h.w_C);
-    double *Vm = reinterpret_cast<double *>(w + h.w_V);
-    double *Wm = reinterpret_cast<double *>(w + h.w_W);
-    double *Sm = reinterpret_cast<double *>(w + h.w_S);
-    const int32_t *KC = reinterpret_cast<const int32_t *>(w + h.w_kcount);
-  if (h.n_v_items) {
-    tk = timing_begin(ST_VECTORS, stream);
-    hipLaunchKernelGGL(k_vectors, dim3(grid_for(h.n_v_items)), dim3(256), 0, stream, h,
-                       (const char *)d_work, (const double *)T, (const double *)Wm,
-                       (const double *)Sm, Vm, KC);
-    BSMPO_HIP(hipGetLastError());
-    timing_end(tk, stream, h.flops_vectors, 0.0);
+// ============================================================ launchers ==
+int launch_tri_values(const GroupHeader &g, const char *d_work, hipStream_t stream);
+int launch_kcount(const GroupHeader &g, const char *d_work, hipStream_t stream);
+int launch_tri_vectors(const GroupHeader &g, const char *d_work, hipStream_t stream);
+
+static inline const UpdDesc *gdesc(const GroupHeader &g, const char *w) {
+  return reinterpret_cast<const UpdDesc *>(w + g.o_desc);
+}
+static inline const int2 *glist(const char *w, int64_t off) {
+  return reinterpret_cast<const int2 *>(w + off);
+}
+
+int launch_contract(const GroupHeader &g, const char *w, int precision, hipStream_t stream) {
+  if (g.n_items == 0) return 0;
+  const int tk = timing_begin(ST_CONTRACT, stream);
+  if (precision == BSMPO_C128) {
+    hipLaunchKernelGGL(k_contract_c128, dim3(grid_for(g.n_items)), dim3(256), 0, stream,
+                       gdesc(g, w), glist(w, g.o_items), g.n_items);
+  } else {
+    int rc = launch_contract_lowp(g, w, precision, stream);
+    if (rc) return rc;
   }
-  tk = timing_begin(ST_WMAT, stream);
-  hipLaunchKernelGGL(k_wmat, dim3(grid_for(h.n_w_items)), dim3(256), 0, stream, h,
-                     (const char *)d_work, (const double *)Vm, (const double *)C, Wm, KC);
   BSMPO_HIP(hipGetLastError());
-  timing_end(tk, stream, h.flops_wmat, 16.0 * (h.v_elems + h.c_elems + h.w_elems));
+  timing_end(tk, stream, g.flops_alg, g.bytes_alg16);
+  return 0;
+}
+
+int launch_svd(const GroupHeader &g, const char *w, hipStream_t stream) {
+  if (g.n_sec == 0) return 0;
+  int tk = timing_begin(ST_GRAM, stream);
+  hipLaunchKernelGGL(k_gram, dim3(grid_for(g.n_gram)), dim3(256), 0, stream, gdesc(g, w),
+                     glist(w, g.o_gram), g.n_gram);
+  BSMPO_HIP(hipGetLastError());
+  timing_end(tk, stream, g.flops_gram, 16.0 * (g.c_elems + g.g_elems));
+  static bool attr_set = false;
+  if (!attr_set) {
+    BSMPO_HIP(hipFuncSetAttribute((const void *)k_jacobi,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 96 * 16));
+    attr_set = true;
+  }
+  tk = timing_begin(ST_JACOBI, stream);
+  for (int c = 0; c < kTriClass; ++c) {
+    const int nsec = g.jac_off[c + 1] - g.jac_off[c];
+    if (nsec == 0) continue;
+    const size_t lds = (size_t)g.jac_max_n[c] * g.jac_max_n[c] * 16;
+    hipLaunchKernelGGL(k_jacobi, dim3(nsec), dim3(256), lds, stream, gdesc(g, w),
+                       glist(w, g.o_jac) + g.jac_off[c]);
+    BSMPO_HIP(hipGetLastError());
+  }
+  timing_end(tk, stream, g.flops_jacobi, 0.0);
+  return launch_tri_values(g, w, stream);
+}
+
+int launch_truncate(const GroupHeader &g, const char *w, hipStream_t stream) {
+  int tk = timing_begin(ST_TOPK, stream);
+  hipLaunchKernelGGL(k_topk, dim3(g.n_upd), dim3(256), 0, stream, gdesc(g, w));
+  BSMPO_HIP(hipGetLastError());
+  timing_end(tk, stream, 0.0, 12.0 * g.sv_total);
+  int rc = launch_kcount(g, w, stream);
+  if (rc) return rc;
+  if ((rc = launch_tri_vectors(g, w, stream))) return rc;
+  if (g.n_v) {
+    tk = timing_begin(ST_VECTORS, stream);
+    hipLaunchKernelGGL(k_vectors, dim3(grid_for(g.n_v)), dim3(256), 0, stream, gdesc(g, w),
+                       glist(w, g.o_v), g.n_v);
+    BSMPO_HIP(hipGetLastError());
+    timing_end(tk, stream, g.flops_vectors, 0.0);
+  }
+  if (g.n_w) {
+    tk = timing_begin(ST_WMAT, stream);
+    hipLaunchKernelGGL(k_wmat, dim3(grid_for(g.n_w)), dim3(256), 0, stream, gdesc(g, w),
+                       glist(w, g.o_w), g.n_w);
+    BSMPO_HIP(hipGetLastError());
+    timing_end(tk, stream, g.flops_wmat, 16.0 * (g.v_elems + g.c_elems + g.w_elems));
   }
   tk = timing_begin(ST_SCATTER, stream);
-  const int chi = h.chi;
-  const int wd = std::min(chi, h.d * h.d);
-  const int w0 = edge ? wd : chi, w1 = edge ? wd : chi, c1 = edge ? 1 : chi;
-  const int64_t tot = (int64_t)chi * w0 + (int64_t)w1 * c1;
-  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<int64_t>((tot + 255) / 256, 16384)),
-                     dim3(256), 0, stream, d_Glc, d_Gcr, chi, w0, w1, c1);
+  hipLaunchKernelGGL(k_zero, dim3(256, g.n_upd), dim3(256), 0, stream, gdesc(g, w));
   BSMPO_HIP(hipGetLastError());
-  if (h.sv_total > 0) {
-    hipLaunchKernelGGL(k_scatter, dim3((unsigned)std::min<int64_t>(h.sv_total, 65535)),
-                       dim3(256), 0, stream, h, (const char *)d_work, (const int32_t *)slot, V, W,
-                       d_Glc, d_Gcr, w0, w1, c1);
+  if (g.max_sv > 0) {
+    hipLaunchKernelGGL(k_scatter, dim3((unsigned)std::min<int64_t>(g.max_sv, 4096), g.n_upd),
+                       dim3(256), 0, stream, gdesc(g, w));
     BSMPO_HIP(hipGetLastError());
   }
-  timing_end(tk, stream, 0.0, 16.0 * ((double)chi * w0 + (double)w1 * c1));
+  timing_end(tk, stream, 0.0, 16.0 * 2.0 * g.n_upd * (double)g.chi * g.chi);
   return 0;
 }
 

@@ -204,6 +204,116 @@ class DeviceMPO:
             return -np.nansum(p * np.log2(p), axis=1)
 
 
+# ---------------------------------------------------------- grouped updates
+
+class GroupRunner:
+    """Runs many independent updates with one launch per stage (libbsmpo's
+    bsmpo_group_*): host plans, one pinned host image, one device workspace.
+
+    The updates of one call must touch disjoint sites (e.g. all beam
+    splitters of a brick-wall layer, across any number of DeviceMPO
+    instances).  ``max_bytes`` bounds the device workspace of one group; a
+    call with more work is split into several groups.
+    """
+
+    def __init__(self, device="cuda", max_bytes: int = 24 << 30):
+        self.device = torch.device(device)
+        self.max_bytes = int(max_bytes)
+        self.work = None
+        self.host = None
+        self.last_info = {}
+
+    def _work(self, n):
+        if self.work is None or self.work.numel() < n:
+            self.work = None
+            torch.cuda.empty_cache()
+            self.work = torch.empty(n, dtype=torch.uint8, device=self.device)
+        return self.work
+
+    def _host(self, n):
+        if self.host is None or self.host.numel() < n:
+            self.host = torch.empty(n, dtype=torch.uint8, pin_memory=True)
+        return self.host
+
+    def run(self, updates, precision: int = C128):
+        """updates: list of (DeviceMPO, mode, U3 device table)."""
+        if not updates:
+            return
+        plans = []
+        for st, mode, U3 in updates:
+            ch = st.charge_host
+            plans.append(Plan(st.d, st.chi, ch[mode], ch[mode + 1], ch[mode + 2]))
+        # split into groups that fit the workspace bound
+        groups, cur, cur_bytes = [], [], 0
+        for k, p in enumerate(plans):
+            est = p.work_bytes
+            if cur and cur_bytes + est > self.max_bytes:
+                groups.append(cur)
+                cur, cur_bytes = [], 0
+            cur.append(k)
+            cur_bytes += est
+        if cur:
+            groups.append(cur)
+        stream = _stream()
+        for grp in groups:
+            n = len(grp)
+            arr = (C.c_void_p * n)(*[plans[k].address for k in grp])
+            hb, wb = C.c_size_t(), C.c_size_t()
+            check(lib().bsmpo_group_query(n, arr, C.byref(hb), C.byref(wb)), "group_query")
+            # the previous group's image must be consumed before it is rewritten
+            torch.cuda.current_stream().synchronize()
+            work = self._work(wb.value)
+            host = self._host(hb.value)
+            io = (_capi.UpdateIO * n)()
+            for j, k in enumerate(grp):
+                st, mode, U3 = updates[k]
+                LR = st.edge if mode == st.M - 2 else st.Lambda[mode + 1]
+                io[j] = _capi.UpdateIO(st.Gamma[mode].data_ptr(), st.Gamma[mode + 1].data_ptr(),
+                                       LR.data_ptr(), U3.data_ptr(),
+                                       st.charge[mode + 1].data_ptr(), st.Lambda[mode].data_ptr(),
+                                       st.num_lambda.data_ptr(), 1 if mode == st.M - 2 else 0, 0)
+            check(lib().bsmpo_group_build(n, arr, io, C.c_void_p(work.data_ptr()),
+                                          C.c_void_p(host.data_ptr()), hb.value), "group_build")
+            check(lib().bsmpo_group_run(C.c_void_p(host.data_ptr()), int(precision),
+                                        C.c_void_p(work.data_ptr()), wb.value, stream),
+                  "group_run")
+        self.last_info = {"groups": len(groups), "updates": len(updates),
+                          "flops_alg": sum(p.info().flops_alg for p in plans)}
+        # host mirror of the new centre charges (one device-to-host copy per state)
+        by_state = {}
+        for st, mode, _ in updates:
+            by_state.setdefault(id(st), (st, []))[1].append(mode + 1)
+        for st, bonds in by_state.values():
+            idx = torch.tensor(bonds, device=st.device)
+            st.charge_host[bonds] = st.charge.index_select(0, idx).cpu().numpy()
+
+
+def simulate_batch(states, meshes, input_modes, layers=None, precision=C128, runner=None,
+                   observe=False):
+    """Layer-synchronous simulation of several independent instances: every
+    non-skipped beam splitter of a layer, over all instances, in one grouped
+    run (one launch per stage)."""
+    runner = runner or GroupRunner(states[0].device)
+    M, d = states[0].M, states[0].d
+    max_input = max(input_modes)
+    probs = [[] for _ in states]
+    n_updates = 0
+    for layer in (range(M) if layers is None else layers):
+        todo = layer_modes(M, layer, max_input)
+        if not todo:
+            continue
+        ups = []
+        for st, mesh in zip(states, meshes):
+            U3 = bs_tables(d, np.stack([mesh[layer, i] for i, _ in todo]), st.device)
+            ups.extend((st, mode, U3[k]) for k, (i, mode) in enumerate(todo))
+        runner.run(ups, precision)
+        n_updates += len(ups)
+        if observe:
+            for k, st in enumerate(states):
+                probs[k].append(st.observables()[0])
+    return n_updates, probs
+
+
 # ------------------------------------------------------------ initial state
 
 def input_density(d: int, eta: float) -> np.ndarray:

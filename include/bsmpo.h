@@ -46,6 +46,17 @@ int bsmpo_abi_version(void);
  */
 int bsmpo_set_eig_threshold(int n);
 
+/*
+ * Truncation tie band (process-wide, affects plans built afterwards): the
+ * kept set is every singular value above t (1 + rel), t the chi-th largest,
+ * then the values within [t (1 - rel), t (1 + rel)] in concatenated order
+ * (sector order, descending sigma).  Exactly degenerate twins are the rule
+ * (sectors (a, b) and (b, a) of the Hermitian density matrix), and rounding
+ * noise must not decide between them.  Default 1e-6; replaces the
+ * unspecified argpartition tie order of src/beam_splitter_parallel/Rank.py:142.
+ */
+int bsmpo_set_tie_rel(double rel);
+
 /* Contraction arithmetic of the sector GEMMs. */
 enum bsmpo_precision {
   BSMPO_C128 = 0,    /* complex128 on FP64 MFMA                             */
@@ -168,6 +179,41 @@ int bsmpo_contract(int d, int chi, const void *h_plan, const double *d_Glc,
  * element offset is returned in h_coff[s].  Host only. */
 int bsmpo_plan_sectors(const void *h_plan, int32_t *h_out, int64_t *h_coff,
                        int32_t *h_rows, int32_t *h_cols);
+
+/* ------------------------------------------------------ grouped updates -- */
+
+/*
+ * Any number of independent updates (all beam splitters of a brick-wall
+ * layer, across any number of interferometer instances) run by ONE launch per
+ * stage.  Each update has its own plan (bsmpo_plan_build) and its own MPO
+ * buffers; the updates must not share Gamma / charge / Lambda rows.
+ *   bsmpo_group_query : sizes of the host image and of the device workspace
+ *   bsmpo_group_build : writes the host image (h_group, preferably pinned)
+ *                       for this d_work; the image holds device addresses,
+ *                       so it is only valid with the d_work it was built for
+ *   bsmpo_group_run   : one host-to-device copy of the image, then the
+ *                       contraction, SVD and truncation stages on `stream`
+ * The per-update semantics are those of bsmpo_site_update.  No counterpart in
+ * the reference, which dispatches one update per MPI rank
+ * (src/beam_splitter_parallel/Full.py:149-166).
+ */
+typedef struct {
+  double *Glc;            /* Gamma[mode]   (chi x chi complex128, in/out)   */
+  double *Gcr;            /* Gamma[mode+1] (chi x chi complex128, in/out)   */
+  const double *LR;       /* Lambda[mode+1] or ones at the edge (chi f64)   */
+  const double *U3;       /* d^3 complex128 BS table                         */
+  int32_t *charge_c;      /* new centre charges [chi][2] (out)               */
+  double *lambda_c;       /* new centre Lambda [chi] (out)                   */
+  int32_t *num_lambda;    /* kept count [1] (out)                            */
+  int32_t edge;           /* last pair of the chain (Full.py:130-132)        */
+  int32_t reserved;
+} bsmpo_update_io;
+int bsmpo_group_query(int n_upd, const void *const *h_plans, size_t *host_bytes,
+                      size_t *work_bytes);
+int bsmpo_group_build(int n_upd, const void *const *h_plans, const bsmpo_update_io *io,
+                      void *d_work, void *h_group, size_t host_bytes);
+int bsmpo_group_run(const void *h_group, int precision, void *d_work, size_t work_bytes,
+                    void *stream);
 
 /* --------------------------------------------------------- observables -- */
 

@@ -183,10 +183,10 @@ def contract_sector_literal(d, U3, CL, CC, CR, Glc, Gcr, tau):
 # noise in any implementation (argpartition on float32 in the reference,
 # Rank.py:142).  Ties are broken by the lower concatenated index, on the CPU
 # and on the GPU alike.
-TIE_REL = 1e-10
+TIE_REL = 1e-6
 
 
-def select_top(svals_per_sector, chi, tie_rel=TIE_REL):
+def select_top(svals_per_sector, chi, tie_rel=None):
     """Global top-chi over all sectors' singular values (Rank.py:139-145).
 
     Deterministic stand-in for argpartition: keep num = min(total, chi)
@@ -196,6 +196,7 @@ def select_top(svals_per_sector, chi, tie_rel=TIE_REL):
     lexsort by charge of Rank.py:182-196 with a defined within-sector order.
     Returns a list of (sector, position).
     """
+    tie_rel = TIE_REL if tie_rel is None else tie_rel
     flat = []
     for s, S in enumerate(svals_per_sector):
         for p, v in enumerate(S):
