@@ -1,17 +1,27 @@
 #!/usr/bin/env python3
-"""Beam-splitter site-update throughput on one MI355X.
+"""Beam-splitter site-update throughput on one MI355X: grouped FP32 vs plain FP64.
 
-Workload (default): lossy boson-sampling MPO, M=64 modes, N=12 single photons
-in modes 0,5,...,55, d=13, chi=1024, eta=0.6, Haar unitary seed 100 as a
-Clements mesh of 2016 beam splitters.  A "step" is one brick-wall layer: all
-non-skipped BS updates of that layer, each = sector contraction on MFMA +
-FP64 sector SVD + global top-chi truncation + scatter, state resident in HBM.
-The first --warmup layers run untimed (they also bring the bond dimension
-up to chi); the next --steps layers are timed.
+Workload: lossy boson-sampling MPO, M=64 modes, N=12 single photons in modes
+0,5,...,55, d=13, chi=1024, eta=0.6; Haar unitaries (seeds 100, 101, ...)
+as Clements meshes of 2016 beam splitters each.
 
-Prints ONE JSON line (value = BS updates/s over the timed layers), with
-  roofline      the dominant kernel's algorithmic work / its HIP-event time
-  cpu_baseline  the FP64 CPU oracle on a bounded sample of the same updates
+A "step" is one brick-wall layer.  The first --warmup layers run untimed
+(they bring the bond dimension up to chi); the next --steps layers are
+timed, for two paths on the same inputs:
+  plain    one instance, complex128 contraction, every update launched on its
+           own (the reference's one-update-at-a-time structure)
+  grouped  --instances independent instances; every update of a layer across
+           all of them in one grouped run (one launch per stage), contraction
+           in --precision (default complex64 on FP32 MFMA), SVD / truncation /
+           BS elements / norm in FP64
+`value` is the grouped path's BS updates/s over its timed layers; the plain
+path's rate and the ratio are in `config`.
+
+Also on the JSON line:
+  roofline      the grouped run's dominant kernel: algorithmic work per launch
+                / its average HIP-event time over the timed region
+  cpu_baseline  the FP64 CPU oracle timed on a bounded sample of the same
+                updates (first updates of the first timed layer)
 """
 from __future__ import annotations
 
@@ -27,9 +37,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 METRIC = "BS updates/s + % MFMA/HBM peak, M=64 χ=1024: FP32-grouped vs plain FP64"
-
-# MI355X published peaks (dense)
-PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}   # MI355X dense
 PEAK_HBM_GBS = 8000.0
 
 
@@ -37,7 +45,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2, help="timed layers")
-    ap.add_argument("--warmup", type=int, default=30, help="untimed layers before the timed ones")
+    ap.add_argument("--warmup", type=int, default=16, help="untimed layers before the timed ones")
     ap.add_argument("--modes", type=int, default=64)
     ap.add_argument("--photons", type=int, default=12)
     ap.add_argument("--stride", type=int, default=5)
@@ -45,7 +53,10 @@ def parse():
     ap.add_argument("--chi", type=int, default=1024)
     ap.add_argument("--eta", type=float, default=0.6)
     ap.add_argument("--seed", type=int, default=100)
-    ap.add_argument("--precision", default="c128", choices=["c128", "c64", "bf16x3"])
+    ap.add_argument("--instances", type=int, default=32)
+    ap.add_argument("--precision", default="c64", choices=["c128", "c64", "bf16x3"])
+    ap.add_argument("--paths", default="plain,grouped")
+    ap.add_argument("--group-gib", type=float, default=24.0, help="workspace bound of one group")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="budget of the CPU-oracle baseline sample (0 = skip)")
     ap.add_argument("--verbose", action="store_true")
@@ -56,11 +67,38 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def stage_table(stages):
+    for s, (ms, fl, by, cnt) in stages.items():
+        if cnt:
+            log(f"[bench]   {s:13s} {ms:10.3f} ms {cnt:6d} launches "
+                f"{fl / max(ms, 1e-9) / 1e9:8.3f} TFLOP/s {by / max(ms, 1e-9) / 1e6:9.2f} GB/s")
+
+
+def roofline(stages, precision):
+    dom = max((s for s in stages if stages[s][3] > 0), key=lambda s: stages[s][0])
+    ms, fl, by, cnt = stages[dom]
+    t = ms / cnt / 1e3
+    if dom in ("topk", "scatter", "observables", "bs_table") or fl == 0.0:
+        ach = by / cnt / t / 1e9 if by else None
+        roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": (ach / PEAK_HBM_GBS) if ach else None, "traffic": None}
+    else:
+        peak = PEAK_TFLOPS["f64"]
+        if dom == "contract" and precision != "c128":
+            peak = PEAK_TFLOPS["f32"] if precision == "c64" else PEAK_TFLOPS["bf16"]
+        ach = fl / cnt / t / 1e12
+        roof = {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+                "frac": ach / peak, "traffic": None}
+    roof.update({"kernel": dom, "launches": cnt, "avg_launch_ms": ms / cnt,
+                 "stage_ms": {s: round(v[0], 3) for s, v in stages.items() if v[3]}})
+    return roof
+
+
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
-                                   "note only (see DESIGN.md), run with --gpus 1"}))
+        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU sharding is a "
+                                   "design note only -- run with --gpus 1"}))
         sys.exit(2)
     import torch
     from bosonsampling_amd import _capi, engine, mesh
@@ -70,102 +108,117 @@ def main():
     prec = {"c128": engine.C128, "c64": engine.C64, "bf16x3": engine.BF16X3}[args.precision]
     M, N, d, chi = args.modes, args.photons, args.d, args.chi
     inputs = list(range(0, N * args.stride, args.stride))
-    mm, _ = mesh.haar_clements_mesh(M, args.seed)
-    t0 = time.time()
-    st = engine.initialize(M, d, chi, inputs, args.eta)
-    torch.cuda.synchronize()
-    log(f"[bench] init {time.time() - t0:.2f}s")
-
-    max_input = max(inputs)
-    counts = []
-
-    def run_layer(layer):
-        todo = engine.layer_modes(M, layer, max_input)
-        if not todo:
-            return 0
-        U3 = engine.bs_tables(d, np.stack([mm[layer, i] for i, _ in todo]), st.device)
-        for k, (i, mode) in enumerate(todo):
-            st.update(mode, U3[k], prec)
-        return len(todo)
-
     W, K = args.warmup, args.steps
     if W + K > M:
         raise SystemExit("warmup + steps exceeds the number of layers")
-    for layer in range(W):
-        t1 = time.time()
-        n = run_layer(layer)
-        torch.cuda.synchronize()
-        if args.verbose:
-            log(f"[bench] warmup layer {layer}: {n} updates {time.time() - t1:.3f}s")
-    # snapshot for the CPU baseline (host copy, outside the timed region)
+    paths = args.paths.split(",")
+    B = args.instances
+    meshes = [mesh.haar_clements_mesh(M, args.seed + b)[0] for b in range(B)]
+    t0 = time.time()
+    init = engine.initialize(M, d, chi, inputs, args.eta)
+    torch.cuda.synchronize()
+    log(f"[bench] init {time.time() - t0:.2f}s")
+    max_input = max(inputs)
+    result = {}
+
+    # ---------------------------------------------------------------- plain
     snap = None
-    if args.cpu_seconds > 0:
-        snap = st.to_host()
-    # timed region
-    _capi.timing_enable(True)
-    _capi.timing_read(reset=True)
-    torch.cuda.synchronize()
-    t_start = time.time()
-    n_upd = 0
-    for layer in range(W, W + K):
-        n_upd += run_layer(layer)
-        log(f"[bench] timed layer {layer}: cumulative {n_upd} updates, {time.time() - t_start:.3f}s")
-    torch.cuda.synchronize()
-    elapsed = time.time() - t_start
-    stages = _capi.timing_read(reset=True)
-    _capi.timing_enable(False)
-    tr, marg = st.observables()
-    log(f"[bench] timed {n_upd} updates in {elapsed:.3f}s; trace {tr:.6f}")
-    for s, (ms, fl, by, cnt) in stages.items():
-        if cnt:
-            log(f"[bench]   {s:12s} {ms:10.3f} ms  {cnt:6d} launches  "
-                f"{fl / max(ms, 1e-9) / 1e9:8.3f} TFLOP/s  {by / max(ms, 1e-9) / 1e6:8.2f} GB/s")
+    if "plain" in paths:
+        st = init.clone()
 
-    # roofline of the dominant kernel
-    dom = max((s for s in stages if stages[s][3] > 0), key=lambda s: stages[s][0])
-    ms, fl, by, cnt = stages[dom]
-    t_launch = ms / cnt / 1e3
-    if dom in ("topk", "scatter", "observables", "bs_table"):
-        ach = by / cnt / t_launch / 1e9
-        roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": ach / PEAK_HBM_GBS, "traffic": None}
-    else:
-        peak = PEAK_TFLOPS["f64"]
-        if dom == "contract" and args.precision != "c128":
-            peak = PEAK_TFLOPS["f32"] if args.precision == "c64" else PEAK_TFLOPS["bf16"]
-        ach = fl / cnt / t_launch / 1e12
-        roof = {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
-                "frac": ach / peak, "traffic": None}
-    roof["kernel"] = dom
-    roof["launches"] = cnt
-    roof["avg_launch_ms"] = ms / cnt
-    roof["stage_ms"] = {s: round(v[0], 3) for s, v in stages.items() if v[3]}
+        def run_layer(layer, precision):
+            todo = engine.layer_modes(M, layer, max_input)
+            if not todo:
+                return 0
+            U3 = engine.bs_tables(d, np.stack([meshes[0][layer, i] for i, _ in todo]), st.device)
+            for k, (i, mode) in enumerate(todo):
+                st.update(mode, U3[k], precision)
+            return len(todo)
 
-    # CPU baseline: the FP64 oracle on the first updates of the timed layers
-    cpu = None
-    if snap is not None:
-        cpu = cpu_baseline(snap, d, chi, mm, inputs, W, args.cpu_seconds)
+        t1 = time.time()
+        for layer in range(W):
+            run_layer(layer, engine.C128)
+        torch.cuda.synchronize()
+        log(f"[bench] plain warmup {W} layers {time.time() - t1:.1f}s")
+        if args.cpu_seconds > 0:
+            snap = st.to_host()
+        _capi.timing_enable(True)
+        _capi.timing_read(reset=True)
+        torch.cuda.synchronize()
+        ts = time.time()
+        n = sum(run_layer(layer, engine.C128) for layer in range(W, W + K))
+        torch.cuda.synchronize()
+        el = time.time() - ts
+        st_plain = _capi.timing_read(reset=True)
+        _capi.timing_enable(False)
+        result["plain"] = {"updates": n, "seconds": el, "rate": n / el,
+                           "trace": st.observables()[0]}
+        log(f"[bench] plain c128: {n} updates in {el:.3f}s = {n / el:.2f} updates/s")
+        stage_table(st_plain)
+        del st
+        torch.cuda.empty_cache()
 
-    value = n_upd / elapsed if elapsed > 0 else 0.0
+    # -------------------------------------------------------------- grouped
+    roof = None
+    if "grouped" in paths:
+        states = [init.clone() for _ in range(B)]
+        runner = engine.GroupRunner(init.device, max_bytes=int(args.group_gib * (1 << 30)))
+        t1 = time.time()
+        for layer in range(W):
+            engine.simulate_batch(states, meshes, inputs, layers=[layer], runner=runner)
+            torch.cuda.synchronize()
+            if args.verbose:
+                log(f"[bench] grouped warmup layer {layer}: {time.time() - t1:.1f}s "
+                    f"groups {runner.last_info.get('groups')}")
+        log(f"[bench] grouped warmup {W} layers x {B} instances {time.time() - t1:.1f}s")
+        _capi.timing_enable(True)
+        _capi.timing_read(reset=True)
+        torch.cuda.synchronize()
+        ts = time.time()
+        n = 0
+        for layer in range(W, W + K):
+            k, _ = engine.simulate_batch(states, meshes, inputs, layers=[layer], runner=runner,
+                                         precision=prec)
+            n += k
+            log(f"[bench] grouped timed layer {layer}: {n} updates {time.time() - ts:.3f}s")
+        torch.cuda.synchronize()
+        el = time.time() - ts
+        st_grp = _capi.timing_read(reset=True)
+        _capi.timing_enable(False)
+        result["grouped"] = {"updates": n, "seconds": el, "rate": n / el,
+                             "trace0": states[0].observables()[0]}
+        log(f"[bench] grouped {args.precision}: {n} updates in {el:.3f}s = {n / el:.2f} updates/s")
+        stage_table(st_grp)
+        roof = roofline(st_grp, args.precision)
+
+    cpu = cpu_baseline(snap, d, chi, meshes[0], inputs, W, args.cpu_seconds) if snap else None
+    g = result.get("grouped")
+    p = result.get("plain")
+    main_path = g or p
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": main_path["rate"],
         "unit": "BS updates/s",
         "n_gpus": 1,
         "steps": K,
         "warmup": W,
-        "ms_per_step": 1e3 * elapsed / K,
+        "ms_per_step": 1e3 * main_path["seconds"] / K,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"c128": "complex128 (f64)", "c64": "complex64 (f32) contraction, f64 SVD",
-                  "bf16x3": "split-bf16x3 contraction, f64 SVD"}[args.precision],
-        "data": "synthetic: Haar unitary (seed %d) as a Clements mesh" % args.seed,
-        "config": {"workload": f"lossy boson-sampling MPO site updates, M={M} N={N} "
-                               f"(modes 0,{args.stride},..) d={d} chi={chi} eta={args.eta}, "
-                               f"one instance, layers {W}..{W + K - 1} timed",
-                   "path": f"plain {args.precision} (per-update launches)",
-                   "updates_timed": n_upd, "trace_after": tr},
+        "dtype": (f"{args.precision} contraction, f64 SVD/truncation" if g else "complex128 (f64)"),
+        "data": "synthetic: Haar unitaries (seeds %d..%d) as Clements meshes"
+                % (args.seed, args.seed + (B - 1 if g else 0)),
+        "config": {
+            "workload": f"lossy boson-sampling MPO site updates, M={M} N={N} "
+                        f"(modes 0,{args.stride},..) d={d} chi={chi} eta={args.eta}; "
+                        f"layers {W}..{W + K - 1} timed",
+            "grouped": ({"instances": B, "precision": args.precision, "updates": g["updates"],
+                         "updates_per_s": g["rate"]} if g else None),
+            "plain_fp64": ({"instances": 1, "updates": p["updates"],
+                            "updates_per_s": p["rate"]} if p else None),
+            "grouped_over_plain": (g["rate"] / p["rate"]) if (g and p) else None,
+        },
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -174,28 +227,28 @@ def main():
 
 def cpu_baseline(snap, d, chi, mm, inputs, layer, budget_s):
     """Time oracle.mpo.apply_update (numpy FP64) on the first updates of
-    `layer` from the GPU snapshot, until ~budget_s of CPU work."""
+    `layer` from the plain path's GPU snapshot, until ~budget_s of CPU work."""
     from oracle import fock, mpo
+    from bosonsampling_amd.engine import layer_modes
     G, L, ch = snap
     st = mpo.MPOState(d, chi, G.copy(), L.copy(), ch.copy(), np.ones(chi))
-    M = G.shape[0]
-    from bosonsampling_amd.engine import layer_modes
-    todo = layer_modes(M, layer, max(inputs))
-    n, t_used = 0, 0.0
+    todo = layer_modes(G.shape[0], layer, max(inputs))
+    n, used = 0, 0.0
     for i, mode in todo:
         U3 = fock.bs_table(d, mm[layer, i])
         t = time.time()
         mpo.apply_update(st, mode, U3)
-        t_used += time.time() - t
+        used += time.time() - t
         n += 1
-        log(f"[bench] cpu oracle update {n} (mode {mode}): {t_used:.2f}s")
-        if t_used >= budget_s:
+        log(f"[bench] cpu oracle update {n} (mode {mode}): {used:.2f}s")
+        if used >= budget_s:
             break
-    cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
-    return {"value": n / t_used if t_used > 0 else None, "unit": "BS updates/s", "cores": cores,
+    threads = os.environ.get("OPENBLAS_NUM_THREADS") or os.environ.get("OMP_NUM_THREADS")
+    cores = int(threads) if threads else len(os.sched_getaffinity(0))
+    return {"value": n / used if used > 0 else None, "unit": "BS updates/s", "cores": cores,
             "kind": "port",
-            "sample": f"{n} FP64 oracle updates (numpy/LAPACK) of layer {layer} from the GPU "
-                      f"snapshot, {t_used:.1f}s"}
+            "sample": f"{n} FP64 oracle updates (numpy + LAPACK) of layer {layer}, from the "
+                      f"plain path's GPU snapshot, {used:.1f}s"}
 
 
 if __name__ == "__main__":

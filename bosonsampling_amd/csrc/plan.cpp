@@ -91,7 +91,7 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
   }
 
   int64_t c_off = 0, g_off = 0, v_off = 0, w_off = 0, s_off = 0;
-  double f_tau = 0.0, theta_elems = 0.0;
+  double theta_elems = 0.0;
   for (int t0 = sr0; t0 <= lc0; ++t0) {
     for (int t1 = sr1; t1 <= lc1; ++t1) {
       if (only_t0 >= 0 && (t0 != only_t0 || t1 != only_t1)) continue;
@@ -150,13 +150,6 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
       for (auto &p : lsel) {
         const Block &bl = b.lblk[p.first];
         add_tiles(b.items, sid, p.first, p.second, p.second + (bl.end - bl.start), 0, R);
-        // F_tau: sum over (cl, ch, cr) with cl >= ch >= cr inside this sector
-        for (int rk : rsel) {
-          const Block &br = b.rblk[rk];
-          for (const Block &bc : b.cblk)
-            if (bl.c0 >= bc.c0 && bc.c0 >= br.c0 && bl.c1 >= bc.c1 && bc.c1 >= br.c1)
-              f_tau += 8.0 * (bl.end - bl.start) * (double)(bc.end - bc.start) * (br.end - br.start);
-        }
       }
       add_tiles(b.gram_items, sid, 0, 0, s.n, 0, s.n);
       add_tiles(b.w_items, sid, 0, 0, s.n, 0, R);
@@ -165,19 +158,41 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
     }
   }
 
-  // algorithmic counts (roofline accounting, complex MAC = 8 flops)
-  double f_alg = 0.0, lc = 0.0, cr = 0.0;
-  for (const Block &bl : b.lblk)
-    for (const Block &bc : b.cblk) {
-      if (!(bl.c0 >= bc.c0 && bl.c1 >= bc.c1)) continue;
-      lc += (double)(bl.end - bl.start) * (bc.end - bc.start);
-      for (const Block &br : b.rblk)
-        if (bc.c0 >= br.c0 && bc.c1 >= br.c1)
-          f_alg += 8.0 * (bl.end - bl.start) * (double)(bc.end - bc.start) * (br.end - br.start);
+  // algorithmic counts (roofline accounting, complex MAC = 8 flops), from
+  // 2-D prefix sums of the centre-charge histogram:
+  //   box(cl, cr) = sum_{cr <= ch <= cl} n_C(ch)
+  //   F_alg = 8 sum_{cl >= cr} n_L n_R box,  F_tau = F_alg weighted by the
+  //   number of sectors tau with cl >= tau >= cr
+  std::vector<double> pc((size_t)(d + 1) * (d + 1), 0.0);  // pc[(a+1)(d+1)+(b+1)] = sum_{c0<=a,c1<=b}
+  for (const Block &bc : b.cblk) pc[(size_t)(bc.c0 + 1) * (d + 1) + bc.c1 + 1] += bc.end - bc.start;
+  for (int a = 1; a <= d; ++a)
+    for (int c = 1; c <= d; ++c)
+      pc[(size_t)a * (d + 1) + c] += pc[(size_t)(a - 1) * (d + 1) + c] +
+                                     pc[(size_t)a * (d + 1) + c - 1] -
+                                     pc[(size_t)(a - 1) * (d + 1) + c - 1];
+  auto rect = [&](int a0, int b0, int a1, int b1) {  // sum over a0..a1, b0..b1 (inclusive)
+    if (a1 < a0 || b1 < b0) return 0.0;
+    auto P = [&](int a, int c) { return pc[(size_t)(a + 1) * (d + 1) + c + 1]; };
+    double r = P(a1, b1);
+    if (a0 > 0) r -= P(a0 - 1, b1);
+    if (b0 > 0) r -= P(a1, b0 - 1);
+    if (a0 > 0 && b0 > 0) r += P(a0 - 1, b0 - 1);
+    return r;
+  };
+  double f_alg = 0.0, f_tau = 0.0, lc = 0.0, cr = 0.0;
+  for (const Block &bl : b.lblk) {
+    const double nl = bl.end - bl.start;
+    lc += nl * rect(0, 0, bl.c0, bl.c1);
+    for (const Block &br : b.rblk) {
+      if (!(bl.c0 >= br.c0 && bl.c1 >= br.c1)) continue;
+      const double box = rect(br.c0, br.c1, bl.c0, bl.c1);
+      const double f = 8.0 * nl * (br.end - br.start) * box;
+      f_alg += f;
+      f_tau += f * (double)(bl.c0 - br.c0 + 1) * (bl.c1 - br.c1 + 1);
     }
-  for (const Block &bc : b.cblk)
-    for (const Block &br : b.rblk)
-      if (bc.c0 >= br.c0 && bc.c1 >= br.c1) cr += (double)(bc.end - bc.start) * (br.end - br.start);
+  }
+  for (const Block &br : b.rblk) cr += (br.end - br.start) * rect(br.c0, br.c1, d - 1, d - 1);
+  (void)theta_elems;
 
   for (int sid = 0; sid < (int)b.sectors.size(); ++sid)
     for (int i = 0; i < b.sectors[sid].n; ++i) b.sv_sector.push_back(sid);
@@ -281,6 +296,40 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
   return 0;
 }
 
+// The last plan built on this thread, so that query + build cost one build.
+struct Cached {
+  bool valid = false;
+  int d = 0, chi = 0, t0 = 0, t1 = 0, thr = 0;
+  double tie = 0.0;
+  std::vector<int32_t> key;
+  Built b;
+};
+thread_local Cached t_cache;
+
+int build_cached(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *CR, int t0,
+                 int t1, const Built **out) {
+  Cached &c = t_cache;
+  const size_t n = (size_t)2 * chi;
+  bool hit = c.valid && c.d == d && c.chi == chi && c.t0 == t0 && c.t1 == t1 &&
+             c.thr == eig_threshold() && c.tie == tie_rel() && c.key.size() == 3 * n &&
+             !std::memcmp(c.key.data(), CL, n * 4) && !std::memcmp(c.key.data() + n, CC, n * 4) &&
+             !std::memcmp(c.key.data() + 2 * n, CR, n * 4);
+  if (!hit) {
+    c.valid = false;
+    c.b = Built();
+    int rc = build(d, chi, CL, CC, CR, t0, t1, c.b);
+    if (rc) return rc;
+    c.key.resize(3 * n);
+    std::memcpy(c.key.data(), CL, n * 4);
+    std::memcpy(c.key.data() + n, CC, n * 4);
+    std::memcpy(c.key.data() + 2 * n, CR, n * 4);
+    c.d = d; c.chi = chi; c.t0 = t0; c.t1 = t1; c.thr = eig_threshold(); c.tie = tie_rel();
+    c.valid = true;
+  }
+  *out = &c.b;
+  return 0;
+}
+
 template <class T>
 void put(char *base, int64_t off, const std::vector<T> &v) {
   if (!v.empty()) std::memcpy(base + off, v.data(), sizeof(T) * v.size());
@@ -322,15 +371,12 @@ int bsmpo_plan_query(int d, int chi, const int32_t *h_CL, const int32_t *h_CC,
                      const int32_t *h_CR, int tau0, int tau1, size_t *plan_bytes,
                      size_t *work_bytes) {
   if (!h_CL || !h_CC || !h_CR || !plan_bytes || !work_bytes) return fail(-1, "NULL argument");
-  Built b;
-  int rc = build(d, chi, h_CL, h_CC, h_CR, tau0, tau1, b);
+  const Built *b = nullptr;
+  int rc = build_cached(d, chi, h_CL, h_CC, h_CR, tau0, tau1, &b);
   if (rc) return rc;
-  *plan_bytes = (size_t)b.h.blob_bytes;
-  // workspace of a single-update run (a group of one)
-  std::vector<char> blob((size_t)b.h.blob_bytes);
-  if ((rc = bsmpo_plan_build(d, chi, h_CL, h_CC, h_CR, tau0, tau1, blob.data(), blob.size())))
-    return rc;
-  const void *plans[1] = {blob.data()};
+  *plan_bytes = (size_t)b->h.blob_bytes;
+  // workspace of a single-update run (a group of one): header-only query
+  const void *plans[1] = {&b->h};
   int64_t hb = 0, wb = 0;
   if ((rc = group_query(1, plans, &hb, &wb))) return rc;
   *work_bytes = (size_t)wb;
@@ -340,12 +386,13 @@ int bsmpo_plan_query(int d, int chi, const int32_t *h_CL, const int32_t *h_CC,
 int bsmpo_plan_build(int d, int chi, const int32_t *h_CL, const int32_t *h_CC,
                      const int32_t *h_CR, int tau0, int tau1, void *h_plan, size_t plan_bytes) {
   if (!h_CL || !h_CC || !h_CR || !h_plan) return fail(-1, "NULL argument");
-  Built b;
-  int rc = build(d, chi, h_CL, h_CC, h_CR, tau0, tau1, b);
+  const Built *bp = nullptr;
+  int rc = build_cached(d, chi, h_CL, h_CC, h_CR, tau0, tau1, &bp);
   if (rc) return rc;
+  const Built &b = *bp;
   if ((int64_t)plan_bytes < b.h.blob_bytes) return fail(-1, "plan buffer too small");
   char *base = reinterpret_cast<char *>(h_plan);
-  std::memset(base, 0, (size_t)b.h.blob_bytes);
+  std::memset(base, 0, (size_t)b.h.dev_begin);
   std::memcpy(base, &b.h, sizeof(PlanHeader));
   char *dev = base + b.h.dev_begin;
   put(dev, b.h.o_sectors, b.sectors);

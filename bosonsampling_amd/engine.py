@@ -71,7 +71,7 @@ class Plan:
         if ws is not None:
             self.buf = ws.host_plan(self.plan_bytes)
         else:
-            self.buf = np.zeros(self.plan_bytes, np.uint8)
+            self.buf = np.empty(self.plan_bytes, np.uint8)
         check(lib().bsmpo_plan_build(d, chi, ptr(CL), ptr(CC), ptr(CR), t0, t1, ptr(self.buf),
                                      self.plan_bytes), "plan_build")
         self.d, self.chi = d, chi
@@ -220,20 +220,25 @@ class GroupRunner:
         self.device = torch.device(device)
         self.max_bytes = int(max_bytes)
         self.work = None
-        self.host = None
+        self.hosts = [None, None]      # pinned host images, used alternately
+        self.events = [None, None]     # consumption of each image by the device
+        self.turn = 0
         self.last_info = {}
 
     def _work(self, n):
         if self.work is None or self.work.numel() < n:
             self.work = None
-            torch.cuda.empty_cache()
             self.work = torch.empty(n, dtype=torch.uint8, device=self.device)
         return self.work
 
     def _host(self, n):
-        if self.host is None or self.host.numel() < n:
-            self.host = torch.empty(n, dtype=torch.uint8, pin_memory=True)
-        return self.host
+        i = self.turn
+        self.turn ^= 1
+        if self.events[i] is not None:
+            self.events[i].synchronize()   # the image's last upload has been consumed
+        if self.hosts[i] is None or self.hosts[i].numel() < n:
+            self.hosts[i] = torch.empty(n, dtype=torch.uint8, pin_memory=True)
+        return i, self.hosts[i]
 
     def run(self, updates, precision: int = C128):
         """updates: list of (DeviceMPO, mode, U3 device table)."""
@@ -260,10 +265,8 @@ class GroupRunner:
             arr = (C.c_void_p * n)(*[plans[k].address for k in grp])
             hb, wb = C.c_size_t(), C.c_size_t()
             check(lib().bsmpo_group_query(n, arr, C.byref(hb), C.byref(wb)), "group_query")
-            # the previous group's image must be consumed before it is rewritten
-            torch.cuda.current_stream().synchronize()
             work = self._work(wb.value)
-            host = self._host(hb.value)
+            slot, host = self._host(hb.value)
             io = (_capi.UpdateIO * n)()
             for j, k in enumerate(grp):
                 st, mode, U3 = updates[k]
@@ -277,15 +280,15 @@ class GroupRunner:
             check(lib().bsmpo_group_run(C.c_void_p(host.data_ptr()), int(precision),
                                         C.c_void_p(work.data_ptr()), wb.value, stream),
                   "group_run")
-        self.last_info = {"groups": len(groups), "updates": len(updates),
-                          "flops_alg": sum(p.info().flops_alg for p in plans)}
-        # host mirror of the new centre charges (one device-to-host copy per state)
-        by_state = {}
-        for st, mode, _ in updates:
-            by_state.setdefault(id(st), (st, []))[1].append(mode + 1)
-        for st, bonds in by_state.values():
-            idx = torch.tensor(bonds, device=st.device)
-            st.charge_host[bonds] = st.charge.index_select(0, idx).cpu().numpy()
+            ev = torch.cuda.Event()
+            ev.record()
+            self.events[slot] = ev
+        self.last_info = {"groups": len(groups), "updates": len(updates)}
+        # host mirror of the new centre charges: one device-to-host copy
+        rows = [st.charge[mode + 1] for st, mode, _ in updates]
+        new = torch.stack(rows).cpu().numpy()
+        for (st, mode, _), ch in zip(updates, new):
+            st.charge_host[mode + 1] = ch
 
 
 def simulate_batch(states, meshes, input_modes, layers=None, precision=C128, runner=None,
