@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--group-gib", type=float, default=24.0, help="workspace bound of one group")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="budget of the CPU-oracle baseline sample (0 = skip)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="MiB moved by one device copy launched first (PMC byte calibration)")
     ap.add_argument("--verbose", action="store_true")
     return ap.parse_args()
 
@@ -74,11 +76,14 @@ def stage_table(stages):
                 f"{fl / max(ms, 1e-9) / 1e9:8.3f} TFLOP/s {by / max(ms, 1e-9) / 1e6:9.2f} GB/s")
 
 
+HBM_BOUND = ("topk", "scatter", "observables", "bs_table", "tridiag")
+
+
 def roofline(stages, precision):
     dom = max((s for s in stages if stages[s][3] > 0), key=lambda s: stages[s][0])
     ms, fl, by, cnt = stages[dom]
     t = ms / cnt / 1e3
-    if dom in ("topk", "scatter", "observables", "bs_table") or fl == 0.0:
+    if dom in HBM_BOUND or fl == 0.0:
         ach = by / cnt / t / 1e9 if by else None
         roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": (ach / PEAK_HBM_GBS) if ach else None, "traffic": None}
@@ -106,6 +111,14 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a HIP device")
     prec = {"c128": engine.C128, "c64": engine.C64, "bf16x3": engine.BF16X3}[args.precision]
+    if args.copy_probe:
+        n = args.copy_probe << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: {n} bytes read + {n} bytes written")
+        del src, dst
     M, N, d, chi = args.modes, args.photons, args.d, args.chi
     inputs = list(range(0, N * args.stride, args.stride))
     W, K = args.warmup, args.steps

@@ -473,7 +473,7 @@ int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
   hipLaunchKernelGGL(k_tridiag, dim3(g.n_tri), dim3(kTriThreads), (size_t)3 * g.tri_max_n * 16,
                      stream, gdesc(g, w), glist(w, g.o_tri));
   BSMPO_HIP(hipGetLastError());
-  timing_end(tk, stream, 0.0, 0.0);
+  timing_end(tk, stream, g.flops_tridiag, g.bytes_tridiag);
   tk = timing_begin(ST_BISECT, stream);
   hipLaunchKernelGGL(k_bisect, dim3(g.n_tri, (g.tri_max_n + 255) / 256), dim3(256), 0, stream,
                      gdesc(g, w), glist(w, g.o_tri));

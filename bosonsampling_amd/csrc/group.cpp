@@ -50,6 +50,8 @@ int layout(int n, const void *const *plans, Layout &L) {
     g.v_elems += h->v_elems;
     g.w_elems += h->w_elems;
     g.sv_total += h->sv_total;
+    g.flops_tridiag += h->flops_tridiag;
+    g.bytes_tridiag += h->bytes_tridiag;
   }
   if (n_items > INT32_MAX || n_gram > INT32_MAX || n_w > INT32_MAX || n_v > INT32_MAX)
     return fail(-1, "group: too many work items");

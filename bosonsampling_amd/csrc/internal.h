@@ -46,6 +46,9 @@ struct PlanHeader {
   // flops of the SVD stages (complex MAC = 8): Gram 8 m n^2, W 8 n L R,
   // vectors 8 L R n, Jacobi one sweep of 8 n^3 (reported per sweep-equivalent)
   double flops_gram, flops_wmat, flops_vectors, flops_jacobi;
+  // tridiagonal route: Householder flops (16/3) n^3 and the fused pass's
+  // streaming traffic 32 sum_{m<n} m^2 bytes (trailing matrix read + written)
+  double flops_tridiag, bytes_tridiag;
   // host byte offsets of the device-copied region and total blob size
   int64_t dev_begin, blob_bytes;
   // byte offsets RELATIVE TO dev_begin (i.e. inside the device copy)
@@ -106,6 +109,7 @@ struct GroupHeader {
   int64_t staging_bytes, work_bytes, max_sv, host_bytes;
   double flops_alg, flops_gram, flops_jacobi, flops_wmat, flops_vectors, bytes_alg16;
   double c_elems, g_elems, v_elems, w_elems, sv_total;
+  double flops_tridiag, bytes_tridiag;
 };
 constexpr int64_t kGroupHostHead = 1024;  // host blob: header, then the staging image
 static_assert(sizeof(GroupHeader) <= kGroupHostHead, "group header too large");

@@ -213,6 +213,9 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
         b.jac_order.push_back(sid);
         b.h.jac_max_n[c] = std::max(b.h.jac_max_n[c], n);
         if (c == kTriClass) {
+          const double nn = n;
+          b.h.flops_tridiag += 16.0 / 3.0 * nn * nn * nn;
+          b.h.bytes_tridiag += 32.0 * (nn - 1.0) * nn * (2.0 * nn - 1.0) / 6.0;
           sc.tw_off = tw;
           tw += (int64_t)4 * n * std::min(n, 1024);  // per-thread inverse-iteration arrays
         }
