@@ -90,7 +90,6 @@ class Plan:
         inf = self.info()
         out = np.zeros(4 * max(inf.n_sectors, 1), np.int32)
         coff = np.zeros(max(inf.n_sectors, 1), np.int64)
-        tot_rows = int(sum(0 for _ in ()))
         check(lib().bsmpo_plan_sectors(C.c_void_p(self.address), ptr(out), ptr(coff), None, None),
               "plan_sectors")
         sec = out[:4 * inf.n_sectors].reshape(-1, 4)
@@ -201,7 +200,9 @@ class DeviceMPO:
         tot = sq.sum(axis=1, keepdims=True)
         with np.errstate(divide="ignore", invalid="ignore"):
             p = sq / tot
-            return -np.nansum(p * np.log2(p), axis=1)
+            out = -np.nansum(p * np.log2(p), axis=1)
+        out[tot[:, 0] <= 0] = np.nan
+        return out
 
 
 # ---------------------------------------------------------- grouped updates
