@@ -60,7 +60,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="budget of the CPU-oracle baseline sample (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
-                    help="MiB moved by one device copy launched first (PMC byte calibration)")
+                    help="MiB read and written by one elementwise kernel run first (PMC byte calibration)")
     ap.add_argument("--verbose", action="store_true")
     return ap.parse_args()
 
@@ -77,6 +77,21 @@ def stage_table(stages):
 
 
 HBM_BOUND = ("topk", "scatter", "observables", "bs_table", "tridiag")
+
+
+def measured_traffic(kernel, workload, instances, precision):
+    """HBM bytes per launch of `kernel` measured on this workload by two
+    rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE, scaled by a probe kernel
+    of known size) and recorded in profiles/<kernel>_*_traffic.json; None if
+    no record matches."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(HERE, "profiles", f"{kernel}_*_traffic.json"))):
+        with open(path) as f:
+            rec = json.load(f)
+        if (rec.get("kernel"), rec.get("workload"), rec.get("instances"), rec.get("precision")) == \
+                (kernel, workload, instances, precision):
+            return rec["traffic_bytes_per_launch"], os.path.relpath(path, HERE)
+    return None, None
 
 
 def roofline(stages, precision):
@@ -112,10 +127,11 @@ def main():
         raise SystemExit("bench.py needs a HIP device")
     prec = {"c128": engine.C128, "c64": engine.C64, "bf16x3": engine.BF16X3}[args.precision]
     if args.copy_probe:
+        # one elementwise kernel (torch's scalar multiply) that streams n bytes in and n out
         n = args.copy_probe << 20
-        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.mul(src, 2.0, out=dst)
         torch.cuda.synchronize()
         log(f"[bench] copy probe: {n} bytes read + {n} bytes written")
         del src, dst
@@ -205,6 +221,13 @@ def main():
         roof = roofline(st_grp, args.precision)
 
     cpu = cpu_baseline(snap, d, chi, meshes[0], inputs, W, args.cpu_seconds) if snap else None
+    workload = (f"lossy boson-sampling MPO site updates, M={M} N={N} "
+                f"(modes 0,{args.stride},..) d={d} chi={chi} eta={args.eta}; "
+                f"layers {W}..{W + K - 1} timed")
+    if roof is not None:
+        roof["traffic"], src = measured_traffic(roof["kernel"], workload, B, args.precision)
+        if src:
+            roof["traffic_source"] = src
     g = result.get("grouped")
     p = result.get("plain")
     main_path = g or p
@@ -223,9 +246,7 @@ def main():
         "data": "synthetic: Haar unitaries (seeds %d..%d) as Clements meshes"
                 % (args.seed, args.seed + (B - 1 if g else 0)),
         "config": {
-            "workload": f"lossy boson-sampling MPO site updates, M={M} N={N} "
-                        f"(modes 0,{args.stride},..) d={d} chi={chi} eta={args.eta}; "
-                        f"layers {W}..{W + K - 1} timed",
+            "workload": workload,
             "grouped": ({"instances": B, "precision": args.precision, "updates": g["updates"],
                          "updates_per_s": g["rate"]} if g else None),
             "plain_fp64": ({"instances": 1, "updates": p["updates"],
