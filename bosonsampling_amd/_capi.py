@@ -37,6 +37,12 @@ SIGNATURES = {
     "bsmpo_group_query": (C.c_int, [_i, _vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "bsmpo_group_build": (C.c_int, [_i, _vp, _vp, _vp, _vp, _sz]),
     "bsmpo_group_run": (C.c_int, [_vp, _i, _vp, _sz, _vp]),
+    "bsmpo_site_update_vjp_query": (C.c_int, [_i, _i, _vp, _i32p, C.POINTER(C.c_size_t)]),
+    "bsmpo_site_update_vjp": (C.c_int, [_i, _i, _vp, _i32p, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                        _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i, _vp, _sz, _vp]),
+    "bsmpo_bs_table_vjp": (C.c_int, [_i, _i, _dp, _dp, _dp, _vp]),
+    "bsmpo_observable_vjp": (C.c_int, [_i, _i, _i, _dp, _dp, _i32p, _i32p, _dp, _dp, _dp, _dp,
+                                       _vp]),
     "bsmpo_timing_enable": (C.c_int, [_i]),
     "bsmpo_timing_read": (C.c_int, [_dp, _dp, _dp, _i64p, _i]),
 }

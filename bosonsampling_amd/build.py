@@ -15,8 +15,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbsmpo.so")
-SOURCES = ["plan.cpp", "group.cpp", "capi.cpp", "update.hip", "contract_lowp.hip", "eig.hip",
-           "observables.hip"]
+SOURCES = ["plan.cpp", "group.cpp", "capi.cpp", "adjoint.cpp", "update.hip", "contract_lowp.hip",
+           "eig.hip", "observables.hip", "adjoint.hip"]
 ARCH = os.environ.get("BSMPO_ARCH", "gfx950")
 
 
@@ -35,7 +35,7 @@ def build(verbose: bool = False) -> str:
               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
               "-Wno-unused-command-line-argument", "-I", os.path.join(HERE, "..", "include")]
     for src in SOURCES:
-        obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
+        obj = os.path.join(build_dir, src + ".o")
         lang = ["-x", "hip"]
         cmd = [hipcc()] + common + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <math.h>
 
 #include "internal.h"
 
@@ -28,6 +29,34 @@ __device__ __forceinline__ cplx cload(const double *p, int64_t i) {
 }
 __device__ __forceinline__ void cstore(double *p, int64_t i, cplx v) {
   reinterpret_cast<double2 *>(p)[i] = make_double2(v.x, v.y);
+}
+
+// sqrt(n!) for n = 0..32, passed by value to the BS-table kernels
+struct BsArgs {
+  double sqf[33];
+};
+inline BsArgs bs_args() {
+  BsArgs a;
+  long double f = 1.0L;
+  for (int n = 0; n <= 32; ++n) {
+    if (n > 0) f *= (long double)n;
+    a.sqf[n] = (double)sqrtl(f);
+  }
+  return a;
+}
+
+__device__ __forceinline__ double binom_u64(int n, int k) {
+  if (k < 0 || k > n) return 0.0;
+  if (k > n - k) k = n - k;
+  unsigned long long r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (unsigned long long)(n - k + i) / (unsigned long long)i;
+  return (double)r;
+}
+
+__device__ __forceinline__ cplx cpow_int(cplx a, int e) {
+  cplx r{1.0, 0.0};
+  for (int i = 0; i < e; ++i) r = cmul(r, a);
+  return r;
 }
 
 // One K=4 step of a complex 16x16 tile on FP64 MFMA.

@@ -20,24 +20,6 @@
 namespace bsmpo {
 
 // ============================================================ BS table ==
-struct BsArgs {
-  double sqf[33];  // sqrt(n!) for n = 0..32
-};
-
-__device__ double binom_u64(int n, int k) {
-  if (k < 0 || k > n) return 0.0;
-  if (k > n - k) k = n - k;
-  unsigned long long r = 1;
-  for (int i = 1; i <= k; ++i) r = r * (unsigned long long)(n - k + i) / (unsigned long long)i;
-  return (double)r;
-}
-
-__device__ cplx cpow_int(cplx a, int e) {
-  cplx r{1.0, 0.0};
-  for (int i = 0; i < e; ++i) r = cmul(r, a);
-  return r;
-}
-
 // One block per beam splitter; U3[n1][n2][p] = <n1, n2|U(B)|p, q>, q = n1+n2-p:
 //   sqrt(n1! n2! / (p! q!)) sum_k C(p,k) C(q,n1-k) B00^k B10^(p-k) B01^(n1-k) B11^(q-n1+k)
 __global__ void k_bs_table(int d, const double *B, double *U3, BsArgs args) {
@@ -69,12 +51,7 @@ __global__ void k_bs_table(int d, const double *B, double *U3, BsArgs args) {
 int launch_bs_table(int d, int nb, const double *d_B, double *d_U3, hipStream_t stream) {
   if (d < 1 || d > 32 || nb < 0) return fail(-1, "bs_table: bad d or nb");
   if (nb == 0) return 0;
-  BsArgs a;
-  long double f = 1.0L;
-  for (int n = 0; n <= 32; ++n) {
-    if (n > 0) f *= (long double)n;
-    a.sqf[n] = (double)sqrtl(f);
-  }
+  const BsArgs a = bs_args();
   const int tk = timing_begin(ST_BSTABLE, stream);
   hipLaunchKernelGGL(k_bs_table, dim3(nb), dim3(256), 0, stream, d, d_B, d_U3, a);
   BSMPO_HIP(hipGetLastError());

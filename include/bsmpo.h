@@ -233,6 +233,61 @@ int bsmpo_observables(int M, int d, int chi, const double *d_Gamma, const double
                       const int32_t *d_charge, double *d_R, double *d_L, double *d_marg,
                       double *d_trace, void *stream);
 
+/* ------------------------------------------------------ reverse mode -- */
+/*
+ * The reference has no derivative code.  These entry points give the
+ * adjoint of the site update above (src/beam_splitter_parallel/Rank.py:90-196
+ * semantics with its truncation held fixed), of the BS table
+ * (src/u_kernel.cu:37-70 values) and of the observables
+ * (src/summary.py:11-26 chain), so that a caller can run reverse mode through
+ * a whole mesh.  Gradients of a real loss L are complex arrays
+ * g = dL/dRe z + i dL/dIm z (PyTorch's convention for complex tensors).
+ *
+ * bsmpo_site_update_vjp: one update backwards.
+ *   h_plan          : plan of the update's INPUT charges (bsmpo_plan_build)
+ *   h_charge_new    : host int32 [chi][2], the forward's new centre charges
+ *   d_Glc, d_Gcr,   : the forward's inputs (Gamma[mode], Gamma[mode+1],
+ *   d_LR, d_U3        LR, BS table), unchanged copies
+ *   d_G0new, d_G1new, d_lambda_new : the forward's outputs (separate buffers)
+ *   d_gG0, d_gG1, d_gLam           : gradients of those outputs
+ *   d_gGlc, d_gGcr (chi x chi complex), d_gU3 (d^3 complex): gradients of
+ *                     the inputs, written (gGlc may alias gG0, gGcr gG1)
+ *   d_gLR (chi)     : gradient of LR, ADDED to what the buffer holds
+ *   precision       : the forward's contraction precision (recomputed)
+ * The workspace size depends on the kept slots: query it with the same
+ * plan and new charges.  The call synchronises the stream once (it uploads
+ * per-call tables from pageable host memory).
+ */
+int bsmpo_site_update_vjp_query(int d, int chi, const void *h_plan, const int32_t *h_charge_new,
+                                size_t *work_bytes);
+int bsmpo_site_update_vjp(int d, int chi, const void *h_plan, const int32_t *h_charge_new,
+                          const double *d_Glc, const double *d_Gcr, const double *d_LR,
+                          const double *d_U3, const double *d_G0new, const double *d_G1new,
+                          const double *d_lambda_new, const double *d_gG0, const double *d_gG1,
+                          const double *d_gLam, double *d_gGlc, double *d_gGcr, double *d_gLR,
+                          double *d_gU3, int precision, void *d_work, size_t work_bytes,
+                          void *stream);
+
+/* d_gB[nb][2][2] complex128 = sum gU3 conj(dU3/dB) for the tables of
+ * bsmpo_bs_table(d, nb, d_B, ...); d_gU3 is [nb][d][d][d] complex128. */
+int bsmpo_bs_table_vjp(int d, int nb, const double *d_B, const double *d_gU3, double *d_gB,
+                       void *stream);
+
+/*
+ * L = -log P(pattern) + sum_j w_j <n_j> of an M-site MPO and dL/dGamma:
+ *   P(pattern) : the diagonal chain with site j's local occupation (ket charge
+ *                difference across the site) restricted to pattern[j]
+ *   <n_j>      : mean photon number of mode j (the marginals' first moment)
+ *   d_pattern  : device int32 [M];  d_weights : device float64 [M]
+ *   d_env      : device scratch, 6 (M+1) chi complex128
+ *   d_gGamma   : device M x chi x chi complex128 (written)
+ *   d_out      : device float64 [3] = {L, P, sum_j w_j <n_j>}
+ */
+int bsmpo_observable_vjp(int M, int d, int chi, const double *d_Gamma, const double *d_Lambda,
+                         const int32_t *d_charge, const int32_t *d_pattern,
+                         const double *d_weights, double *d_env, double *d_gGamma, double *d_out,
+                         void *stream);
+
 #ifdef __cplusplus
 }
 #endif
