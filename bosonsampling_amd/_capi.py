@@ -43,6 +43,9 @@ SIGNATURES = {
     "bsmpo_bs_table_vjp": (C.c_int, [_i, _i, _dp, _dp, _dp, _vp]),
     "bsmpo_observable_vjp": (C.c_int, [_i, _i, _i, _dp, _dp, _i32p, _i32p, _dp, _dp, _dp, _dp,
                                        _vp]),
+    "bsmpo_group_graph_create": (C.c_int, [_vp, _i, _vp, _sz, _vp, C.POINTER(C.c_void_p)]),
+    "bsmpo_graph_launch": (C.c_int, [_vp, _vp]),
+    "bsmpo_graph_destroy": (C.c_int, [_vp]),
     "bsmpo_timing_enable": (C.c_int, [_i]),
     "bsmpo_timing_read": (C.c_int, [_dp, _dp, _dp, _i64p, _i]),
 }

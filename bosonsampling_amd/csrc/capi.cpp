@@ -211,6 +211,42 @@ int bsmpo_group_run(const void *h_group, int precision, void *d_work, size_t wor
   return run_group(h_group, precision, d_work, work_bytes, (hipStream_t)stream, false);
 }
 
+int bsmpo_group_graph_create(const void *h_group, int precision, void *d_work, size_t work_bytes,
+                             void *stream, void **graph_exec) {
+  if (!graph_exec) return fail(-1, "group_graph_create: NULL graph_exec");
+  *graph_exec = nullptr;
+  hipStream_t st = (hipStream_t)stream;
+  if (!st) return fail(-1, "group_graph_create: capture needs a non-default stream");
+  if (timing_on()) return fail(-1, "group_graph_create: disable timing before capturing");
+  BSMPO_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  const int rc = run_group(h_group, precision, d_work, work_bytes, st, false);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &graph);
+  if (rc) {
+    if (graph) hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail((int)e, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e2 != hipSuccess)
+    return fail((int)e2, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+  *graph_exec = exec;
+  return 0;
+}
+
+int bsmpo_graph_launch(void *graph_exec, void *stream) {
+  if (!graph_exec) return fail(-1, "graph_launch: NULL graph");
+  BSMPO_HIP(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream));
+  return 0;
+}
+
+int bsmpo_graph_destroy(void *graph_exec) {
+  if (graph_exec) BSMPO_HIP(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+  return 0;
+}
+
 int bsmpo_observables(int M, int d, int chi, const double *d_Gamma, const double *d_Lambda,
                       const int32_t *d_charge, double *d_R, double *d_L, double *d_marg,
                       double *d_trace, void *stream) {

@@ -56,13 +56,21 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// Householder tridiagonalisation G = Q T Q^H, one block of 1024 threads per
-// sector.  Step j reduces column j; its rank-2 update
-// A22 -= v_j w_j^H + w_j v_j^H is fused with the matrix-vector product of
-// step j+1 into one read-modify-write pass over the trailing matrix.
-constexpr int kTriThreads = 1024;
-constexpr int kTriRowsPT = 2;  // rows per thread: n <= 2048
-constexpr int kTriUnroll = 8;
+// Householder tridiagonalisation G = Q T Q^H, one block of 512 threads per
+// sector, blocked by panels of kTriNB columns (the LAPACK zhetrd / zlatrd
+// organisation, restated):
+//   column j of a panel: apply the panel's earlier reflectors to column j,
+//   form its reflector v_j, then w_j from p = A22 v_j, where A22 is the
+//   trailing matrix as of the panel start and the panel's pending rank-2
+//   updates enter as corrections -V (W^H v) - W (V^H v);
+//   after the panel: one pass A22 -= V W^H + W V^H over the trailing matrix.
+// Per column the trailing matrix is read once (the product A22 v); it is
+// written once per panel.  V and W of the panel live in the sector's
+// tridiagonal work area (n x kTriNB each, column-major).
+constexpr int kTriThreads = 512;
+constexpr int kTriRowsPT = 4;  // rows per thread: n <= 2048
+constexpr int kTriNB = kTriPanel;
+constexpr int kTriChunk = 64;  // trailing-update columns staged in LDS at a time
 
 __device__ double block_sum_big(double v, double *red) {
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
@@ -75,130 +83,216 @@ __device__ double block_sum_big(double v, double *red) {
   return t;
 }
 
+__device__ __forceinline__ double2 csub_mul(double2 a, double2 x, double2 y) {  // a - x conj(y)
+  a.x -= x.x * y.x + x.y * y.y;
+  a.y -= x.y * y.x - x.x * y.y;
+  return a;
+}
+__device__ __forceinline__ double2 cfma2(double2 acc, double2 a, double2 b) {  // acc + a b
+  acc.x += a.x * b.x - a.y * b.y;
+  acc.y += a.x * b.y + a.y * b.x;
+  return acc;
+}
+
 __global__ __launch_bounds__(kTriThreads) void k_tridiag(const UpdDesc *__restrict__ Ud,
                                                          const int2 *__restrict__ list) {
   extern __shared__ double2 tsm[];
   __shared__ double red[16];
+  __shared__ double red4[16][4 * kTriNB];
+  __shared__ double2 yz[2 * kTriNB];                   // y_l = W_l^H v, z_l = V_l^H v
+  __shared__ double2 cw[kTriChunk][kTriNB], cv[kTriChunk][kTriNB];  // conj(W(c, l)), conj(V(c, l))
   const int2 ref = list[blockIdx.x];
   const UpdDesc &u = Ud[ref.x];
   const Sector s = u.sectors[ref.y];
   const int n = s.n;
   double2 *A = reinterpret_cast<double2 *>(u.G) + s.g_off;  // A(r, c) = A[c n + r]
-  double2 *vo = tsm, *wo = tsm + n, *vn = tsm + 2 * n;        // v_{j-1}, w_{j-1}, v_j
+  double2 *vs = tsm;                                         // v_j, n entries
+  double2 *Vp = reinterpret_cast<double2 *>(u.triwork + s.tw_off);
+  double2 *Wp = Vp + (int64_t)n * kTriNB;
   double *d = u.td + s.s_off, *e = u.te + s.s_off;
   double2 *tau = reinterpret_cast<double2 *>(u.tau) + s.s_off;
-  const int t = threadIdx.x, nt = blockDim.x;
+  const int t = threadIdx.x, nt = blockDim.x, lane = t & 63, wave = t >> 6;
   if (n == 1) {
     if (t == 0) { d[0] = A[0].x; tau[0] = make_double2(0.0, 0.0); }
     return;
   }
-  for (int r = t; r < n; r += nt) { vo[r] = make_double2(0.0, 0.0); wo[r] = make_double2(0.0, 0.0); }
-  __syncthreads();
-  // j runs over columns to reduce; U_{j-1} = (vo, wo) is pending on A[j:, j:]
-  for (int j = 0; j + 1 < n; ++j) {
-    // (a) column j gets U_{j-1}: A(r, j) -= vo_r conj(wo_j) + wo_r conj(vo_j), r >= j
-    const double2 voj = vo[j], woj = wo[j];
-    double xn2 = 0.0;
-    for (int r = j + t; r < n; r += nt) {
-      double2 a = A[(int64_t)j * n + r];
-      const double2 vr = vo[r], wr = wo[r];
-      a.x -= vr.x * woj.x + vr.y * woj.y + wr.x * voj.x + wr.y * voj.y;
-      a.y -= vr.y * woj.x - vr.x * woj.y + wr.y * voj.x - wr.x * voj.y;
-      A[(int64_t)j * n + r] = a;
-      if (r >= j + 2) xn2 += a.x * a.x + a.y * a.y;
-    }
-    xn2 = block_sum_big(xn2, red);  // also orders the column writes
-    // (b) reflector of column j
-    const double2 alpha = A[(int64_t)j * n + j + 1];
-    const Reflector R = make_reflector(alpha, xn2);
-    if (t == 0) {
-      d[j] = A[(int64_t)j * n + j].x;
-      e[j] = R.beta;
-      tau[j] = R.tau;
-    }
-    for (int r = j + 1 + t; r < n; r += nt) {
-      double2 vr;
-      if (r == j + 1) {
-        vr = make_double2(1.0, 0.0);
-      } else {
-        vr = cmul2(A[(int64_t)j * n + r], R.scale);
-        A[(int64_t)j * n + r] = vr;  // reflector kept below the subdiagonal
+  for (int j0 = 0; j0 < n - 1; j0 += kTriNB) {
+    const int pb = min(kTriNB, n - 1 - j0);
+    for (int i = 0; i < pb; ++i) {
+      const int j = j0 + i;
+      // (a) column j, rows r >= j, gets the panel's columns l < i
+      double xn2 = 0.0;
+      for (int r = j + t; r < n; r += nt) {
+        double2 a = A[(int64_t)j * n + r];
+        for (int l = 0; l < i; ++l) {
+          a = csub_mul(a, Vp[(int64_t)l * n + r], Wp[(int64_t)l * n + j]);
+          a = csub_mul(a, Wp[(int64_t)l * n + r], Vp[(int64_t)l * n + j]);
+        }
+        A[(int64_t)j * n + r] = a;
+        if (r >= j + 2) xn2 += a.x * a.x + a.y * a.y;
       }
-      vn[r] = vr;
-    }
-    __syncthreads();
-    // (c) fused pass over r, c >= j+1: apply U_{j-1}, accumulate p = A v_j
-    double2 acc[kTriRowsPT], vr_[kTriRowsPT], wr_[kTriRowsPT];
+      xn2 = block_sum_big(xn2, red);  // also orders the column writes
+      // (b) reflector of column j, kept below the subdiagonal
+      const double2 alpha = A[(int64_t)j * n + j + 1];
+      const Reflector R = make_reflector(alpha, xn2);
+      if (t == 0) {
+        d[j] = A[(int64_t)j * n + j].x;
+        e[j] = R.beta;
+        tau[j] = R.tau;
+      }
+      for (int r = j + 1 + t; r < n; r += nt) {
+        double2 vr;
+        if (r == j + 1) {
+          vr = make_double2(1.0, 0.0);
+        } else {
+          vr = cmul2(A[(int64_t)j * n + r], R.scale);
+          A[(int64_t)j * n + r] = vr;
+        }
+        vs[r] = vr;
+      }
+      __syncthreads();
+      // (c) p = A22 v over rows r >= j+1 (A22 as of the panel start)
+      double2 acc[kTriRowsPT];
 #pragma unroll
-    for (int k = 0; k < kTriRowsPT; ++k) {
-      const int r = j + 1 + t + k * nt;
-      acc[k] = make_double2(0.0, 0.0);
-      vr_[k] = r < n ? vo[r] : make_double2(0.0, 0.0);
-      wr_[k] = r < n ? wo[r] : make_double2(0.0, 0.0);
-    }
-    for (int c0 = j + 1; c0 < n; c0 += kTriUnroll) {
+      for (int k = 0; k < kTriRowsPT; ++k) acc[k] = make_double2(0.0, 0.0);
+      for (int c0 = j + 1; c0 < n; c0 += 8) {
+#pragma unroll
+        for (int k = 0; k < kTriRowsPT; ++k) {
+          const int r = j + 1 + t + k * nt;
+          if (r >= n) continue;
+          double2 a[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q)
+            a[q] = c0 + q < n ? A[(int64_t)(c0 + q) * n + r] : make_double2(0.0, 0.0);
+#pragma unroll
+          for (int q = 0; q < 8; ++q)
+            if (c0 + q < n) acc[k] = cfma2(acc[k], a[q], vs[c0 + q]);
+        }
+      }
+      // (d) corrections for the panel's pending updates
+      if (i > 0) {
+        for (int l = 0; l < i; ++l) {
+          double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+#pragma unroll
+          for (int k = 0; k < kTriRowsPT; ++k) {
+            const int r = j + 1 + t + k * nt;
+            if (r >= n) continue;
+            const double2 v = vs[r];
+            const double2 w_ = Wp[(int64_t)l * n + r], v_ = Vp[(int64_t)l * n + r];
+            p0 += w_.x * v.x + w_.y * v.y;  // conj(w_l) v
+            p1 += w_.x * v.y - w_.y * v.x;
+            p2 += v_.x * v.x + v_.y * v.y;  // conj(v_l) v
+            p3 += v_.x * v.y - v_.y * v.x;
+          }
+          for (int o = 32; o >= 1; o >>= 1) {
+            p0 += __shfl_xor(p0, o);
+            p1 += __shfl_xor(p1, o);
+            p2 += __shfl_xor(p2, o);
+            p3 += __shfl_xor(p3, o);
+          }
+          if (lane == 0) {
+            red4[wave][4 * l + 0] = p0;
+            red4[wave][4 * l + 1] = p1;
+            red4[wave][4 * l + 2] = p2;
+            red4[wave][4 * l + 3] = p3;
+          }
+        }
+        __syncthreads();
+        if (t < 2 * i) {
+          double sr = 0.0, si = 0.0;
+          for (int wv = 0; wv < (int)(nt >> 6); ++wv) {
+            sr += red4[wv][2 * t];
+            si += red4[wv][2 * t + 1];
+          }
+          yz[t] = make_double2(sr, si);  // yz[2l] = y_l, yz[2l+1] = z_l
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kTriRowsPT; ++k) {
+          const int r = j + 1 + t + k * nt;
+          if (r >= n) continue;
+          for (int l = 0; l < i; ++l) {
+            const double2 y = yz[2 * l], z = yz[2 * l + 1];
+            const double2 v_ = Vp[(int64_t)l * n + r], w_ = Wp[(int64_t)l * n + r];
+            acc[k].x -= v_.x * y.x - v_.y * y.y + w_.x * z.x - w_.y * z.y;
+            acc[k].y -= v_.x * y.y + v_.y * y.x + w_.x * z.y + w_.y * z.x;
+          }
+        }
+      }
+      // (e) p = tau A v ; w = p - 1/2 tau (p^H v) v ; store the panel's column i
+      double pr = 0.0, pi = 0.0;
 #pragma unroll
       for (int k = 0; k < kTriRowsPT; ++k) {
         const int r = j + 1 + t + k * nt;
-        if (r >= n) continue;
-        double2 a[kTriUnroll];
-#pragma unroll
-        for (int u = 0; u < kTriUnroll; ++u) {
-          const int c = c0 + u;
-          a[u] = c < n ? A[(int64_t)c * n + r] : make_double2(0.0, 0.0);
+        if (r < n) {
+          acc[k] = cmul2(R.tau, acc[k]);
+          const double2 v = vs[r];
+          pr += acc[k].x * v.x + acc[k].y * v.y;
+          pi += acc[k].x * v.y - acc[k].y * v.x;
         }
+      }
+      pr = block_sum_big(pr, red);
+      pi = block_sum_big(pi, red);
+      const double2 a2 = make_double2(-0.5 * (R.tau.x * pr - R.tau.y * pi),
+                                      -0.5 * (R.tau.x * pi + R.tau.y * pr));
 #pragma unroll
-        for (int u = 0; u < kTriUnroll; ++u) {
-          const int c = c0 + u;
-          if (c < n) {
-            const double2 vc = vo[c], wc = wo[c], nv = vn[c];
-            double2 x = a[u];
-            x.x -= vr_[k].x * wc.x + vr_[k].y * wc.y + wr_[k].x * vc.x + wr_[k].y * vc.y;
-            x.y -= vr_[k].y * wc.x - vr_[k].x * wc.y + wr_[k].y * vc.x - wr_[k].x * vc.y;
-            A[(int64_t)c * n + r] = x;
-            acc[k].x += x.x * nv.x - x.y * nv.y;
-            acc[k].y += x.x * nv.y + x.y * nv.x;
+      for (int k = 0; k < kTriRowsPT; ++k) {
+        const int r = j + 1 + t + k * nt;
+        if (r < n) {
+          const double2 v = vs[r];
+          const double2 q = cmul2(a2, v);
+          Wp[(int64_t)i * n + r] = make_double2(acc[k].x + q.x, acc[k].y + q.y);
+          Vp[(int64_t)i * n + r] = v;
+        }
+      }
+      __syncthreads();
+    }
+    // (f) trailing update A(r, c) -= sum_l V(r, l) conj(W(c, l)) + W(r, l) conj(V(c, l)),
+    //     r, c >= jn; rows over threads, columns staged kTriChunk at a time
+    const int jn = j0 + pb;
+    for (int rbase = jn; rbase < n; rbase += nt) {
+      const int r = rbase + t;
+      const bool rv = r < n;
+      double2 vr[kTriNB], wr[kTriNB];
+#pragma unroll
+      for (int l = 0; l < kTriNB; ++l) {
+        vr[l] = (rv && l < pb) ? Vp[(int64_t)l * n + r] : make_double2(0.0, 0.0);
+        wr[l] = (rv && l < pb) ? Wp[(int64_t)l * n + r] : make_double2(0.0, 0.0);
+      }
+      for (int c0 = jn; c0 < n; c0 += kTriChunk) {
+        const int nc = min(kTriChunk, n - c0);
+        __syncthreads();
+        for (int q = t; q < kTriChunk * kTriNB; q += nt) {
+          const int cc = q / kTriNB, l = q % kTriNB;
+          double2 x = make_double2(0.0, 0.0), y = make_double2(0.0, 0.0);
+          if (cc < nc && l < pb) {
+            x = Wp[(int64_t)l * n + c0 + cc];
+            y = Vp[(int64_t)l * n + c0 + cc];
+          }
+          cw[cc][l] = make_double2(x.x, -x.y);
+          cv[cc][l] = make_double2(y.x, -y.y);
+        }
+        __syncthreads();
+        if (rv) {
+          for (int cc = 0; cc < nc; ++cc) {
+            const int64_t o = (int64_t)(c0 + cc) * n + r;
+            double2 a = A[o];
+#pragma unroll
+            for (int l = 0; l < kTriNB; ++l) {
+              const double2 x = cw[cc][l], y = cv[cc][l];
+              a.x -= vr[l].x * x.x - vr[l].y * x.y + wr[l].x * y.x - wr[l].y * y.y;
+              a.y -= vr[l].x * x.y + vr[l].y * x.x + wr[l].x * y.y + wr[l].y * y.x;
+            }
+            A[o] = a;
           }
         }
       }
     }
-    // (d) p = tau A v ; w = p - 1/2 tau (p^H v) v
-    double pr = 0.0, pi = 0.0;
-#pragma unroll
-    for (int k = 0; k < kTriRowsPT; ++k) {
-      const int r = j + 1 + t + k * nt;
-      if (r < n) {
-        acc[k] = cmul2(R.tau, acc[k]);
-        const double2 v = vn[r];
-        pr += acc[k].x * v.x + acc[k].y * v.y;
-        pi += acc[k].x * v.y - acc[k].y * v.x;
-      }
-    }
-    pr = block_sum_big(pr, red);
-    pi = block_sum_big(pi, red);
-    const double2 a2 = make_double2(-0.5 * (R.tau.x * pr - R.tau.y * pi),
-                                    -0.5 * (R.tau.x * pi + R.tau.y * pr));
-#pragma unroll
-    for (int k = 0; k < kTriRowsPT; ++k) {
-      const int r = j + 1 + t + k * nt;
-      if (r < n) {
-        const double2 v = vn[r];
-        const double2 q = cmul2(a2, v);
-        wo[r] = make_double2(acc[k].x + q.x, acc[k].y + q.y);
-        vo[r] = v;
-      }
-    }
-    if (t == 0) { vo[j] = make_double2(0.0, 0.0); wo[j] = make_double2(0.0, 0.0); }
     __syncthreads();
   }
   if (t == 0) {
-    // last diagonal element gets the final pending update
-    const int r = n - 1;
-    double2 a = A[(int64_t)r * n + r];
-    const double2 vr = vo[r], wr = wo[r];
-    a.x -= 2.0 * (vr.x * wr.x + vr.y * wr.y);
-    d[r] = a.x;
-    tau[r] = make_double2(0.0, 0.0);
+    d[n - 1] = A[(int64_t)(n - 1) * n + n - 1].x;
+    tau[n - 1] = make_double2(0.0, 0.0);
   }
 }
 
@@ -466,11 +560,11 @@ int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
   static bool attr = false;
   if (!attr) {
     BSMPO_HIP(hipFuncSetAttribute((const void *)k_tridiag,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 3 * 2048 * 16));
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 2048 * 16));
     attr = true;
   }
   int tk = timing_begin(ST_TRIDIAG, stream);
-  hipLaunchKernelGGL(k_tridiag, dim3(g.n_tri), dim3(kTriThreads), (size_t)3 * g.tri_max_n * 16,
+  hipLaunchKernelGGL(k_tridiag, dim3(g.n_tri), dim3(kTriThreads), (size_t)g.tri_max_n * 16,
                      stream, gdesc(g, w), glist(w, g.o_tri));
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, g.flops_tridiag, g.bytes_tridiag);

@@ -33,6 +33,7 @@ constexpr int32_t kPlanMagic = 0x4253504c;  // "BSPL"
 constexpr int kJacClasses = 5;
 constexpr int kJacLimit[kJacClasses] = {16, 32, 64, 96, 1 << 30};
 constexpr int kTriClass = kJacClasses - 1;
+constexpr int kTriPanel = 8;  // column panel of the blocked Householder reduction
 int eig_threshold();  // sectors with n > threshold take the tridiagonal route
 double tie_rel();     // relative width of the tie band at the chi-th singular value
 
@@ -46,8 +47,8 @@ struct PlanHeader {
   // flops of the SVD stages (complex MAC = 8): Gram 8 m n^2, W 8 n L R,
   // vectors 8 L R n, Jacobi one sweep of 8 n^3 (reported per sweep-equivalent)
   double flops_gram, flops_wmat, flops_vectors, flops_jacobi;
-  // tridiagonal route: Householder flops (16/3) n^3 and the fused pass's
-  // streaming traffic 32 sum_{m<n} m^2 bytes (trailing matrix read + written)
+  // tridiagonal route: Householder flops (16/3) n^3 and the blocked reduction's
+  // trailing-matrix traffic (16 m^2 bytes read per column, 32 m^2 per panel)
   double flops_tridiag, bytes_tridiag;
   // host byte offsets of the device-copied region and total blob size
   int64_t dev_begin, blob_bytes;

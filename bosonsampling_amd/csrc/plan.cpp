@@ -215,7 +215,12 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
         if (c == kTriClass) {
           const double nn = n;
           b.h.flops_tridiag += 16.0 / 3.0 * nn * nn * nn;
-          b.h.bytes_tridiag += 32.0 * (nn - 1.0) * nn * (2.0 * nn - 1.0) / 6.0;
+          // trailing matrix read once per column (A22 v), read + written once per panel
+          for (int j = 0; j + 1 < n; ++j) {
+            const double m = n - 1 - j;
+            b.h.bytes_tridiag += 16.0 * m * m;
+            if (j % kTriPanel == kTriPanel - 1 || j + 2 == n) b.h.bytes_tridiag += 32.0 * m * m;
+          }
           sc.tw_off = tw;
           tw += (int64_t)4 * n * std::min(n, 1024);  // per-thread inverse-iteration arrays
         }

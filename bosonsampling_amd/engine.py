@@ -217,12 +217,15 @@ class GroupRunner:
     call with more work is split into several groups.
     """
 
-    def __init__(self, device="cuda", max_bytes: int = 24 << 30):
+    def __init__(self, device="cuda", max_bytes: int = 24 << 30, use_graph: bool = False):
         self.device = torch.device(device)
         self.max_bytes = int(max_bytes)
+        self.use_graph = bool(use_graph)
         self.work = None
         self.hosts = [None, None]      # pinned host images, used alternately
         self.events = [None, None]     # consumption of each image by the device
+        self.graphs = [None, None]     # graph that uploads each image (use_graph)
+        self.side = None               # capture / replay stream (use_graph)
         self.turn = 0
         self.last_info = {}
 
@@ -237,6 +240,9 @@ class GroupRunner:
         self.turn ^= 1
         if self.events[i] is not None:
             self.events[i].synchronize()   # the image's last upload has been consumed
+        if self.graphs[i] is not None:
+            check(lib().bsmpo_graph_destroy(self.graphs[i]), "graph_destroy")
+            self.graphs[i] = None
         if self.hosts[i] is None or self.hosts[i].numel() < n:
             self.hosts[i] = torch.empty(n, dtype=torch.uint8, pin_memory=True)
         return i, self.hosts[i]
@@ -278,9 +284,23 @@ class GroupRunner:
                                        st.num_lambda.data_ptr(), 1 if mode == st.M - 2 else 0, 0)
             check(lib().bsmpo_group_build(n, arr, io, C.c_void_p(work.data_ptr()),
                                           C.c_void_p(host.data_ptr()), hb.value), "group_build")
-            check(lib().bsmpo_group_run(C.c_void_p(host.data_ptr()), int(precision),
-                                        C.c_void_p(work.data_ptr()), wb.value, stream),
-                  "group_run")
+            if self.use_graph:
+                # the group's upload + launch sequence as one HIP graph, replayed once
+                if self.side is None:
+                    self.side = torch.cuda.Stream(device=self.device)
+                side = C.c_void_p(self.side.cuda_stream)
+                self.side.wait_stream(torch.cuda.current_stream())
+                ex = C.c_void_p()
+                check(lib().bsmpo_group_graph_create(C.c_void_p(host.data_ptr()), int(precision),
+                                                     C.c_void_p(work.data_ptr()), wb.value, side,
+                                                     C.byref(ex)), "group_graph_create")
+                self.graphs[slot] = ex
+                check(lib().bsmpo_graph_launch(ex, side), "graph_launch")
+                torch.cuda.current_stream().wait_stream(self.side)
+            else:
+                check(lib().bsmpo_group_run(C.c_void_p(host.data_ptr()), int(precision),
+                                            C.c_void_p(work.data_ptr()), wb.value, stream),
+                      "group_run")
             ev = torch.cuda.Event()
             ev.record()
             self.events[slot] = ev

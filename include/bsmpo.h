@@ -215,6 +215,20 @@ int bsmpo_group_build(int n_upd, const void *const *h_plans, const bsmpo_update_
 int bsmpo_group_run(const void *h_group, int precision, void *d_work, size_t work_bytes,
                     void *stream);
 
+/*
+ * A group run captured as a HIP graph: the image upload and the fixed launch
+ * sequence of bsmpo_group_run, recorded on `stream` (a non-default stream,
+ * timing disabled) and instantiated.  bsmpo_graph_launch replays it on any
+ * stream; the image upload reads h_group at replay time, so rebuilding the
+ * same group (same plans, same d_work) into h_group and replaying repeats
+ * the run on the buffers the image names.  No reference counterpart (the
+ * reference launches per sector, src/beam_splitter_parallel/Rank.py:90-127).
+ */
+int bsmpo_group_graph_create(const void *h_group, int precision, void *d_work, size_t work_bytes,
+                             void *stream, void **graph_exec);
+int bsmpo_graph_launch(void *graph_exec, void *stream);
+int bsmpo_graph_destroy(void *graph_exec);
+
 /* --------------------------------------------------------- observables -- */
 
 /*

@@ -132,3 +132,94 @@ def _lowp_run(prec, tol, M, d, chi, inputs, eta):
     err_t = abs(tr - mpo.trace(ref))
     print(f"prec={prec}: marginal error {err_m:.3e}, trace error {err_t:.3e}")
     assert err_m <= tol and err_t <= tol
+
+
+def test_group_graph_replays_match_eager(gpu):
+    """One layer's group captured as a HIP graph and replayed three times on
+    restored inputs, with eager work and synchronisation in between; every
+    replay equals the eager group run."""
+    import torch
+    from bosonsampling_amd import _capi, engine
+    from bosonsampling_amd._capi import check, lib
+    M, d, chi, inputs, eta = CASE
+    meshes = _meshes(M, [6, 7])
+    ref = mpo.initialize(M, d, chi, inputs, eta)
+    mpo.simulate(ref, meshes[0], inputs, layers=range(2))
+    layer = 2
+    todo = engine.layer_modes(M, layer, max(inputs))
+
+    def fresh():
+        return [engine.DeviceMPO.from_host(ref.Gamma, ref.Lambda, ref.charge, d) for _ in meshes]
+
+    def updates(states):
+        out = []
+        for st, mm in zip(states, meshes):
+            U3 = engine.bs_tables(d, np.stack([mm[layer, i] for i, _ in todo]), st.device)
+            out += [(st, mode, U3[k]) for k, (i, mode) in enumerate(todo)]
+        return out
+
+    eager = fresh()
+    engine.GroupRunner().run(updates(eager))
+    want = [st.to_host() for st in eager]
+
+    states = fresh()
+    saved = [(st.Gamma.clone(), st.Lambda.clone(), st.charge.clone()) for st in states]
+    ups = updates(states)
+    plans = []
+    for st, mode, U3 in ups:
+        ch = st.charge_host
+        plans.append(engine.Plan(d, chi, ch[mode], ch[mode + 1], ch[mode + 2]))
+    n = len(plans)
+    arr = (C.c_void_p * n)(*[p.address for p in plans])
+    hb, wb = C.c_size_t(), C.c_size_t()
+    check(lib().bsmpo_group_query(n, arr, C.byref(hb), C.byref(wb)), "group_query")
+    work = torch.empty(wb.value, dtype=torch.uint8, device=gpu)
+    host = torch.empty(hb.value, dtype=torch.uint8, pin_memory=True)
+    io = (_capi.UpdateIO * n)()
+    for j, (st, mode, U3) in enumerate(ups):
+        LR = st.edge if mode == M - 2 else st.Lambda[mode + 1]
+        io[j] = _capi.UpdateIO(st.Gamma[mode].data_ptr(), st.Gamma[mode + 1].data_ptr(),
+                               LR.data_ptr(), U3.data_ptr(), st.charge[mode + 1].data_ptr(),
+                               st.Lambda[mode].data_ptr(), st.num_lambda.data_ptr(),
+                               1 if mode == M - 2 else 0, 0)
+    check(lib().bsmpo_group_build(n, arr, io, C.c_void_p(work.data_ptr()),
+                                  C.c_void_p(host.data_ptr()), hb.value), "group_build")
+    side = torch.cuda.Stream(device=gpu)
+    side.wait_stream(torch.cuda.current_stream())
+    ex = C.c_void_p()
+    check(lib().bsmpo_group_graph_create(C.c_void_p(host.data_ptr()), 0, C.c_void_p(work.data_ptr()),
+                                         wb.value, C.c_void_p(side.cuda_stream), C.byref(ex)),
+          "group_graph_create")
+    try:
+        for rep in range(3):
+            print(f"graph replay {rep}", flush=True)
+            for st, (G, L, ch) in zip(states, saved):
+                st.Gamma.copy_(G)
+                st.Lambda.copy_(L)
+                st.charge.copy_(ch)
+            side.wait_stream(torch.cuda.current_stream())
+            check(lib().bsmpo_graph_launch(ex, C.c_void_p(side.cuda_stream)), "graph_launch")
+            torch.cuda.current_stream().wait_stream(side)
+            torch.cuda.synchronize()
+            for st, (Gw, Lw, chw) in zip(states, want):
+                G, L, ch = st.to_host()
+                assert np.array_equal(ch, chw)
+                assert np.abs(L - Lw).max() <= 1e-12 * np.abs(Lw).max()
+                assert np.abs(G - Gw).max() <= 1e-12 * np.abs(Gw).max()
+            _ = engine.GroupRunner().run(updates(fresh()))  # eager work between replays
+    finally:
+        check(lib().bsmpo_graph_destroy(ex), "graph_destroy")
+
+
+def test_group_runner_with_graphs_matches_oracle(gpu):
+    from bosonsampling_amd import engine
+    M, d, chi, inputs, eta = CASE
+    meshes = _meshes(M, [8, 9])
+    ref = mpo.initialize(M, d, chi, inputs, eta)
+    batch = [engine.DeviceMPO.from_host(ref.Gamma, ref.Lambda, ref.charge, d) for _ in meshes]
+    engine.simulate_batch(batch, meshes, inputs, runner=engine.GroupRunner(use_graph=True))
+    for st, mm in zip(batch, meshes):
+        orc = ref.copy()
+        mpo.simulate(orc, mm, inputs)
+        mo = mpo.marginals(orc)
+        assert np.abs(st.observables()[1] - mo).max() <= 1e-10 * np.abs(mo).max()
