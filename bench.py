@@ -61,6 +61,8 @@ def parse():
                     help="budget of the CPU-oracle baseline sample (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB read and written by one elementwise kernel run first (PMC byte calibration)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="time the eigensolver route split on the last warmup layer (grouped path)")
     ap.add_argument("--verbose", action="store_true")
     return ap.parse_args()
 
@@ -193,7 +195,16 @@ def main():
         states = [init.clone() for _ in range(B)]
         runner = engine.GroupRunner(init.device, max_bytes=int(args.group_gib * (1 << 30)))
         t1 = time.time()
+        tuned = None
         for layer in range(W):
+            if args.autotune and layer == W - 1:
+                # tune the eigensolver route split on the last warmup layer (clones)
+                best, times = engine.autotune_eig_threshold(states, meshes, inputs, layer,
+                                                            precision=prec, runner=runner)
+                tuned = {"layer": layer, "eig_threshold": best,
+                         "seconds": {str(k): round(v, 3) for k, v in times.items()}}
+                log(f"[bench] autotuned eigensolver split on layer {layer}: threshold {best} "
+                    f"{times}")
             engine.simulate_batch(states, meshes, inputs, layers=[layer], runner=runner)
             torch.cuda.synchronize()
             if args.verbose:
@@ -215,7 +226,7 @@ def main():
         st_grp = _capi.timing_read(reset=True)
         _capi.timing_enable(False)
         result["grouped"] = {"updates": n, "seconds": el, "rate": n / el,
-                             "trace0": states[0].observables()[0]}
+                             "trace0": states[0].observables()[0], "tuned": tuned}
         log(f"[bench] grouped {args.precision}: {n} updates in {el:.3f}s = {n / el:.2f} updates/s")
         stage_table(st_grp)
         roof = roofline(st_grp, args.precision)
@@ -248,7 +259,7 @@ def main():
         "config": {
             "workload": workload,
             "grouped": ({"instances": B, "precision": args.precision, "updates": g["updates"],
-                         "updates_per_s": g["rate"]} if g else None),
+                         "updates_per_s": g["rate"], "autotune": g.get("tuned")} if g else None),
             "plain_fp64": ({"instances": 1, "updates": p["updates"],
                             "updates_per_s": p["rate"]} if p else None),
             "grouped_over_plain": (g["rate"] / p["rate"]) if (g and p) else None,

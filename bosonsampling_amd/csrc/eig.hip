@@ -13,6 +13,8 @@
 #include <float.h>
 #include <math.h>
 
+#include <algorithm>
+
 #include "common.h"
 
 namespace bsmpo {
@@ -56,9 +58,9 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// Householder tridiagonalisation G = Q T Q^H, one block of 512 threads per
-// sector, blocked by panels of kTriNB columns (the LAPACK zhetrd / zlatrd
-// organisation, restated):
+// Householder tridiagonalisation G = Q T Q^H, one block per sector, blocked
+// by panels of kTriNB columns (the LAPACK zhetrd / zlatrd organisation,
+// restated):
 //   column j of a panel: apply the panel's earlier reflectors to column j,
 //   form its reflector v_j, then w_j from p = A22 v_j, where A22 is the
 //   trailing matrix as of the panel start and the panel's pending rank-2
@@ -66,22 +68,13 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
 //   after the panel: one pass A22 -= V W^H + W V^H over the trailing matrix.
 // Per column the trailing matrix is read once (the product A22 v); it is
 // written once per panel.  V and W of the panel live in the sector's
-// tridiagonal work area (n x kTriNB each, column-major).
-constexpr int kTriThreads = 512;
-constexpr int kTriRowsPT = 4;  // rows per thread: n <= 2048
+// tridiagonal work area (n x kTriNB each, column-major).  Five barriers per
+// column: block reductions alternate between two LDS buffers, so each costs
+// one barrier.  Two block shapes: 256 threads x 2 rows (n <= 512) and
+// 512 threads x 4 rows (n <= 2048); each launch skips the other size class.
 constexpr int kTriNB = kTriPanel;
 constexpr int kTriChunk = 64;  // trailing-update columns staged in LDS at a time
-
-__device__ double block_sum_big(double v, double *red) {
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double t = 0.0;
-  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
-  __syncthreads();
-  return t;
-}
+constexpr int kTriMaxN = 2048;
 
 __device__ __forceinline__ double2 csub_mul(double2 a, double2 x, double2 y) {  // a - x conj(y)
   a.x -= x.x * y.x + x.y * y.y;
@@ -94,24 +87,45 @@ __device__ __forceinline__ double2 cfma2(double2 acc, double2 a, double2 b) {  /
   return acc;
 }
 
-__global__ __launch_bounds__(kTriThreads) void k_tridiag(const UpdDesc *__restrict__ Ud,
-                                                         const int2 *__restrict__ list) {
+template <int NW>
+__device__ __forceinline__ double2 block_sum2(double a, double b, double2 (*red)[16], int &par) {
+  for (int o = 32; o >= 1; o >>= 1) {
+    a += __shfl_xor(a, o);
+    b += __shfl_xor(b, o);
+  }
+  if ((threadIdx.x & 63) == 0) red[par][threadIdx.x >> 6] = make_double2(a, b);
+  __syncthreads();
+  double2 t = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int i = 0; i < NW; ++i) {
+    t.x += red[par][i].x;
+    t.y += red[par][i].y;
+  }
+  par ^= 1;
+  return t;
+}
+
+template <int NT, int RPT>
+__global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
+                                                const int2 *__restrict__ list, int nmin, int nmax) {
+  constexpr int NW = NT / 64;
   extern __shared__ double2 tsm[];
-  __shared__ double red[16];
-  __shared__ double red4[16][4 * kTriNB];
-  __shared__ double2 yz[2 * kTriNB];                   // y_l = W_l^H v, z_l = V_l^H v
+  __shared__ double2 red[2][16];
+  __shared__ double red4[NW][4 * kTriNB];
   __shared__ double2 cw[kTriChunk][kTriNB], cv[kTriChunk][kTriNB];  // conj(W(c, l)), conj(V(c, l))
   const int2 ref = list[blockIdx.x];
   const UpdDesc &u = Ud[ref.x];
   const Sector s = u.sectors[ref.y];
   const int n = s.n;
+  if (n < nmin || n > nmax) return;  // the other block shape's sector
   double2 *A = reinterpret_cast<double2 *>(u.G) + s.g_off;  // A(r, c) = A[c n + r]
   double2 *vs = tsm;                                         // v_j, n entries
   double2 *Vp = reinterpret_cast<double2 *>(u.triwork + s.tw_off);
   double2 *Wp = Vp + (int64_t)n * kTriNB;
   double *d = u.td + s.s_off, *e = u.te + s.s_off;
   double2 *tau = reinterpret_cast<double2 *>(u.tau) + s.s_off;
-  const int t = threadIdx.x, nt = blockDim.x, lane = t & 63, wave = t >> 6;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  int par = 0;
   if (n == 1) {
     if (t == 0) { d[0] = A[0].x; tau[0] = make_double2(0.0, 0.0); }
     return;
@@ -122,7 +136,7 @@ __global__ __launch_bounds__(kTriThreads) void k_tridiag(const UpdDesc *__restri
       const int j = j0 + i;
       // (a) column j, rows r >= j, gets the panel's columns l < i
       double xn2 = 0.0;
-      for (int r = j + t; r < n; r += nt) {
+      for (int r = j + t; r < n; r += NT) {
         double2 a = A[(int64_t)j * n + r];
         for (int l = 0; l < i; ++l) {
           a = csub_mul(a, Vp[(int64_t)l * n + r], Wp[(int64_t)l * n + j]);
@@ -131,7 +145,7 @@ __global__ __launch_bounds__(kTriThreads) void k_tridiag(const UpdDesc *__restri
         A[(int64_t)j * n + r] = a;
         if (r >= j + 2) xn2 += a.x * a.x + a.y * a.y;
       }
-      xn2 = block_sum_big(xn2, red);  // also orders the column writes
+      xn2 = block_sum2<NW>(xn2, 0.0, red, par).x;  // also orders the column writes
       // (b) reflector of column j, kept below the subdiagonal
       const double2 alpha = A[(int64_t)j * n + j + 1];
       const Reflector R = make_reflector(alpha, xn2);
@@ -140,7 +154,7 @@ __global__ __launch_bounds__(kTriThreads) void k_tridiag(const UpdDesc *__restri
         e[j] = R.beta;
         tau[j] = R.tau;
       }
-      for (int r = j + 1 + t; r < n; r += nt) {
+      for (int r = j + 1 + t; r < n; r += NT) {
         double2 vr;
         if (r == j + 1) {
           vr = make_double2(1.0, 0.0);
@@ -152,13 +166,13 @@ __global__ __launch_bounds__(kTriThreads) void k_tridiag(const UpdDesc *__restri
       }
       __syncthreads();
       // (c) p = A22 v over rows r >= j+1 (A22 as of the panel start)
-      double2 acc[kTriRowsPT];
+      double2 acc[RPT];
 #pragma unroll
-      for (int k = 0; k < kTriRowsPT; ++k) acc[k] = make_double2(0.0, 0.0);
+      for (int k = 0; k < RPT; ++k) acc[k] = make_double2(0.0, 0.0);
       for (int c0 = j + 1; c0 < n; c0 += 8) {
 #pragma unroll
-        for (int k = 0; k < kTriRowsPT; ++k) {
-          const int r = j + 1 + t + k * nt;
+        for (int k = 0; k < RPT; ++k) {
+          const int r = j + 1 + t + k * NT;
           if (r >= n) continue;
           double2 a[8];
 #pragma unroll
@@ -174,8 +188,8 @@ __global__ __launch_bounds__(kTriThreads) void k_tridiag(const UpdDesc *__restri
         for (int l = 0; l < i; ++l) {
           double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
 #pragma unroll
-          for (int k = 0; k < kTriRowsPT; ++k) {
-            const int r = j + 1 + t + k * nt;
+          for (int k = 0; k < RPT; ++k) {
+            const int r = j + 1 + t + k * NT;
             if (r >= n) continue;
             const double2 v = vs[r];
             const double2 w_ = Wp[(int64_t)l * n + r], v_ = Vp[(int64_t)l * n + r];
@@ -198,21 +212,19 @@ __global__ __launch_bounds__(kTriThreads) void k_tridiag(const UpdDesc *__restri
           }
         }
         __syncthreads();
-        if (t < 2 * i) {
-          double sr = 0.0, si = 0.0;
-          for (int wv = 0; wv < (int)(nt >> 6); ++wv) {
-            sr += red4[wv][2 * t];
-            si += red4[wv][2 * t + 1];
-          }
-          yz[t] = make_double2(sr, si);  // yz[2l] = y_l, yz[2l+1] = z_l
-        }
-        __syncthreads();
+        for (int l = 0; l < i; ++l) {
+          double2 y = make_double2(0.0, 0.0), z = make_double2(0.0, 0.0);  // W_l^H v, V_l^H v
 #pragma unroll
-        for (int k = 0; k < kTriRowsPT; ++k) {
-          const int r = j + 1 + t + k * nt;
-          if (r >= n) continue;
-          for (int l = 0; l < i; ++l) {
-            const double2 y = yz[2 * l], z = yz[2 * l + 1];
+          for (int wv = 0; wv < NW; ++wv) {
+            y.x += red4[wv][4 * l + 0];
+            y.y += red4[wv][4 * l + 1];
+            z.x += red4[wv][4 * l + 2];
+            z.y += red4[wv][4 * l + 3];
+          }
+#pragma unroll
+          for (int k = 0; k < RPT; ++k) {
+            const int r = j + 1 + t + k * NT;
+            if (r >= n) continue;
             const double2 v_ = Vp[(int64_t)l * n + r], w_ = Wp[(int64_t)l * n + r];
             acc[k].x -= v_.x * y.x - v_.y * y.y + w_.x * z.x - w_.y * z.y;
             acc[k].y -= v_.x * y.y + v_.y * y.x + w_.x * z.y + w_.y * z.x;
@@ -222,8 +234,8 @@ __global__ __launch_bounds__(kTriThreads) void k_tridiag(const UpdDesc *__restri
       // (e) p = tau A v ; w = p - 1/2 tau (p^H v) v ; store the panel's column i
       double pr = 0.0, pi = 0.0;
 #pragma unroll
-      for (int k = 0; k < kTriRowsPT; ++k) {
-        const int r = j + 1 + t + k * nt;
+      for (int k = 0; k < RPT; ++k) {
+        const int r = j + 1 + t + k * NT;
         if (r < n) {
           acc[k] = cmul2(R.tau, acc[k]);
           const double2 v = vs[r];
@@ -231,13 +243,12 @@ __global__ __launch_bounds__(kTriThreads) void k_tridiag(const UpdDesc *__restri
           pi += acc[k].x * v.y - acc[k].y * v.x;
         }
       }
-      pr = block_sum_big(pr, red);
-      pi = block_sum_big(pi, red);
-      const double2 a2 = make_double2(-0.5 * (R.tau.x * pr - R.tau.y * pi),
-                                      -0.5 * (R.tau.x * pi + R.tau.y * pr));
+      const double2 ph = block_sum2<NW>(pr, pi, red, par);
+      const double2 a2 = make_double2(-0.5 * (R.tau.x * ph.x - R.tau.y * ph.y),
+                                      -0.5 * (R.tau.x * ph.y + R.tau.y * ph.x));
 #pragma unroll
-      for (int k = 0; k < kTriRowsPT; ++k) {
-        const int r = j + 1 + t + k * nt;
+      for (int k = 0; k < RPT; ++k) {
+        const int r = j + 1 + t + k * NT;
         if (r < n) {
           const double2 v = vs[r];
           const double2 q = cmul2(a2, v);
@@ -250,7 +261,7 @@ __global__ __launch_bounds__(kTriThreads) void k_tridiag(const UpdDesc *__restri
     // (f) trailing update A(r, c) -= sum_l V(r, l) conj(W(c, l)) + W(r, l) conj(V(c, l)),
     //     r, c >= jn; rows over threads, columns staged kTriChunk at a time
     const int jn = j0 + pb;
-    for (int rbase = jn; rbase < n; rbase += nt) {
+    for (int rbase = jn; rbase < n; rbase += NT) {
       const int r = rbase + t;
       const bool rv = r < n;
       double2 vr[kTriNB], wr[kTriNB];
@@ -262,7 +273,7 @@ __global__ __launch_bounds__(kTriThreads) void k_tridiag(const UpdDesc *__restri
       for (int c0 = jn; c0 < n; c0 += kTriChunk) {
         const int nc = min(kTriChunk, n - c0);
         __syncthreads();
-        for (int q = t; q < kTriChunk * kTriNB; q += nt) {
+        for (int q = t; q < kTriChunk * kTriNB; q += NT) {
           const int cc = q / kTriNB, l = q % kTriNB;
           double2 x = make_double2(0.0, 0.0), y = make_double2(0.0, 0.0);
           if (cc < nc && l < pb) {
@@ -555,18 +566,25 @@ static inline const int2 *glist(const char *w, int64_t off) {
 
 int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
   if (g.n_tri == 0) return 0;
-  if (g.tri_max_n > kTriThreads * kTriRowsPT)
-    return fail(-1, "sector too large for the tridiagonal route (n > 2048)");
+  if (g.tri_max_n > kTriMaxN) return fail(-1, "sector too large for the tridiagonal route (n > 2048)");
   static bool attr = false;
   if (!attr) {
-    BSMPO_HIP(hipFuncSetAttribute((const void *)k_tridiag,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 2048 * 16));
+    BSMPO_HIP(hipFuncSetAttribute((const void *)k_tridiag<256, 2>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 512 * 16));
+    BSMPO_HIP(hipFuncSetAttribute((const void *)k_tridiag<512, 4>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, kTriMaxN * 16));
     attr = true;
   }
   int tk = timing_begin(ST_TRIDIAG, stream);
-  hipLaunchKernelGGL(k_tridiag, dim3(g.n_tri), dim3(kTriThreads), (size_t)g.tri_max_n * 16,
-                     stream, gdesc(g, w), glist(w, g.o_tri));
+  hipLaunchKernelGGL((k_tridiag<256, 2>), dim3(g.n_tri), dim3(256),
+                     (size_t)std::min(g.tri_max_n, 512) * 16, stream, gdesc(g, w),
+                     glist(w, g.o_tri), 1, 512);
   BSMPO_HIP(hipGetLastError());
+  if (g.tri_max_n > 512) {
+    hipLaunchKernelGGL((k_tridiag<512, 4>), dim3(g.n_tri), dim3(512), (size_t)g.tri_max_n * 16,
+                       stream, gdesc(g, w), glist(w, g.o_tri), 513, kTriMaxN);
+    BSMPO_HIP(hipGetLastError());
+  }
   timing_end(tk, stream, g.flops_tridiag, g.bytes_tridiag);
   tk = timing_begin(ST_BISECT, stream);
   hipLaunchKernelGGL(k_bisect, dim3(g.n_tri, (g.tri_max_n + 255) / 256), dim3(256), 0, stream,

@@ -340,6 +340,31 @@ def simulate_batch(states, meshes, input_modes, layers=None, precision=C128, run
 
 # ------------------------------------------------------------ initial state
 
+def autotune_eig_threshold(states, meshes, input_modes, layer, precision=C128, runner=None,
+                           candidates=(0, 32, 64, 96)):
+    """Pick the split between the two sector eigensolver routes (one-sided
+    Jacobi in LDS for Gram size n <= threshold, the tridiagonal route above)
+    by timing one grouped layer per candidate on clones of `states`, and set
+    the fastest with bsmpo_set_eig_threshold.  Every candidate gives the same
+    results to rounding; only the time differs.  Returns (best, {thr: s})."""
+    import time
+    runner = runner or GroupRunner(states[0].device)
+    times = {}
+    for thr in candidates:
+        check(lib().bsmpo_set_eig_threshold(int(thr)), "set_eig_threshold")
+        clones = [st.clone() for st in states]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        simulate_batch(clones, meshes, input_modes, layers=[layer], precision=precision,
+                       runner=runner)
+        torch.cuda.synchronize()
+        times[int(thr)] = time.perf_counter() - t0
+        del clones
+    best = min(times, key=times.get)
+    check(lib().bsmpo_set_eig_threshold(best), "set_eig_threshold")
+    return best, times
+
+
 def input_density(d: int, eta: float) -> np.ndarray:
     """Single photon through a pure-loss channel of transmission eta:
     (1 - eta)|0><0| + eta|1><1| (src/MPO.py:41-44 with loss = 1 - eta)."""
