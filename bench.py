@@ -81,7 +81,7 @@ def stage_table(stages):
 HBM_BOUND = ("topk", "scatter", "observables", "bs_table", "tridiag")
 
 
-def measured_traffic(kernel, workload, instances, precision):
+def measured_traffic(kernel, workload, instances, precision, eig_threshold):
     """HBM bytes per launch of `kernel` measured on this workload by two
     rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE, scaled by a probe kernel
     of known size) and recorded in profiles/<kernel>_*_traffic.json; None if
@@ -90,8 +90,8 @@ def measured_traffic(kernel, workload, instances, precision):
     for path in sorted(glob.glob(os.path.join(HERE, "profiles", f"{kernel}_*_traffic.json"))):
         with open(path) as f:
             rec = json.load(f)
-        if (rec.get("kernel"), rec.get("workload"), rec.get("instances"), rec.get("precision")) == \
-                (kernel, workload, instances, precision):
+        if (rec.get("kernel"), rec.get("workload"), rec.get("instances"), rec.get("precision"),
+                rec.get("eig_threshold")) == (kernel, workload, instances, precision, eig_threshold):
             return rec["traffic_bytes_per_launch"], os.path.relpath(path, HERE)
     return None, None
 
@@ -183,7 +183,7 @@ def main():
         st_plain = _capi.timing_read(reset=True)
         _capi.timing_enable(False)
         result["plain"] = {"updates": n, "seconds": el, "rate": n / el,
-                           "trace": st.observables()[0]}
+                           "observables": st.observables()}
         log(f"[bench] plain c128: {n} updates in {el:.3f}s = {n / el:.2f} updates/s")
         stage_table(st_plain)
         del st
@@ -205,7 +205,8 @@ def main():
                          "seconds": {str(k): round(v, 3) for k, v in times.items()}}
                 log(f"[bench] autotuned eigensolver split on layer {layer}: threshold {best} "
                     f"{times}")
-            engine.simulate_batch(states, meshes, inputs, layers=[layer], runner=runner)
+            engine.simulate_batch(states, meshes, inputs, layers=[layer], runner=runner,
+                                  precision=prec)
             torch.cuda.synchronize()
             if args.verbose:
                 log(f"[bench] grouped warmup layer {layer}: {time.time() - t1:.1f}s "
@@ -226,7 +227,7 @@ def main():
         st_grp = _capi.timing_read(reset=True)
         _capi.timing_enable(False)
         result["grouped"] = {"updates": n, "seconds": el, "rate": n / el,
-                             "trace0": states[0].observables()[0], "tuned": tuned}
+                             "observables0": states[0].observables(), "tuned": tuned}
         log(f"[bench] grouped {args.precision}: {n} updates in {el:.3f}s = {n / el:.2f} updates/s")
         stage_table(st_grp)
         roof = roofline(st_grp, args.precision)
@@ -235,12 +236,14 @@ def main():
     workload = (f"lossy boson-sampling MPO site updates, M={M} N={N} "
                 f"(modes 0,{args.stride},..) d={d} chi={chi} eta={args.eta}; "
                 f"layers {W}..{W + K - 1} timed")
-    if roof is not None:
-        roof["traffic"], src = measured_traffic(roof["kernel"], workload, B, args.precision)
-        if src:
-            roof["traffic_source"] = src
     g = result.get("grouped")
     p = result.get("plain")
+    if roof is not None:
+        thr = (g or {}).get("tuned") or {}
+        roof["traffic"], src = measured_traffic(roof["kernel"], workload, B, args.precision,
+                                                thr.get("eig_threshold", 96))
+        if src:
+            roof["traffic_source"] = src
     main_path = g or p
     out = {
         "metric": METRIC,
@@ -263,11 +266,25 @@ def main():
             "plain_fp64": ({"instances": 1, "updates": p["updates"],
                             "updates_per_s": p["rate"]} if p else None),
             "grouped_over_plain": (g["rate"] / p["rate"]) if (g and p) else None,
+            "precision_check": precision_check(g, p, args.precision, W + K),
         },
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def precision_check(g, p, precision, layers):
+    """Instance 0 of the grouped path (every layer in `precision`) against the
+    plain complex128 path on the same mesh: output photon-number marginals
+    P(n_j = n) and trace after the same layers."""
+    if not (g and p):
+        return None
+    tg, mg = g["observables0"]
+    tp, mp = p["observables"]
+    return {"precision": precision, "layers": layers, "reference": "plain complex128, same mesh",
+            "marginal_max_abs_err": float(np.abs(mg - mp).max()),
+            "trace_abs_err": float(abs(tg - tp)), "trace": float(tp)}
 
 
 def cpu_baseline(snap, d, chi, mm, inputs, layer, budget_s):

@@ -92,6 +92,10 @@ struct UpdDesc {
   int32_t d, chi, edge, n_sectors, n_cblk, pad0;
   int64_t sv_total;
   double tie_rel;
+  // pair formulation of the contraction
+  const PairItem *pitems;
+  const int32_t *tau_sector, *cblk_of, *rowoff, *coloff;
+  int32_t n_lblk, n_rblk;
 };
 
 // Plan arrays as seen by kernels (device copy at the front of the workspace).

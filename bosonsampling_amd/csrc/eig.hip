@@ -75,6 +75,7 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
 constexpr int kTriNB = kTriPanel;
 constexpr int kTriChunk = 64;  // trailing-update columns staged in LDS at a time
 constexpr int kTriMaxN = 2048;
+constexpr int kTriFewSectors = 256;  // below this many sectors per launch: one block shape
 
 __device__ __forceinline__ double2 csub_mul(double2 a, double2 x, double2 y) {  // a - x conj(y)
   a.x -= x.x * y.x + x.y * y.y;
@@ -576,13 +577,19 @@ int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
     attr = true;
   }
   int tk = timing_begin(ST_TRIDIAG, stream);
-  hipLaunchKernelGGL((k_tridiag<256, 2>), dim3(g.n_tri), dim3(256),
-                     (size_t)std::min(g.tri_max_n, 512) * 16, stream, gdesc(g, w),
-                     glist(w, g.o_tri), 1, 512);
-  BSMPO_HIP(hipGetLastError());
-  if (g.tri_max_n > 512) {
+  // Few sectors (a single update): the larger block shape for every sector, so
+  // that each sector's column chain gets more threads.  Many sectors (a group
+  // across instances): 256-thread blocks for n <= 512, more blocks per CU.
+  const int split = g.n_tri < kTriFewSectors ? 0 : 512;
+  if (split > 0) {
+    hipLaunchKernelGGL((k_tridiag<256, 2>), dim3(g.n_tri), dim3(256),
+                       (size_t)std::min(g.tri_max_n, 512) * 16, stream, gdesc(g, w),
+                       glist(w, g.o_tri), 1, split);
+    BSMPO_HIP(hipGetLastError());
+  }
+  if (g.tri_max_n > split) {
     hipLaunchKernelGGL((k_tridiag<512, 4>), dim3(g.n_tri), dim3(512), (size_t)g.tri_max_n * 16,
-                       stream, gdesc(g, w), glist(w, g.o_tri), 513, kTriMaxN);
+                       stream, gdesc(g, w), glist(w, g.o_tri), split + 1, kTriMaxN);
     BSMPO_HIP(hipGetLastError());
   }
   timing_end(tk, stream, g.flops_tridiag, g.bytes_tridiag);

@@ -24,11 +24,15 @@ int layout(int n, const void *const *plans, Layout &L) {
   std::memset(&g, 0, sizeof(g));
   g.magic = kGroupMagic;
   g.n_upd = n;
-  int64_t n_items = 0, n_gram = 0, n_tri = 0, n_sec = 0, n_v = 0, n_w = 0, n_jac = 0;
+  int64_t n_items = 0, n_gram = 0, n_tri = 0, n_sec = 0, n_v = 0, n_w = 0, n_jac = 0, n_pit = 0;
+  g.pair_ok = 1;
   for (int u = 0; u < n; ++u) {
     const PlanHeader *h = header(plans[u]);
     if (!h || h->magic != kPlanMagic) return fail(-1, "group: entry is not a bsmpo plan");
     n_items += h->n_items;
+    n_pit += h->n_pitems;
+    if (!h->pair_ok) g.pair_ok = 0;
+    g.flops_pair_weight += h->flops_pair_weight;
     n_gram += h->n_gram_items;
     n_v += h->n_v_items;
     n_w += h->n_w_items;
@@ -53,9 +57,11 @@ int layout(int n, const void *const *plans, Layout &L) {
     g.flops_tridiag += h->flops_tridiag;
     g.bytes_tridiag += h->bytes_tridiag;
   }
-  if (n_items > INT32_MAX || n_gram > INT32_MAX || n_w > INT32_MAX || n_v > INT32_MAX)
+  if (n_items > INT32_MAX || n_gram > INT32_MAX || n_w > INT32_MAX || n_v > INT32_MAX ||
+      n_pit > INT32_MAX)
     return fail(-1, "group: too many work items");
   g.n_items = (int32_t)n_items;
+  g.n_pitems = (int32_t)n_pit;
   g.n_gram = (int32_t)n_gram;
   g.n_tri = (int32_t)n_tri;
   g.n_sec = (int32_t)n_sec;
@@ -69,6 +75,7 @@ int layout(int n, const void *const *plans, Layout &L) {
   };
   g.o_desc = place((int64_t)sizeof(UpdDesc) * n);
   g.o_items = place(8 * n_items);
+  g.o_pitems = place(8 * n_pit);
   g.o_gram = place(8 * n_gram);
   g.o_jac = place(8 * n_jac);
   g.o_tri = place(8 * n_tri);
@@ -110,13 +117,14 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
   std::memset(img, 0, (size_t)g.staging_bytes);
   UpdDesc *desc = reinterpret_cast<UpdDesc *>(img + g.o_desc);
   int2 *items = reinterpret_cast<int2 *>(img + g.o_items);
+  int2 *pit = reinterpret_cast<int2 *>(img + g.o_pitems);
   int2 *gram = reinterpret_cast<int2 *>(img + g.o_gram);
   int2 *jac = reinterpret_cast<int2 *>(img + g.o_jac);
   int2 *tri = reinterpret_cast<int2 *>(img + g.o_tri);
   int2 *sec = reinterpret_cast<int2 *>(img + g.o_sec);
   int2 *vl = reinterpret_cast<int2 *>(img + g.o_v);
   int2 *wl = reinterpret_cast<int2 *>(img + g.o_w);
-  int64_t ni = 0, ng = 0, nt = 0, ns = 0, nv = 0, nw = 0;
+  int64_t ni = 0, ng = 0, nt = 0, ns = 0, nv = 0, nw = 0, np = 0;
   for (int u = 0; u < n; ++u) {
     const PlanHeader *h = header(plans[u]);
     std::memcpy(img + L.plan_off[u], reinterpret_cast<const char *>(plans[u]) + h->dev_begin,
@@ -167,7 +175,15 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
     D.n_cblk = h->n_cblk;
     D.sv_total = h->sv_total;
     D.tie_rel = h->tie_rel;
+    D.pitems = reinterpret_cast<const PairItem *>(pb + h->o_pitems);
+    D.tau_sector = reinterpret_cast<const int32_t *>(pb + h->o_tau_sector);
+    D.cblk_of = reinterpret_cast<const int32_t *>(pb + h->o_cblk_of);
+    D.rowoff = reinterpret_cast<const int32_t *>(pb + h->o_rowoff);
+    D.coloff = reinterpret_cast<const int32_t *>(pb + h->o_coloff);
+    D.n_lblk = h->n_lblk;
+    D.n_rblk = h->n_rblk;
     for (int i = 0; i < h->n_items; ++i) items[ni++] = make_int2(u, i);
+    for (int i = 0; i < h->n_pitems; ++i) pit[np++] = make_int2(u, i);
     for (int i = 0; i < h->n_gram_items; ++i) gram[ng++] = make_int2(u, i);
     for (int i = 0; i < h->n_v_items; ++i) vl[nv++] = make_int2(u, i);
     for (int i = 0; i < h->n_w_items; ++i) wl[nw++] = make_int2(u, i);

@@ -64,6 +64,26 @@ struct PlanHeader {
   int64_t w_td, w_te, w_tau, w_kcount, w_triwork, w_total;
   int32_t eig_threshold, pad_t;
   double tie_rel;
+  // pair formulation of the contraction (see PairItem)
+  int32_t n_pitems, pair_ok;  // pair_ok = 0: a box side exceeds kPairMaxB, use the tile kernel
+  int64_t o_pitems, o_tau_sector, o_cblk_of, o_rowoff, o_coloff;
+  double flops_pair_weight;   // 8 sum_pairs nL nR B0 B1 (B0 + B1)
+};
+
+// Pair formulation of the sector contraction.  For a left block a (charge cl)
+// and a right block b (cr <= cl) every tau and every centre charge cc of the
+// box cr <= q <= cl take part, and the BS weight factorises over the two
+// charge components:
+//   u(cl, cc, cr; tau) = alpha[tau0][cc0] conj(beta[tau1][cc1]),
+//   alpha[t][c] = U3[cl0 - t][t - cr0][cl0 - c], beta the same on component 1.
+// So C_tau[a, b] = sum_cc u P_cc with P_cc = Glc[a, cc] Gcr[cc, b] is
+//   C[t0][t1] = sum_c0 alpha[t0][c0] sum_c1 conj(beta[t1][c1]) P[c0][c1],
+// computed once per (a, b) for every tau.  A PairItem is a tile of tl x tr
+// elements of one (a, b) pair.
+constexpr int kPairMaxB = 16;     // largest box side handled (d <= 16 always fits)
+constexpr int kPairLds = 3072;    // B0 B1 tl tr <= kPairLds (LDS elements per item)
+struct PairItem {
+  int32_t a, b, l0, r0, tl, tr, pad0, pad1;
 };
 
 struct Sector {
@@ -111,6 +131,9 @@ struct GroupHeader {
   double flops_alg, flops_gram, flops_jacobi, flops_wmat, flops_vectors, bytes_alg16;
   double c_elems, g_elems, v_elems, w_elems, sv_total;
   double flops_tridiag, bytes_tridiag;
+  int32_t n_pitems, pair_ok;  // pair_ok: every plan of the group supports the pair kernel
+  int64_t o_pitems;
+  double flops_pair_weight;
 };
 constexpr int64_t kGroupHostHead = 1024;  // host blob: header, then the staging image
 static_assert(sizeof(GroupHeader) <= kGroupHostHead, "group header too large");
@@ -147,6 +170,8 @@ void timing_end(int token, hipStream_t stream, double flops, double bytes);
 int launch_bs_table(int d, int nb, const double *d_B, double *d_U3, hipStream_t stream);
 int launch_contract(const GroupHeader &g, const char *d_work, int precision, hipStream_t stream);
 int launch_contract_lowp(const GroupHeader &g, const char *d_work, int precision,
+                         hipStream_t stream);
+int launch_contract_pair(const GroupHeader &g, const char *d_work, int precision,
                          hipStream_t stream);
 int launch_svd(const GroupHeader &g, const char *d_work, hipStream_t stream);
 int launch_truncate(const GroupHeader &g, const char *d_work, hipStream_t stream);

@@ -40,6 +40,8 @@ struct Built {
   std::vector<int32_t> charges;  // CL, CC, CR: 3 x chi x 2
   std::vector<int32_t> sv_sector;  // sector of every singular-value slot
   std::vector<int32_t> jac_order;  // sector indices grouped by Jacobi class
+  std::vector<PairItem> pitems;    // pair-formulation work items
+  std::vector<int32_t> tau_sector, cblk_of, rowoff, coloff;
 };
 
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
@@ -229,6 +231,55 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
   }
   b.h.jac_off[kJacClasses] = (int32_t)b.jac_order.size();
 
+  // pair formulation: tau -> sector, centre charge -> block, block offsets in sectors
+  {
+    const int D1 = d + 1;
+    const int ns = (int)b.sectors.size(), nl = (int)b.lblk.size(), nr = (int)b.rblk.size();
+    b.tau_sector.assign((size_t)D1 * D1, -1);
+    b.cblk_of.assign((size_t)D1 * D1, -1);
+    for (int s = 0; s < ns; ++s) b.tau_sector[(size_t)b.sectors[s].t0 * D1 + b.sectors[s].t1] = s;
+    for (int c = 0; c < (int)b.cblk.size(); ++c)
+      b.cblk_of[(size_t)b.cblk[c].c0 * D1 + b.cblk[c].c1] = c;
+    b.rowoff.assign((size_t)ns * nl, -1);
+    b.coloff.assign((size_t)ns * nr, -1);
+    for (int s = 0; s < ns; ++s) {
+      const Sector &x = b.sectors[s];
+      int o = 0;
+      for (int a = 0; a < nl; ++a)
+        if (b.lblk[a].c0 >= x.t0 && b.lblk[a].c1 >= x.t1) {
+          b.rowoff[(size_t)s * nl + a] = o;
+          o += b.lblk[a].end - b.lblk[a].start;
+        }
+      o = 0;
+      for (int c = 0; c < nr; ++c)
+        if (b.rblk[c].c0 <= x.t0 && b.rblk[c].c1 <= x.t1) {
+          b.coloff[(size_t)s * nr + c] = o;
+          o += b.rblk[c].end - b.rblk[c].start;
+        }
+    }
+    b.h.pair_ok = 1;
+    for (int a = 0; a < nl; ++a) {
+      const Block &la = b.lblk[a];
+      for (int c = 0; c < nr; ++c) {
+        const Block &rb = b.rblk[c];
+        if (!(la.c0 >= rb.c0 && la.c1 >= rb.c1)) continue;
+        if (only_t0 >= 0 && !(la.c0 >= only_t0 && only_t0 >= rb.c0 && la.c1 >= only_t1 &&
+                              only_t1 >= rb.c1))
+          continue;
+        const int B0 = la.c0 - rb.c0 + 1, B1 = la.c1 - rb.c1 + 1;
+        if (B0 > kPairMaxB || B1 > kPairMaxB) b.h.pair_ok = 0;
+        const int na = la.end - la.start, nb = rb.end - rb.start;
+        const int emax = std::max(1, std::min(64, kPairLds / (B0 * B1)));
+        const int tr = std::min(nb, std::min(8, emax));
+        const int tl = std::min(na, std::max(1, emax / tr));
+        for (int l0 = 0; l0 < na; l0 += tl)
+          for (int r0 = 0; r0 < nb; r0 += tr)
+            b.pitems.push_back(PairItem{a, c, l0, r0, std::min(tl, na - l0), std::min(tr, nb - r0), 0, 0});
+        b.h.flops_pair_weight += 8.0 * na * nb * (double)(B0 * B1) * (B0 + B1);
+      }
+    }
+  }
+
   b.charges.resize((size_t)6 * chi);
   std::memcpy(b.charges.data(), CL, sizeof(int32_t) * 2 * chi);
   std::memcpy(b.charges.data() + 2 * chi, CC, sizeof(int32_t) * 2 * chi);
@@ -277,6 +328,12 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
   h.o_charges = place(sizeof(int32_t) * b.charges.size());
   h.o_sv_sector = place(sizeof(int32_t) * b.sv_sector.size());
   h.o_jac_order = place(sizeof(int32_t) * b.jac_order.size());
+  h.o_pitems = place(sizeof(PairItem) * b.pitems.size());
+  h.o_tau_sector = place(sizeof(int32_t) * b.tau_sector.size());
+  h.o_cblk_of = place(sizeof(int32_t) * b.cblk_of.size());
+  h.o_rowoff = place(sizeof(int32_t) * b.rowoff.size());
+  h.o_coloff = place(sizeof(int32_t) * b.coloff.size());
+  h.n_pitems = (int32_t)b.pitems.size();
   h.dev_bytes = o;
   h.blob_bytes = h.dev_begin + h.dev_bytes;
 
@@ -416,6 +473,11 @@ int bsmpo_plan_build(int d, int chi, const int32_t *h_CL, const int32_t *h_CC,
   put(dev, b.h.o_charges, b.charges);
   put(dev, b.h.o_sv_sector, b.sv_sector);
   put(dev, b.h.o_jac_order, b.jac_order);
+  put(dev, b.h.o_pitems, b.pitems);
+  put(dev, b.h.o_tau_sector, b.tau_sector);
+  put(dev, b.h.o_cblk_of, b.cblk_of);
+  put(dev, b.h.o_rowoff, b.rowoff);
+  put(dev, b.h.o_coloff, b.coloff);
   return 0;
 }
 

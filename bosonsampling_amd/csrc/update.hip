@@ -565,7 +565,10 @@ static inline const int2 *glist(const char *w, int64_t off) {
 int launch_contract(const GroupHeader &g, const char *w, int precision, hipStream_t stream) {
   if (g.n_items == 0) return 0;
   const int tk = timing_begin(ST_CONTRACT, stream);
-  if (precision == BSMPO_C128) {
+  if (g.pair_ok && (precision == BSMPO_C128 || precision == BSMPO_C64)) {
+    int rc = launch_contract_pair(g, w, precision, stream);
+    if (rc) return rc;
+  } else if (precision == BSMPO_C128) {
     hipLaunchKernelGGL(k_contract_c128, dim3(grid_for(g.n_items)), dim3(256), 0, stream,
                        gdesc(g, w), glist(w, g.o_items), g.n_items);
   } else {
