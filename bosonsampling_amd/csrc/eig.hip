@@ -308,27 +308,27 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
   }
 }
 
-// number of eigenvalues of the symmetric tridiagonal (d, e) below x
-__device__ int sturm_below(const double *d, const double *e, int n, double x, double pivmin) {
+// number of eigenvalues of the symmetric tridiagonal (d, e) below x; e2 = e^2
+__device__ int sturm_below(const double *d, const double *e2, int n, double x, double pivmin) {
   int c = 0;
   double q = d[0] - x;
   if (fabs(q) < pivmin) q = -pivmin;
   if (q < 0.0) ++c;
   for (int i = 1; i < n; ++i) {
-    q = d[i] - x - e[i - 1] * e[i - 1] / q;
+    q = d[i] - x - e2[i - 1] / q;
     if (fabs(q) < pivmin) q = -pivmin;
     if (q < 0.0) ++c;
   }
   return c;
 }
 
-__device__ double bisect_kth(const double *d, const double *e, int n, int j, double lo, double hi,
+__device__ double bisect_kth(const double *d, const double *e2, int n, int j, double lo, double hi,
                              double pivmin) {
   const int target = n - 1 - j;  // j-th largest = (n-1-j)-th smallest
   for (int it = 0; it < 128; ++it) {
     const double mid = 0.5 * (lo + hi);
     if (!(mid > lo && mid < hi)) break;
-    if (sturm_below(d, e, n, mid, pivmin) <= target)
+    if (sturm_below(d, e2, n, mid, pivmin) <= target)
       lo = mid;
     else
       hi = mid;
@@ -336,32 +336,50 @@ __device__ double bisect_kth(const double *d, const double *e, int n, int j, dou
   return 0.5 * (lo + hi);
 }
 
-// grid (large sectors, ceil(n_max / 256)); thread -> eigenvalue index j
+// grid (large sectors, ceil(n_max / 256)); thread -> eigenvalue index j.  The
+// tridiagonal (d, e^2) is staged in LDS; the largest eigenvalue (for the drop
+// rule) is bisected once per block, by the thread that owns index 0 in the
+// first block and by thread 0 in the others.
 __global__ __launch_bounds__(256) void k_bisect(const UpdDesc *__restrict__ Ud,
                                                 const int2 *__restrict__ list) {
+  extern __shared__ double bsm[];
+  __shared__ double sh_lo, sh_hi, sh_piv, sh_lmax;
   const int2 ref = list[blockIdx.x];
   const UpdDesc &u = Ud[ref.x];
   const Sector s = u.sectors[ref.y];
   const int n = s.n;
+  if ((int)(blockIdx.y * blockDim.x) >= n) return;  // no eigenvalue of this sector here
   const double *d = u.td + s.s_off, *e = u.te + s.s_off;
   double *lam = u.lam, *S = u.S;
   int32_t *flags = u.flag;
-  // Gershgorin interval and pivot guard (redundant per thread; O(n))
-  double lo = d[0], hi = d[0], emax2 = 0.0;
-  for (int i = 0; i < n; ++i) {
-    const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0);
-    lo = fmin(lo, d[i] - r);
-    hi = fmax(hi, d[i] + r);
-    if (i + 1 < n) emax2 = fmax(emax2, e[i] * e[i]);
+  double *ds = bsm, *e2s = bsm + n;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    ds[i] = d[i];
+    e2s[i] = i + 1 < n ? e[i] * e[i] : 0.0;
   }
-  const double span = fmax(fabs(lo), fabs(hi));
-  lo -= 4.0 * DBL_EPSILON * span + DBL_MIN;
-  hi += 4.0 * DBL_EPSILON * span + DBL_MIN;
-  const double pivmin = DBL_MIN * fmax(1.0, emax2);
-  const double lmax = bisect_kth(d, e, n, 0, lo, hi, pivmin);
+  if (threadIdx.x == 0) {
+    // Gershgorin interval and pivot guard
+    double lo = d[0], hi = d[0], emax2 = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0);
+      lo = fmin(lo, d[i] - r);
+      hi = fmax(hi, d[i] + r);
+      if (i + 1 < n) emax2 = fmax(emax2, e[i] * e[i]);
+    }
+    const double span = fmax(fabs(lo), fabs(hi));
+    sh_lo = lo - (4.0 * DBL_EPSILON * span + DBL_MIN);
+    sh_hi = hi + (4.0 * DBL_EPSILON * span + DBL_MIN);
+    sh_piv = DBL_MIN * fmax(1.0, emax2);
+  }
+  __syncthreads();
+  const double lo = sh_lo, hi = sh_hi, pivmin = sh_piv;
   const int j = blockIdx.y * blockDim.x + threadIdx.x;
+  double l = 0.0;
+  if (j < n) l = bisect_kth(ds, e2s, n, j, lo, hi, pivmin);
+  if (threadIdx.x == 0) sh_lmax = blockIdx.y == 0 ? l : bisect_kth(ds, e2s, n, 0, lo, hi, pivmin);
+  __syncthreads();
   if (j >= n) return;
-  const double l = j == 0 ? lmax : bisect_kth(d, e, n, j, lo, hi, pivmin);
+  const double lmax = sh_lmax;
   const bool keep = lmax > 0.0 && l > kDropRel2Tri * lmax;
   lam[s.s_off + j] = l;
   S[s.s_off + j] = keep ? sqrt(l) : 0.0;
@@ -423,19 +441,28 @@ __device__ void tri_solve(const double *__restrict__ d, const double *__restrict
   }
 }
 
+constexpr int kTriVecThreads = 128;  // one thread per cluster of kept eigenvalues
+
 // One block per large sector: eigenvectors of the kept eigenvalues j < k,
 // written as complex columns (n long) into the sector's Y slot, then
 // back-transformed by the Householder reflectors stored in G.
-__global__ __launch_bounds__(256) void k_tri_vectors(const UpdDesc *__restrict__ Ud,
+__global__ __launch_bounds__(kTriVecThreads) void k_tri_vectors(const UpdDesc *__restrict__ Ud,
                                                      const int2 *__restrict__ list) {
   __shared__ int cstart[2048 + 1];
   __shared__ int ncl;
+  extern __shared__ double tvs[];  // d, e of the sector (2 n)
   const int2 ref = list[blockIdx.x];
   const UpdDesc &u = Ud[ref.x];
   const Sector s = u.sectors[ref.y];
   const int n = s.n, k = u.kcount[ref.y];
   if (k == 0) return;
-  const double *d = u.td + s.s_off, *e = u.te + s.s_off, *l = u.lam + s.s_off;
+  const double *l = u.lam + s.s_off;
+  double *d = tvs, *e = tvs + n;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    d[i] = u.td[s.s_off + i];
+    e[i] = u.te[s.s_off + i];
+  }
+  __syncthreads();
   double *triwork = u.triwork;
   double2 *X = s.left_small ? reinterpret_cast<double2 *>(u.V) + s.v_off
                             : reinterpret_cast<double2 *>(u.W) + s.w_off;  // column j: X[j n + r]
@@ -480,16 +507,20 @@ __global__ __launch_bounds__(256) void k_tri_vectors(const UpdDesc *__restrict__
   }
 }
 
-// x = H_0 H_1 ... H_{n-2} y for the kept eigenvectors of every large sector
+// x = H_0 H_1 ... H_{n-2} y for the kept eigenvectors of every large sector;
+// one wave per vector, reflectors staged in LDS up to kBtChunk at a time
+constexpr int kBtChunk = 8;
+constexpr int kBtLds = 4096;  // complex elements of the reflector cache (64 KiB)
 template <int kBtRows>
 __global__ __launch_bounds__(kBtRows == 16 ? 1024 : 256) void k_tri_backtransform(
     const UpdDesc *__restrict__ Ud, const int2 *__restrict__ list) {
-  extern __shared__ double2 vsm[];  // reflector cache (n)
+  extern __shared__ double2 vsm[];  // reflector cache: per x n
   const int2 ref = list[blockIdx.x];
   const UpdDesc &u = Ud[ref.x];
   const Sector s = u.sectors[ref.y];
   const int n = s.n, k = u.kcount[ref.y];
   if (k == 0) return;
+  const int per = max(1, min(kBtChunk, kBtLds / max(n, 1)));
   double2 *X = s.left_small ? reinterpret_cast<double2 *>(u.V) + s.v_off
                             : reinterpret_cast<double2 *>(u.W) + s.w_off;
   const double2 *A = reinterpret_cast<const double2 *>(u.G) + s.g_off;
@@ -506,44 +537,53 @@ __global__ __launch_bounds__(kBtRows == 16 ? 1024 : 256) void k_tri_backtransfor
         const int r = lane + 64 * q;
         x[u][q] = (jb + u < k && r < n) ? X[(int64_t)(jb + u) * n + r] : make_double2(0.0, 0.0);
       }
-    for (int i = n - 2; i >= 0; --i) {
-      const double2 ti = tau[i];
-      if (ti.x == 0.0 && ti.y == 0.0) continue;  // uniform
+    // reflectors n-2 .. 0, staged `per` at a time in LDS (one barrier pair per chunk)
+    for (int ihi = n - 2; ihi >= 0; ihi -= per) {
+      const int ilo = max(0, ihi - per + 1);
       __syncthreads();
-      for (int r = i + 1 + threadIdx.x; r < n; r += blockDim.x)
-        vsm[r] = r == i + 1 ? make_double2(1.0, 0.0) : A[(int64_t)i * n + r];
+      for (int e = threadIdx.x; e < (ihi - ilo + 1) * n; e += blockDim.x) {
+        const int i = ilo + e / n, r = e % n;
+        vsm[e] = r <= i ? make_double2(0.0, 0.0)
+                        : (r == i + 1 ? make_double2(1.0, 0.0) : A[(int64_t)i * n + r]);
+      }
       __syncthreads();
-      double sr[kBtVec], si[kBtVec];
+      for (int i = ihi; i >= ilo; --i) {
+        const double2 ti = tau[i];
+        if (ti.x == 0.0 && ti.y == 0.0) continue;  // uniform
+        const double2 *v = vsm + (int64_t)(i - ilo) * n;
+        double sr[kBtVec], si[kBtVec];
 #pragma unroll
-      for (int u = 0; u < kBtVec; ++u) { sr[u] = 0.0; si[u] = 0.0; }
+        for (int u = 0; u < kBtVec; ++u) { sr[u] = 0.0; si[u] = 0.0; }
 #pragma unroll
-      for (int q = 0; q < kBtRows; ++q) {
-        const int r = lane + 64 * q;
-        if (r > i && r < n) {
-          const double2 a = vsm[r];
+        for (int q = 0; q < kBtRows; ++q) {
+          const int r = lane + 64 * q;
+          if (r > i && r < n) {
+            const double2 a = v[r];
 #pragma unroll
-          for (int u = 0; u < kBtVec; ++u) {
-            sr[u] += a.x * x[u][q].x + a.y * x[u][q].y;
-            si[u] += a.x * x[u][q].y - a.y * x[u][q].x;
+            for (int u = 0; u < kBtVec; ++u) {
+              sr[u] += a.x * x[u][q].x + a.y * x[u][q].y;
+              si[u] += a.x * x[u][q].y - a.y * x[u][q].x;
+            }
           }
         }
-      }
 #pragma unroll
-      for (int u = 0; u < kBtVec; ++u)
-        for (int o = 32; o >= 1; o >>= 1) {
-          sr[u] += __shfl_xor(sr[u], o);
-          si[u] += __shfl_xor(si[u], o);
-        }
+        for (int u = 0; u < kBtVec; ++u)
+          for (int o = 32; o >= 1; o >>= 1) {
+            sr[u] += __shfl_xor(sr[u], o);
+            si[u] += __shfl_xor(si[u], o);
+          }
 #pragma unroll
-      for (int q = 0; q < kBtRows; ++q) {
-        const int r = lane + 64 * q;
-        if (r > i && r < n) {
-          const double2 a = vsm[r];
+        for (int q = 0; q < kBtRows; ++q) {
+          const int r = lane + 64 * q;
+          if (r > i && r < n) {
+            const double2 a = v[r];
 #pragma unroll
-          for (int u = 0; u < kBtVec; ++u) {
-            const double2 f = make_double2(ti.x * sr[u] - ti.y * si[u], ti.x * si[u] + ti.y * sr[u]);
-            x[u][q].x -= a.x * f.x - a.y * f.y;
-            x[u][q].y -= a.x * f.y + a.y * f.x;
+            for (int u = 0; u < kBtVec; ++u) {
+              const double2 f =
+                  make_double2(ti.x * sr[u] - ti.y * si[u], ti.x * si[u] + ti.y * sr[u]);
+              x[u][q].x -= a.x * f.x - a.y * f.y;
+              x[u][q].y -= a.x * f.y + a.y * f.x;
+            }
           }
         }
       }
@@ -594,8 +634,8 @@ int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
   }
   timing_end(tk, stream, g.flops_tridiag, g.bytes_tridiag);
   tk = timing_begin(ST_BISECT, stream);
-  hipLaunchKernelGGL(k_bisect, dim3(g.n_tri, (g.tri_max_n + 255) / 256), dim3(256), 0, stream,
-                     gdesc(g, w), glist(w, g.o_tri));
+  hipLaunchKernelGGL(k_bisect, dim3(g.n_tri, (g.tri_max_n + 255) / 256), dim3(256),
+                     (size_t)g.tri_max_n * 16, stream, gdesc(g, w), glist(w, g.o_tri));
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, 0.0, 0.0);
   return 0;
@@ -612,12 +652,12 @@ int launch_kcount(const GroupHeader &g, const char *w, hipStream_t stream) {
 int launch_tri_vectors(const GroupHeader &g, const char *w, hipStream_t stream) {
   if (g.n_tri == 0) return 0;
   int tk = timing_begin(ST_TRIVEC, stream);
-  hipLaunchKernelGGL(k_tri_vectors, dim3(g.n_tri), dim3(256), 0, stream, gdesc(g, w),
-                     glist(w, g.o_tri));
+  hipLaunchKernelGGL(k_tri_vectors, dim3(g.n_tri), dim3(kTriVecThreads),
+                     (size_t)g.tri_max_n * 16, stream, gdesc(g, w), glist(w, g.o_tri));
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, 0.0, 0.0);
   tk = timing_begin(ST_BACKTRANS, stream);
-  const size_t lds = (size_t)g.tri_max_n * 16;
+  const size_t lds = (size_t)kBtLds * 16;
   if (g.tri_max_n <= 1024)
     hipLaunchKernelGGL(k_tri_backtransform<16>, dim3(g.n_tri), dim3(1024), lds, stream,
                        gdesc(g, w), glist(w, g.o_tri));

@@ -224,7 +224,10 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
             if (j % kTriPanel == kTriPanel - 1 || j + 2 == n) b.h.bytes_tridiag += 32.0 * m * m;
           }
           sc.tw_off = tw;
-          tw += (int64_t)4 * n * std::min(n, 1024);  // per-thread inverse-iteration arrays
+          // per-thread inverse-iteration arrays (4 n doubles for each of up to
+          // min(n, 1024) threads); the Householder panels V, W (2 n kTriPanel complex)
+          // use the same area first
+          tw += (int64_t)4 * n * std::max(std::min(n, 1024), kTriPanel);
         }
       }
     }
