@@ -167,7 +167,15 @@ def main():
             return len(todo)
 
         t1 = time.time()
+        plain_tuned = None
         for layer in range(W):
+            if args.autotune and layer == W - 1:
+                best, times = engine.autotune_eig_threshold([st], [meshes[0]], inputs, layer,
+                                                            precision=engine.C128, grouped=False)
+                plain_tuned = {"layer": layer, "eig_threshold": best,
+                               "seconds": {str(k): round(v, 3) for k, v in times.items()}}
+                log(f"[bench] plain path: autotuned eigensolver split on layer {layer}: "
+                    f"threshold {best} {times}")
             run_layer(layer, engine.C128)
         torch.cuda.synchronize()
         log(f"[bench] plain warmup {W} layers {time.time() - t1:.1f}s")
@@ -183,7 +191,7 @@ def main():
         st_plain = _capi.timing_read(reset=True)
         _capi.timing_enable(False)
         result["plain"] = {"updates": n, "seconds": el, "rate": n / el,
-                           "observables": st.observables()}
+                           "observables": st.observables(), "tuned": plain_tuned}
         log(f"[bench] plain c128: {n} updates in {el:.3f}s = {n / el:.2f} updates/s")
         stage_table(st_plain)
         del st
@@ -264,7 +272,7 @@ def main():
             "grouped": ({"instances": B, "precision": args.precision, "updates": g["updates"],
                          "updates_per_s": g["rate"], "autotune": g.get("tuned")} if g else None),
             "plain_fp64": ({"instances": 1, "updates": p["updates"],
-                            "updates_per_s": p["rate"]} if p else None),
+                            "updates_per_s": p["rate"], "autotune": p.get("tuned")} if p else None),
             "grouped_over_plain": (g["rate"] / p["rate"]) if (g and p) else None,
             "precision_check": precision_check(g, p, args.precision, W + K),
         },

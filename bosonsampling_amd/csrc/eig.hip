@@ -14,6 +14,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "common.h"
 
@@ -66,9 +67,10 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
 //   trailing matrix as of the panel start and the panel's pending rank-2
 //   updates enter as corrections -V (W^H v) - W (V^H v);
 //   after the panel: one pass A22 -= V W^H + W V^H over the trailing matrix.
-// Per column the trailing matrix is read once (the product A22 v); it is
-// written once per panel.  V and W of the panel live in the sector's
-// tridiagonal work area (n x kTriNB each, column-major).  Five barriers per
+// Only the lower triangle of the (Hermitian) trailing matrix is read or
+// written: per column it is read once (the product A22 v, each entry below the
+// diagonal serving A v and A^H v alike); it is written once per panel.  V and W of the panel live in the sector's
+// tridiagonal work area (n x kTriNB each, column-major).  Six barriers per
 // column: block reductions alternate between two LDS buffers, so each costs
 // one barrier.  Two block shapes: 256 threads x 2 rows (n <= 512) and
 // 512 threads x 4 rows (n <= 2048); each launch skips the other size class.
@@ -106,6 +108,31 @@ __device__ __forceinline__ double2 block_sum2(double a, double b, double2 (*red)
   return t;
 }
 
+// x[q] (q = 0..7) summed over the 64 lanes of a wave by recursive halving:
+// lane l returns the total of x[(l >> 3) & 7].
+__device__ __forceinline__ double2 wave_colsum8(const double2 (&x)[8], int lane) {
+  const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8;
+  double2 y4[4], y2[2];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double2 snd = h5 ? x[q] : x[q + 4], kp = h5 ? x[q + 4] : x[q];
+    y4[q] = make_double2(kp.x + __shfl_xor(snd.x, 32), kp.y + __shfl_xor(snd.y, 32));
+  }
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const double2 snd = h4 ? y4[q] : y4[q + 2], kp = h4 ? y4[q + 2] : y4[q];
+    y2[q] = make_double2(kp.x + __shfl_xor(snd.x, 16), kp.y + __shfl_xor(snd.y, 16));
+  }
+  const double2 snd = h3 ? y2[0] : y2[1], kp = h3 ? y2[1] : y2[0];
+  double2 y = make_double2(kp.x + __shfl_xor(snd.x, 8), kp.y + __shfl_xor(snd.y, 8));
+#pragma unroll
+  for (int o = 4; o >= 1; o >>= 1) {
+    y.x += __shfl_xor(y.x, o);
+    y.y += __shfl_xor(y.y, o);
+  }
+  return y;
+}
+
 template <int NT, int RPT>
 __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
                                                 const int2 *__restrict__ list, int nmin, int nmax) {
@@ -121,6 +148,7 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
   if (n < nmin || n > nmax) return;  // the other block shape's sector
   double2 *A = reinterpret_cast<double2 *>(u.G) + s.g_off;  // A(r, c) = A[c n + r]
   double2 *vs = tsm;                                         // v_j, n entries
+  double2 *ys = tsm + n;                                     // A22 v_j, n entries
   double2 *Vp = reinterpret_cast<double2 *>(u.triwork + s.tw_off);
   double2 *Wp = Vp + (int64_t)n * kTriNB;
   double *d = u.td + s.s_off, *e = u.te + s.s_off;
@@ -164,25 +192,46 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
           A[(int64_t)j * n + r] = vr;
         }
         vs[r] = vr;
+        ys[r] = make_double2(0.0, 0.0);
       }
       __syncthreads();
-      // (c) p = A22 v over rows r >= j+1 (A22 as of the panel start)
-      double2 acc[RPT];
+      // (c) p = A22 v over rows r >= j+1, A22 as of the panel start with only
+      //     its lower triangle read: strips of 8 columns go round the waves;
+      //     lane rows r >= c feed p(r) += A(r, c) v(c) and, below the
+      //     diagonal, p(c) += conj(A(r, c)) v(r).  Both sums go to ys by LDS
+      //     atomics (row sums per lane, column sums after a wave reduction).
+      for (int s0 = j + 1 + 8 * wave; s0 < n; s0 += 8 * NW) {
+        double2 cp[8];
 #pragma unroll
-      for (int k = 0; k < RPT; ++k) acc[k] = make_double2(0.0, 0.0);
-      for (int c0 = j + 1; c0 < n; c0 += 8) {
-#pragma unroll
-        for (int k = 0; k < RPT; ++k) {
-          const int r = j + 1 + t + k * NT;
-          if (r >= n) continue;
+        for (int q = 0; q < 8; ++q) cp[q] = make_double2(0.0, 0.0);
+        for (int r = s0 + lane; r < n; r += 64) {
           double2 a[8];
 #pragma unroll
           for (int q = 0; q < 8; ++q)
-            a[q] = c0 + q < n ? A[(int64_t)(c0 + q) * n + r] : make_double2(0.0, 0.0);
+            a[q] = s0 + q <= r ? A[(int64_t)(s0 + q) * n + r] : make_double2(0.0, 0.0);
+          const double2 vr = vs[r];
+          double2 rp = make_double2(0.0, 0.0);
 #pragma unroll
-          for (int q = 0; q < 8; ++q)
-            if (c0 + q < n) acc[k] = cfma2(acc[k], a[q], vs[c0 + q]);
+          for (int q = 0; q < 8; ++q) {
+            rp = cfma2(rp, a[q], vs[min(s0 + q, n - 1)]);
+            if (s0 + q < r) cp[q] = cfma2(cp[q], make_double2(a[q].x, -a[q].y), vr);
+          }
+          atomicAdd(&ys[r].x, rp.x);
+          atomicAdd(&ys[r].y, rp.y);
         }
+        const double2 cs = wave_colsum8(cp, lane);
+        const int cq = s0 + (lane >> 3);
+        if ((lane & 7) == 0 && cq < n) {
+          atomicAdd(&ys[cq].x, cs.x);
+          atomicAdd(&ys[cq].y, cs.y);
+        }
+      }
+      __syncthreads();
+      double2 acc[RPT];
+#pragma unroll
+      for (int k = 0; k < RPT; ++k) {
+        const int r = j + 1 + t + k * NT;
+        acc[k] = r < n ? ys[r] : make_double2(0.0, 0.0);
       }
       // (d) corrections for the panel's pending updates
       if (i > 0) {
@@ -260,18 +309,20 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
       __syncthreads();
     }
     // (f) trailing update A(r, c) -= sum_l V(r, l) conj(W(c, l)) + W(r, l) conj(V(c, l)),
-    //     r, c >= jn; rows over threads, columns staged kTriChunk at a time
+    //     n > r >= c >= jn (lower triangle); rows over threads, columns staged
+    //     kTriChunk at a time
     const int jn = j0 + pb;
     for (int rbase = jn; rbase < n; rbase += NT) {
       const int r = rbase + t;
       const bool rv = r < n;
+      const int cend = min(n, rbase + NT);  // columns past the block's last row are upper
       double2 vr[kTriNB], wr[kTriNB];
 #pragma unroll
       for (int l = 0; l < kTriNB; ++l) {
         vr[l] = (rv && l < pb) ? Vp[(int64_t)l * n + r] : make_double2(0.0, 0.0);
         wr[l] = (rv && l < pb) ? Wp[(int64_t)l * n + r] : make_double2(0.0, 0.0);
       }
-      for (int c0 = jn; c0 < n; c0 += kTriChunk) {
+      for (int c0 = jn; c0 < cend; c0 += kTriChunk) {
         const int nc = min(kTriChunk, n - c0);
         __syncthreads();
         for (int q = t; q < kTriChunk * kTriNB; q += NT) {
@@ -286,7 +337,8 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
         }
         __syncthreads();
         if (rv) {
-          for (int cc = 0; cc < nc; ++cc) {
+          const int ncr = min(nc, r - c0 + 1);
+          for (int cc = 0; cc < ncr; ++cc) {
             const int64_t o = (int64_t)(c0 + cc) * n + r;
             double2 a = A[o];
 #pragma unroll
@@ -404,66 +456,56 @@ __global__ void k_kcount(const UpdDesc *__restrict__ Ud, const int2 *__restrict_
   }
 }
 
-// Solve (T - lam I) x = b in place (b -> x), partial pivoting (as LAPACK
-// dgtsv).  The elimination carries the running row in registers and only
-// stores what the back-substitution needs (pivot row: dd, du, fill-in dl).
-__device__ void tri_solve(const double *__restrict__ d, const double *__restrict__ e, int n,
-                          double lam, double tiny, double *__restrict__ dd,
-                          double *__restrict__ du, double *__restrict__ dl,
-                          double *__restrict__ b) {
-  // running (not yet eliminated) row i: [.., cd, cu] with rhs cb
-  double cd = d[0] - lam, cu = n > 1 ? e[0] : 0.0, cb = b[0];
-  for (int i = 0; i + 1 < n; ++i) {
-    const double sub = e[i];                          // T(i+1, i)
-    const double nd = d[i + 1] - lam;                 // T(i+1, i+1)
-    const double nu = i + 2 < n ? e[i + 1] : 0.0;     // T(i+1, i+2)
-    const double nb = b[i + 1];
-    if (fabs(cd) >= fabs(sub)) {
-      if (cd == 0.0) cd = tiny;
-      const double f = sub / cd;
-      dd[i] = cd; du[i] = cu; dl[i] = 0.0; b[i] = cb;
-      cd = nd - f * cu; cu = nu; cb = nb - f * cb;
-    } else {
-      const double f = cd / sub;
-      dd[i] = sub; du[i] = nd; dl[i] = nu; b[i] = nb;   // swapped: row i+1 becomes pivot
-      cd = cu - f * nd; cu = -f * nu; cb = cb - f * nb;
-    }
-  }
-  if (cd == 0.0) cd = tiny;
-  dd[n - 1] = cd;
-  double x1 = cb / cd, x2 = 0.0;
-  b[n - 1] = x1;
-  for (int i = n - 2; i >= 0; --i) {
-    const double x = (b[i] - du[i] * x1 - dl[i] * x2) / dd[i];
-    b[i] = x;
-    x2 = x1;
-    x1 = x;
-  }
+// Inverse iteration for the kept eigenvalues j < k of a large sector (the
+// LAPACK dlagtf / dlagts / dstein organisation, restated).  Per eigenvalue,
+// P (T - lam I) = L U with partial pivoting is formed once: step i keeps
+// 1/U(i,i), U(i,i+1), U(i,i+2), the multiplier and the row swap.  Three solves
+// U x = L^-1 P b follow, each from the previous iterate; its norm is divided
+// out while the next forward pass reads it (and on output).  Clusters of
+// close eigenvalues run in one lane, Gram-Schmidt against earlier members.
+// The five factor arrays and the iterate (6 n doubles per lane) live in LDS
+// for sectors with n <= nsplit and few kept vectors per lane; the other
+// sectors keep them in their tridiagonal work area, interleaved over lanes.
+constexpr int kVecArrays = 6;
+constexpr int kVecLds = 40960;        // dynamic LDS of the LDS-resident variant
+constexpr int kVecLdsThreads = 64;
+constexpr int kVecSerial = 4;         // at most this many vectors per LDS lane
+
+__host__ __device__ inline int vec_lds_head(int n, int k) {  // d, e, cluster starts (bytes)
+  return (16 * n + 4 * (k + 1) + 15) / 16 * 16;
+}
+__host__ __device__ inline int vec_lds_lanes(int n, int k) {
+  const int room = kVecLds - vec_lds_head(n, k);
+  return room <= 0 ? 0 : min(kVecLdsThreads, room / (kVecArrays * 8 * n));
+}
+__host__ __device__ inline bool vec_in_lds(int n, int k, int nsplit) {
+  const int p = vec_lds_lanes(n, k);
+  return n <= nsplit && p > 0 && k <= kVecSerial * p;
 }
 
-constexpr int kTriVecThreads = 128;  // one thread per cluster of kept eigenvalues
+__device__ __forceinline__ double vec_start(int i, int j, int n) {
+  return 1.0 / sqrt((double)n) * (1.0 + 1e-3 * ((i * 7919 + j * 104729) % 1000) / 1000.0);
+}
 
-// One block per large sector: eigenvectors of the kept eigenvalues j < k,
-// written as complex columns (n long) into the sector's Y slot, then
-// back-transformed by the Householder reflectors stored in G.
-__global__ __launch_bounds__(kTriVecThreads) void k_tri_vectors(const UpdDesc *__restrict__ Ud,
-                                                     const int2 *__restrict__ list) {
-  __shared__ int cstart[2048 + 1];
+template <bool kInLds>
+__global__ __launch_bounds__(kInLds ? kVecLdsThreads : kTriVecThreads) void k_tri_vectors(
+    const UpdDesc *__restrict__ Ud, const int2 *__restrict__ list, int nsplit) {
+  extern __shared__ double tvs[];  // d, e (2 n), cluster starts (k + 1), lane arrays (LDS variant)
   __shared__ int ncl;
-  extern __shared__ double tvs[];  // d, e of the sector (2 n)
+  __shared__ double sh_tiny;
   const int2 ref = list[blockIdx.x];
   const UpdDesc &u = Ud[ref.x];
   const Sector s = u.sectors[ref.y];
   const int n = s.n, k = u.kcount[ref.y];
-  if (k == 0) return;
+  if (k == 0 || vec_in_lds(n, k, nsplit) != kInLds) return;  // nothing kept, or the other variant's
   const double *l = u.lam + s.s_off;
   double *d = tvs, *e = tvs + n;
+  int *cstart = reinterpret_cast<int *>(tvs + 2 * n);
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     d[i] = u.td[s.s_off + i];
     e[i] = u.te[s.s_off + i];
   }
   __syncthreads();
-  double *triwork = u.triwork;
   double2 *X = s.left_small ? reinterpret_cast<double2 *>(u.V) + s.v_off
                             : reinterpret_cast<double2 *>(u.W) + s.w_off;  // column j: X[j n + r]
   // clusters of close eigenvalues (gap <= kClusterRel |T|) are orthogonalised
@@ -471,40 +513,103 @@ __global__ __launch_bounds__(kTriVecThreads) void k_tri_vectors(const UpdDesc *_
   // eps |T| / gap <= 2e-10 orthogonality (LAPACK's stein uses 1e-3 |T|, which
   // would chain most kept eigenvalues of a sector into one serial cluster)
   if (threadIdx.x == 0) {
-    double tn = 0.0;
-    for (int i = 0; i < n; ++i)
+    double tn = 0.0, tn1 = 0.0;
+    for (int i = 0; i < n; ++i) {
       tn = fmax(tn, fabs(d[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0));
+      tn1 = fmax(tn1, fabs(d[i]) + (i + 1 < n ? 2.0 * fabs(e[i]) : 0.0));
+    }
     const double ortol = kClusterRel * tn;
     int c = 0;
     for (int j = 0; j < k; ++j)
       if (j == 0 || fabs(l[j - 1] - l[j]) > ortol) cstart[c++] = j;
     cstart[c] = k;
     ncl = c;
+    sh_tiny = DBL_EPSILON * fmax(tn1, DBL_MIN);
   }
   __syncthreads();
-  double tn1 = 0.0;
-  for (int i = 0; i < n; ++i) tn1 = fmax(tn1, fabs(d[i]) + (i + 1 < n ? 2.0 * fabs(e[i]) : 0.0));
-  const double tiny = DBL_EPSILON * fmax(tn1, DBL_MIN);
-  double *wk = triwork + s.tw_off + (int64_t)threadIdx.x * 4 * n;  // < min(n, 1024) threads used
-  double *dd = wk, *du = wk + n, *dl = wk + 2 * n, *b = wk + 3 * n;
-  for (int c = threadIdx.x; c < ncl; c += blockDim.x) {
+  const double tiny = sh_tiny;
+  int P, si;
+  int64_t sa;
+  double *base;
+  if (kInLds) {
+    P = vec_lds_lanes(n, k);
+    base = reinterpret_cast<double *>(reinterpret_cast<char *>(tvs) + vec_lds_head(n, k)) +
+           (int64_t)threadIdx.x * kVecArrays * n;
+    sa = n;
+    si = 1;
+  } else {
+    P = min(n, (int)blockDim.x);
+    base = u.triwork + s.tw_off + threadIdx.x;
+    sa = (int64_t)n * P;
+    si = P;
+  }
+  if ((int)threadIdx.x >= P) return;
+  double *rd = base, *du = base + sa, *dl = base + 2 * sa, *fm = base + 3 * sa,
+         *sw = base + 4 * sa, *b = base + 5 * sa;
+  for (int c = threadIdx.x; c < ncl; c += P) {
     for (int j = cstart[c]; j < cstart[c + 1]; ++j) {
-      for (int i = 0; i < n; ++i) b[i] = 1.0 / sqrt((double)n) * (1.0 + 1e-3 * ((i * 7919 + j * 104729) % 1000) / 1000.0);
-      for (int it = 0; it < 3; ++it) {
-        tri_solve(d, e, n, l[j], tiny, dd, du, dl, b);
-        for (int q = cstart[c]; q < j; ++q) {  // Gram-Schmidt against earlier members
-          double dot = 0.0;
-          for (int i = 0; i < n; ++i) dot += X[(int64_t)q * n + i].x * b[i];
-          for (int i = 0; i < n; ++i) b[i] -= dot * X[(int64_t)q * n + i].x;
-        }
-        double nr = 0.0;
-        for (int i = 0; i < n; ++i) nr += b[i] * b[i];
-        nr = nr > 0.0 ? 1.0 / sqrt(nr) : 0.0;
-        for (int i = 0; i < n; ++i) b[i] *= nr;
+      const double lam = l[j];
+      // P (T - lam I) = L U, the running row [cd, cu] carried in registers
+      double cd = d[0] - lam, cu = n > 1 ? e[0] : 0.0;
+      for (int i = 0; i + 1 < n; ++i) {
+        const double sub = e[i], nd = d[i + 1] - lam, nu = i + 2 < n ? e[i + 1] : 0.0;
+        const bool swp = fabs(cd) < fabs(sub);
+        if (!swp && cd == 0.0) cd = tiny;
+        const double r = 1.0 / (swp ? sub : cd);
+        const double f = (swp ? cd : sub) * r;
+        rd[i * si] = r;
+        du[i * si] = swp ? nd : cu;
+        dl[i * si] = swp ? nu : 0.0;
+        fm[i * si] = f;
+        sw[i * si] = swp ? 1.0 : 0.0;
+        const double ncd = swp ? cu - f * nd : nd - f * cu;
+        cu = swp ? -f * nu : nu;
+        cd = ncd;
       }
-      for (int i = 0; i < n; ++i) X[(int64_t)j * n + i] = make_double2(b[i], 0.0);
+      if (cd == 0.0) cd = tiny;
+      rd[(n - 1) * si] = 1.0 / cd;
+      for (int i = 0; i < n; ++i) b[i * si] = vec_start(i, j, n);
+      double scale = 1.0;
+      for (int it = 0; it < 3; ++it) {
+        // y = L^-1 P (scale b), in place
+        double cb = b[0] * scale;
+        for (int i = 0; i + 1 < n; ++i) {
+          const double nb = b[(i + 1) * si] * scale, f = fm[i * si];
+          const bool swp = sw[i * si] != 0.0;
+          b[i * si] = swp ? nb : cb;
+          cb = swp ? cb - f * nb : nb - f * cb;
+        }
+        // x = U^-1 y, in place, with |x|^2
+        double x1 = cb * rd[(n - 1) * si], x2 = 0.0, nr = x1 * x1;
+        b[(n - 1) * si] = x1;
+        for (int i = n - 2; i >= 0; --i) {
+          const double x = (b[i * si] - du[i * si] * x1 - dl[i * si] * x2) * rd[i * si];
+          b[i * si] = x;
+          nr += x * x;
+          x2 = x1;
+          x1 = x;
+        }
+        if (j > cstart[c]) {  // Gram-Schmidt against the cluster's earlier (unit) vectors
+          for (int q = cstart[c]; q < j; ++q) {
+            double dot = 0.0;
+            for (int i = 0; i < n; ++i) dot += X[(int64_t)q * n + i].x * b[i * si];
+            for (int i = 0; i < n; ++i) b[i * si] -= dot * X[(int64_t)q * n + i].x;
+          }
+          nr = 0.0;
+          for (int i = 0; i < n; ++i) nr += b[i * si] * b[i * si];
+        }
+        scale = nr > 0.0 ? 1.0 / sqrt(nr) : 0.0;
+      }
+      for (int i = 0; i < n; ++i) X[(int64_t)j * n + i] = make_double2(b[i * si] * scale, 0.0);
     }
   }
+}
+
+// largest n whose LDS variant holds at least one lane (k = n at most)
+static int vec_lds_nsplit() {
+  int n = 1;
+  while (n < kTriMaxN && vec_lds_lanes(n + 1, n + 1) > 0) ++n;
+  return n;
 }
 
 // x = H_0 H_1 ... H_{n-2} y for the kept eigenvectors of every large sector;
@@ -605,32 +710,73 @@ static inline const int2 *glist(const char *w, int64_t off) {
   return reinterpret_cast<const int2 *>(w + off);
 }
 
+// Per-device side stream (default priority, non-blocking) with a fork and a
+// join event, created on first use and kept for the life of the process.
+struct SideStream {
+  hipStream_t stream;
+  hipEvent_t fork, join;
+};
+static int side_stream(SideStream **out) {
+  static std::mutex mu;
+  static SideStream per_dev[64];
+  static bool made[64] = {};
+  int dev = 0;
+  BSMPO_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(-1, "device index out of range");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!made[dev]) {
+    SideStream &x = per_dev[dev];
+    BSMPO_HIP(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
+    BSMPO_HIP(hipEventCreateWithFlags(&x.fork, hipEventDisableTiming));
+    BSMPO_HIP(hipEventCreateWithFlags(&x.join, hipEventDisableTiming));
+    made[dev] = true;
+  }
+  *out = &per_dev[dev];
+  return 0;
+}
+
 int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
   if (g.n_tri == 0) return 0;
   if (g.tri_max_n > kTriMaxN) return fail(-1, "sector too large for the tridiagonal route (n > 2048)");
   static bool attr = false;
   if (!attr) {
     BSMPO_HIP(hipFuncSetAttribute((const void *)k_tridiag<256, 2>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 512 * 16));
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 512 * 32));
     BSMPO_HIP(hipFuncSetAttribute((const void *)k_tridiag<512, 4>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, kTriMaxN * 16));
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, kTriMaxN * 32));
     attr = true;
   }
   int tk = timing_begin(ST_TRIDIAG, stream);
   // Few sectors (a single update): the larger block shape for every sector, so
   // that each sector's column chain gets more threads.  Many sectors (a group
   // across instances): 256-thread blocks for n <= 512, more blocks per CU.
+  // When both shapes run, the large-sector launch goes to a side stream so that
+  // its long column chains overlap the many short ones (fork / join by events).
   const int split = g.n_tri < kTriFewSectors ? 0 : 512;
+  const bool both = split > 0 && g.tri_max_n > split;
+  hipStream_t big = stream;
+  SideStream *ss = nullptr;
+  if (both) {
+    int rc = side_stream(&ss);
+    if (rc) return rc;
+    BSMPO_HIP(hipEventRecord(ss->fork, stream));
+    BSMPO_HIP(hipStreamWaitEvent(ss->stream, ss->fork, 0));
+    big = ss->stream;
+  }
+  if (g.tri_max_n > split) {
+    hipLaunchKernelGGL((k_tridiag<512, 4>), dim3(g.n_tri), dim3(512), (size_t)g.tri_max_n * 32,
+                       big, gdesc(g, w), glist(w, g.o_tri), split + 1, kTriMaxN);
+    BSMPO_HIP(hipGetLastError());
+  }
   if (split > 0) {
     hipLaunchKernelGGL((k_tridiag<256, 2>), dim3(g.n_tri), dim3(256),
-                       (size_t)std::min(g.tri_max_n, 512) * 16, stream, gdesc(g, w),
+                       (size_t)std::min(g.tri_max_n, 512) * 32, stream, gdesc(g, w),
                        glist(w, g.o_tri), 1, split);
     BSMPO_HIP(hipGetLastError());
   }
-  if (g.tri_max_n > split) {
-    hipLaunchKernelGGL((k_tridiag<512, 4>), dim3(g.n_tri), dim3(512), (size_t)g.tri_max_n * 16,
-                       stream, gdesc(g, w), glist(w, g.o_tri), split + 1, kTriMaxN);
-    BSMPO_HIP(hipGetLastError());
+  if (both) {
+    BSMPO_HIP(hipEventRecord(ss->join, ss->stream));
+    BSMPO_HIP(hipStreamWaitEvent(stream, ss->join, 0));
   }
   timing_end(tk, stream, g.flops_tridiag, g.bytes_tridiag);
   tk = timing_begin(ST_BISECT, stream);
@@ -652,8 +798,14 @@ int launch_kcount(const GroupHeader &g, const char *w, hipStream_t stream) {
 int launch_tri_vectors(const GroupHeader &g, const char *w, hipStream_t stream) {
   if (g.n_tri == 0) return 0;
   int tk = timing_begin(ST_TRIVEC, stream);
-  hipLaunchKernelGGL(k_tri_vectors, dim3(g.n_tri), dim3(kTriVecThreads),
-                     (size_t)g.tri_max_n * 16, stream, gdesc(g, w), glist(w, g.o_tri));
+  static const int nsplit = vec_lds_nsplit();
+  hipLaunchKernelGGL((k_tri_vectors<true>), dim3(g.n_tri), dim3(kVecLdsThreads), (size_t)kVecLds,
+                     stream, gdesc(g, w), glist(w, g.o_tri), nsplit);
+  BSMPO_HIP(hipGetLastError());
+  // sectors past nsplit, or with more kept vectors than the LDS lanes take
+  hipLaunchKernelGGL((k_tri_vectors<false>), dim3(g.n_tri), dim3(kTriVecThreads),
+                     (size_t)vec_lds_head(g.tri_max_n, g.tri_max_n), stream, gdesc(g, w),
+                     glist(w, g.o_tri), nsplit);
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, 0.0, 0.0);
   tk = timing_begin(ST_BACKTRANS, stream);

@@ -34,6 +34,7 @@ constexpr int kJacClasses = 5;
 constexpr int kJacLimit[kJacClasses] = {16, 32, 64, 96, 1 << 30};
 constexpr int kTriClass = kJacClasses - 1;
 constexpr int kTriPanel = 8;  // column panel of the blocked Householder reduction
+constexpr int kTriVecThreads = 128;  // inverse-iteration threads per sector
 int eig_threshold();  // sectors with n > threshold take the tridiagonal route
 double tie_rel();     // relative width of the tie band at the chi-th singular value
 

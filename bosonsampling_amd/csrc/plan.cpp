@@ -92,6 +92,7 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
     sr1 = std::min(sr1, CR[2 * i + 1]);
   }
 
+  const int thr = eig_threshold();  // sectors with n > thr take the tridiagonal route
   int64_t c_off = 0, g_off = 0, v_off = 0, w_off = 0, s_off = 0;
   double theta_elems = 0.0;
   for (int t0 = sr0; t0 <= lc0; ++t0) {
@@ -141,9 +142,11 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
       w_off += (int64_t)s.n * R;
       s_off += s.n;
       theta_elems += (double)L * R;
+      // sectors on the tridiagonal route read only the lower triangle of G
+      const bool lower_only = s.n > thr;
       {
         const double n = s.n, m = std::max(L, R);
-        b.h.flops_gram += 8.0 * m * n * n;
+        b.h.flops_gram += lower_only ? 4.0 * m * n * (n + 1) : 8.0 * m * n * n;
         b.h.flops_wmat += 8.0 * n * L * (double)R;
         if (!s.left_small) b.h.flops_vectors += 8.0 * L * (double)R * n;
         b.h.flops_jacobi += 8.0 * n * n * n;
@@ -153,7 +156,14 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
         const Block &bl = b.lblk[p.first];
         add_tiles(b.items, sid, p.first, p.second, p.second + (bl.end - bl.start), 0, R);
       }
-      add_tiles(b.gram_items, sid, 0, 0, s.n, 0, s.n);
+      if (lower_only) {  // 16 x 16 tiles on or below the diagonal
+        for (int r = 0; r < s.n; r += 16)
+          for (int c = 0; c <= r; c += 16)
+            b.gram_items.push_back(
+                Item{sid, 0, r, std::min(16, s.n - r), c, std::min(16, s.n - c), 0, 0});
+      } else {
+        add_tiles(b.gram_items, sid, 0, 0, s.n, 0, s.n);
+      }
       add_tiles(b.w_items, sid, 0, 0, s.n, 0, R);
       if (!s.left_small) add_tiles(b.v_items, sid, 0, 0, L, 0, s.n);
       b.sectors.push_back(s);
@@ -198,7 +208,6 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
 
   for (int sid = 0; sid < (int)b.sectors.size(); ++sid)
     for (int i = 0; i < b.sectors[sid].n; ++i) b.sv_sector.push_back(sid);
-  const int thr = eig_threshold();
   b.h.eig_threshold = thr;
   b.h.tie_rel = tie_rel();
   int64_t tw = 0;
@@ -217,17 +226,18 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
         if (c == kTriClass) {
           const double nn = n;
           b.h.flops_tridiag += 16.0 / 3.0 * nn * nn * nn;
-          // trailing matrix read once per column (A22 v), read + written once per panel
+          // lower triangle of the trailing matrix (m (m + 1) / 2 complex entries)
+          // read once per column (A22 v), read + written once per panel
           for (int j = 0; j + 1 < n; ++j) {
             const double m = n - 1 - j;
-            b.h.bytes_tridiag += 16.0 * m * m;
-            if (j % kTriPanel == kTriPanel - 1 || j + 2 == n) b.h.bytes_tridiag += 32.0 * m * m;
+            b.h.bytes_tridiag += 8.0 * m * (m + 1);
+            if (j % kTriPanel == kTriPanel - 1 || j + 2 == n) b.h.bytes_tridiag += 16.0 * m * (m + 1);
           }
           sc.tw_off = tw;
-          // per-thread inverse-iteration arrays (4 n doubles for each of up to
-          // min(n, 1024) threads); the Householder panels V, W (2 n kTriPanel complex)
-          // use the same area first
-          tw += (int64_t)4 * n * std::max(std::min(n, 1024), kTriPanel);
+          // per-lane inverse-iteration arrays (6 n doubles for each of the
+          // min(n, kTriVecThreads) lanes of the sector's block); the Householder
+          // panels V, W (2 n kTriPanel complex) use the same area first
+          tw += (int64_t)n * std::max(6 * std::min(n, kTriVecThreads), 4 * kTriPanel);
         }
       }
     }

@@ -341,12 +341,14 @@ def simulate_batch(states, meshes, input_modes, layers=None, precision=C128, run
 # ------------------------------------------------------------ initial state
 
 def autotune_eig_threshold(states, meshes, input_modes, layer, precision=C128, runner=None,
-                           candidates=(0, 32, 64, 96)):
+                           candidates=(0, 32, 64, 96), grouped=True):
     """Pick the split between the two sector eigensolver routes (one-sided
     Jacobi in LDS for Gram size n <= threshold, the tridiagonal route above)
-    by timing one grouped layer per candidate on clones of `states`, and set
-    the fastest with bsmpo_set_eig_threshold.  Every candidate gives the same
-    results to rounding; only the time differs.  Returns (best, {thr: s})."""
+    by timing one layer per candidate on clones of `states`, and set the
+    fastest with bsmpo_set_eig_threshold.  grouped: the layer runs as grouped
+    launches (simulate_batch), else one update at a time (simulate).  Every
+    candidate gives the same results to rounding; only the time differs.
+    Returns (best, {thr: seconds})."""
     import time
     runner = runner or GroupRunner(states[0].device)
     times = {}
@@ -355,8 +357,12 @@ def autotune_eig_threshold(states, meshes, input_modes, layer, precision=C128, r
         clones = [st.clone() for st in states]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        simulate_batch(clones, meshes, input_modes, layers=[layer], precision=precision,
-                       runner=runner)
+        if grouped:
+            simulate_batch(clones, meshes, input_modes, layers=[layer], precision=precision,
+                           runner=runner)
+        else:
+            for st, mm in zip(clones, meshes):
+                simulate(st, mm, input_modes, layers=[layer], precision=precision, observe=False)
         torch.cuda.synchronize()
         times[int(thr)] = time.perf_counter() - t0
         del clones
