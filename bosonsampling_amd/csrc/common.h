@@ -12,6 +12,33 @@ namespace bsmpo {
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
+// Pointer into global memory (address space 1).  Descriptor fields are
+// generic pointers, whose loads compile to flat instructions; those count
+// against the LDS wait counter too, so every LDS wait would also wait for
+// the global loads in flight.  Hot loops cast to this type first.  (The host
+// pass of a .hip file parses kernel bodies without generating them; there the
+// HIP vector types do not accept qualified address spaces, so it sees a plain
+// pointer.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+#else
+template <typename T>
+using gptr = T *;
+#endif
+template <typename T>
+__device__ __forceinline__ gptr<T> gp(T *p) {
+  return (gptr<T>)p;
+}
+typedef double dbl2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 gload2(gptr<const double2> p, int64_t i) {
+  const dbl2v v = ((gptr<const dbl2v>)p)[i];
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void gstore2(gptr<double2> p, int64_t i, double x, double y) {
+  ((gptr<dbl2v>)p)[i] = dbl2v{x, y};
+}
+
 struct cplx {
   double x, y;
 };

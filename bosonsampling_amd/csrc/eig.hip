@@ -146,13 +146,13 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
   const Sector s = u.sectors[ref.y];
   const int n = s.n;
   if (n < nmin || n > nmax) return;  // the other block shape's sector
-  double2 *A = reinterpret_cast<double2 *>(u.G) + s.g_off;  // A(r, c) = A[c n + r]
-  double2 *vs = tsm;                                         // v_j, n entries
-  double2 *ys = tsm + n;                                     // A22 v_j, n entries
-  double2 *Vp = reinterpret_cast<double2 *>(u.triwork + s.tw_off);
-  double2 *Wp = Vp + (int64_t)n * kTriNB;
-  double *d = u.td + s.s_off, *e = u.te + s.s_off;
-  double2 *tau = reinterpret_cast<double2 *>(u.tau) + s.s_off;
+  gptr<double2> A = gp(reinterpret_cast<double2 *>(u.G) + s.g_off);  // A(r, c) = A[c n + r]
+  double2 *vs = tsm;                                                  // v_j, n entries
+  double2 *ys = tsm + n;                                              // A22 v_j, n entries
+  gptr<double2> Vp = gp(reinterpret_cast<double2 *>(u.triwork + s.tw_off));
+  gptr<double2> Wp = Vp + (int64_t)n * kTriNB;
+  gptr<double> d = gp(u.td + s.s_off), e = gp(u.te + s.s_off);
+  gptr<double2> tau = gp(reinterpret_cast<double2 *>(u.tau) + s.s_off);
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   int par = 0;
   if (n == 1) {
@@ -204,20 +204,31 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
         double2 cp[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) cp[q] = make_double2(0.0, 0.0);
-        for (int r = s0 + lane; r < n; r += 64) {
-          double2 a[8];
+        // row blocks of 64, the next block's loads in flight while this one is summed
+        double2 a[8], b[8];
+        int r = s0 + lane;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          a[q] = (r < n && s0 + q <= r) ? A[(int64_t)(s0 + q) * n + r] : make_double2(0.0, 0.0);
+        for (int r0 = s0; r0 < n; r0 += 64) {  // wave-uniform
+          const int rn = r + 64;
 #pragma unroll
           for (int q = 0; q < 8; ++q)
-            a[q] = s0 + q <= r ? A[(int64_t)(s0 + q) * n + r] : make_double2(0.0, 0.0);
-          const double2 vr = vs[r];
-          double2 rp = make_double2(0.0, 0.0);
+            b[q] = (rn < n && s0 + q <= rn) ? A[(int64_t)(s0 + q) * n + rn] : make_double2(0.0, 0.0);
+          if (r < n) {
+            const double2 vr = vs[r];
+            double2 rp = make_double2(0.0, 0.0);
 #pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            rp = cfma2(rp, a[q], vs[min(s0 + q, n - 1)]);
-            if (s0 + q < r) cp[q] = cfma2(cp[q], make_double2(a[q].x, -a[q].y), vr);
+            for (int q = 0; q < 8; ++q) {
+              rp = cfma2(rp, a[q], vs[min(s0 + q, n - 1)]);
+              if (s0 + q < r) cp[q] = cfma2(cp[q], make_double2(a[q].x, -a[q].y), vr);
+            }
+            atomicAdd(&ys[r].x, rp.x);
+            atomicAdd(&ys[r].y, rp.y);
           }
-          atomicAdd(&ys[r].x, rp.x);
-          atomicAdd(&ys[r].y, rp.y);
+#pragma unroll
+          for (int q = 0; q < 8; ++q) a[q] = b[q];
+          r = rn;
         }
         const double2 cs = wave_colsum8(cp, lane);
         const int cq = s0 + (lane >> 3);
@@ -374,10 +385,13 @@ __device__ int sturm_below(const double *d, const double *e2, int n, double x, d
   return c;
 }
 
+// j-th largest eigenvalue in [lo, hi] by bisection on the Sturm count.  Stops
+// early once hi < below: such an eigenvalue is dropped, its value not used.
 __device__ double bisect_kth(const double *d, const double *e2, int n, int j, double lo, double hi,
-                             double pivmin) {
+                             double pivmin, double below = -1.0) {
   const int target = n - 1 - j;  // j-th largest = (n-1-j)-th smallest
   for (int it = 0; it < 128; ++it) {
+    if (hi < below) return hi;
     const double mid = 0.5 * (lo + hi);
     if (!(mid > lo && mid < hi)) break;
     if (sturm_below(d, e2, n, mid, pivmin) <= target)
@@ -389,13 +403,22 @@ __device__ double bisect_kth(const double *d, const double *e2, int n, int j, do
 }
 
 // grid (large sectors, ceil(n_max / 256)); thread -> eigenvalue index j.  The
-// tridiagonal (d, e^2) is staged in LDS; the largest eigenvalue (for the drop
-// rule) is bisected once per block, by the thread that owns index 0 in the
-// first block and by thread 0 in the others.
+// tridiagonal (d, e^2) is staged in LDS.  Each block first takes Sturm counts
+// at 256 points spaced a quarter octave apart below the Gershgorin bound (one
+// per thread), so that each eigenvalue starts from a bracket of relative width
+// 2^(1/4) - 1 instead of the whole spectrum; an eigenvalue whose bracket falls
+// below the drop threshold (kDropRel2Tri times a lower bound of the largest
+// eigenvalue) stops there.  The largest eigenvalue itself is bisected once per
+// block, by the thread that owns index 0 in the first block and by thread 0 in
+// the others.
+constexpr int kBisSamples = 256;
+constexpr double kBisOctaves = 0.25;
 __global__ __launch_bounds__(256) void k_bisect(const UpdDesc *__restrict__ Ud,
                                                 const int2 *__restrict__ list) {
   extern __shared__ double bsm[];
   __shared__ double sh_lo, sh_hi, sh_piv, sh_lmax;
+  __shared__ double xs[kBisSamples + 1];
+  __shared__ int cs[kBisSamples + 1];
   const int2 ref = list[blockIdx.x];
   const UpdDesc &u = Ud[ref.x];
   const Sector s = u.sectors[ref.y];
@@ -425,10 +448,55 @@ __global__ __launch_bounds__(256) void k_bisect(const UpdDesc *__restrict__ Ud,
   }
   __syncthreads();
   const double lo = sh_lo, hi = sh_hi, pivmin = sh_piv;
+  {  // sample counts: xs[m] = hi 2^(-m / 4), m < 256, decreasing; xs[256] = lo
+    const int m = threadIdx.x;
+    const double x = hi * exp2(-kBisOctaves * m);
+    xs[m] = x;
+    cs[m] = (hi > 0.0 && x > lo) ? sturm_below(ds, e2s, n, x, pivmin) : 0;
+    if (m == 0) {
+      xs[kBisSamples] = lo;
+      cs[kBisSamples] = 0;
+    }
+  }
+  __syncthreads();
+  // a lower bound of the largest eigenvalue: the largest sample with an
+  // eigenvalue at or above it
+  double lmax_lb = 0.0;
+  for (int m = 0; m < kBisSamples; ++m)
+    if (cs[m] < n) {
+      lmax_lb = xs[m];
+      break;
+    }
+  // bracket of the j-th largest: [xs[m + 1], xs[m]] with cs[m] > n-1-j >= cs[m + 1]
+  // (cs[0] = n, cs[256] = 0); the whole [lo, hi] when no eigenvalue is positive
+  auto bracket = [&](int jj, double &blo, double &bhi) {
+    if (!(cs[0] == n)) {
+      blo = lo;
+      bhi = hi;
+      return;
+    }
+    const int target = n - 1 - jj;
+    int m = 0;
+    while (cs[m + 1] > target) ++m;  // stops at m = 255 at the latest
+    blo = fmax(xs[m + 1], lo);
+    bhi = xs[m];
+  };
   const int j = blockIdx.y * blockDim.x + threadIdx.x;
   double l = 0.0;
-  if (j < n) l = bisect_kth(ds, e2s, n, j, lo, hi, pivmin);
-  if (threadIdx.x == 0) sh_lmax = blockIdx.y == 0 ? l : bisect_kth(ds, e2s, n, 0, lo, hi, pivmin);
+  if (j < n) {
+    double blo, bhi;
+    bracket(j, blo, bhi);
+    l = bisect_kth(ds, e2s, n, j, blo, bhi, pivmin, kDropRel2Tri * lmax_lb);
+  }
+  if (threadIdx.x == 0) {
+    if (blockIdx.y == 0) {
+      sh_lmax = l;
+    } else {
+      double blo, bhi;
+      bracket(0, blo, bhi);
+      sh_lmax = bisect_kth(ds, e2s, n, 0, blo, bhi, pivmin);
+    }
+  }
   __syncthreads();
   if (j >= n) return;
   const double lmax = sh_lmax;
@@ -465,14 +533,16 @@ __global__ void k_kcount(const UpdDesc *__restrict__ Ud, const int2 *__restrict_
 // close eigenvalues run in one lane, Gram-Schmidt against earlier members.
 // The five factor arrays and the iterate (6 n doubles per lane) live in LDS
 // for sectors with n <= nsplit and few kept vectors per lane; the other
-// sectors keep them in their tridiagonal work area, interleaved over lanes.
+// sectors keep them in their tridiagonal work area, interleaved over 128
+// lanes (global address space, so that LDS waits do not wait on them).
 constexpr int kVecArrays = 6;
 constexpr int kVecLds = 40960;        // dynamic LDS of the LDS-resident variant
 constexpr int kVecLdsThreads = 64;
 constexpr int kVecSerial = 4;         // at most this many vectors per LDS lane
+constexpr int kVecStep = 8;           // solve steps whose loads are issued together
 
-__host__ __device__ inline int vec_lds_head(int n, int k) {  // d, e, cluster starts (bytes)
-  return (16 * n + 4 * (k + 1) + 15) / 16 * 16;
+__host__ __device__ inline int vec_lds_head(int n, int k) {  // d, e, lambda, cluster starts (bytes)
+  return (16 * n + 8 * k + 4 * (k + 1) + 15) / 16 * 16;
 }
 __host__ __device__ inline int vec_lds_lanes(int n, int k) {
   const int room = kVecLds - vec_lds_head(n, k);
@@ -487,65 +557,13 @@ __device__ __forceinline__ double vec_start(int i, int j, int n) {
   return 1.0 / sqrt((double)n) * (1.0 + 1e-3 * ((i * 7919 + j * 104729) % 1000) / 1000.0);
 }
 
-template <bool kInLds>
-__global__ __launch_bounds__(kInLds ? kVecLdsThreads : kTriVecThreads) void k_tri_vectors(
-    const UpdDesc *__restrict__ Ud, const int2 *__restrict__ list, int nsplit) {
-  extern __shared__ double tvs[];  // d, e (2 n), cluster starts (k + 1), lane arrays (LDS variant)
-  __shared__ int ncl;
-  __shared__ double sh_tiny;
-  const int2 ref = list[blockIdx.x];
-  const UpdDesc &u = Ud[ref.x];
-  const Sector s = u.sectors[ref.y];
-  const int n = s.n, k = u.kcount[ref.y];
-  if (k == 0 || vec_in_lds(n, k, nsplit) != kInLds) return;  // nothing kept, or the other variant's
-  const double *l = u.lam + s.s_off;
-  double *d = tvs, *e = tvs + n;
-  int *cstart = reinterpret_cast<int *>(tvs + 2 * n);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    d[i] = u.td[s.s_off + i];
-    e[i] = u.te[s.s_off + i];
-  }
-  __syncthreads();
-  double2 *X = s.left_small ? reinterpret_cast<double2 *>(u.V) + s.v_off
-                            : reinterpret_cast<double2 *>(u.W) + s.w_off;  // column j: X[j n + r]
-  // clusters of close eigenvalues (gap <= kClusterRel |T|) are orthogonalised
-  // explicitly; outside them inverse iteration loses at most
-  // eps |T| / gap <= 2e-10 orthogonality (LAPACK's stein uses 1e-3 |T|, which
-  // would chain most kept eigenvalues of a sector into one serial cluster)
-  if (threadIdx.x == 0) {
-    double tn = 0.0, tn1 = 0.0;
-    for (int i = 0; i < n; ++i) {
-      tn = fmax(tn, fabs(d[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0));
-      tn1 = fmax(tn1, fabs(d[i]) + (i + 1 < n ? 2.0 * fabs(e[i]) : 0.0));
-    }
-    const double ortol = kClusterRel * tn;
-    int c = 0;
-    for (int j = 0; j < k; ++j)
-      if (j == 0 || fabs(l[j - 1] - l[j]) > ortol) cstart[c++] = j;
-    cstart[c] = k;
-    ncl = c;
-    sh_tiny = DBL_EPSILON * fmax(tn1, DBL_MIN);
-  }
-  __syncthreads();
-  const double tiny = sh_tiny;
-  int P, si;
-  int64_t sa;
-  double *base;
-  if (kInLds) {
-    P = vec_lds_lanes(n, k);
-    base = reinterpret_cast<double *>(reinterpret_cast<char *>(tvs) + vec_lds_head(n, k)) +
-           (int64_t)threadIdx.x * kVecArrays * n;
-    sa = n;
-    si = 1;
-  } else {
-    P = min(n, (int)blockDim.x);
-    base = u.triwork + s.tw_off + threadIdx.x;
-    sa = (int64_t)n * P;
-    si = P;
-  }
-  if ((int)threadIdx.x >= P) return;
-  double *rd = base, *du = base + sa, *dl = base + 2 * sa, *fm = base + 3 * sa,
-         *sw = base + 4 * sa, *b = base + 5 * sa;
+// One lane's clusters: factor arrays at base + a * sa + i * si (a = 0..5).
+template <typename PT>
+__device__ __forceinline__ void vec_lanes(PT base, int64_t sa, int si, int P, int n, int ncl,
+                                          const int *cstart, const double *d, const double *e,
+                                          const double *l, double tiny, gptr<double2> X) {
+  PT rd = base, du = base + sa, dl = base + 2 * sa, fm = base + 3 * sa, sw = base + 4 * sa,
+     b = base + 5 * sa;
   for (int c = threadIdx.x; c < ncl; c += P) {
     for (int j = cstart[c]; j < cstart[c + 1]; ++j) {
       const double lam = l[j];
@@ -571,23 +589,53 @@ __global__ __launch_bounds__(kInLds ? kVecLdsThreads : kTriVecThreads) void k_tr
       for (int i = 0; i < n; ++i) b[i * si] = vec_start(i, j, n);
       double scale = 1.0;
       for (int it = 0; it < 3; ++it) {
-        // y = L^-1 P (scale b), in place
+        // y = L^-1 P (scale b), in place; steps in blocks of kVecStep whose
+        // loads are all issued before the block's dependent chain
         double cb = b[0] * scale;
-        for (int i = 0; i + 1 < n; ++i) {
-          const double nb = b[(i + 1) * si] * scale, f = fm[i * si];
-          const bool swp = sw[i * si] != 0.0;
-          b[i * si] = swp ? nb : cb;
-          cb = swp ? cb - f * nb : nb - f * cb;
+        for (int i0 = 0; i0 + 1 < n; i0 += kVecStep) {
+          double nbv[kVecStep], fv[kVecStep], sv[kVecStep];
+#pragma unroll
+          for (int q = 0; q < kVecStep; ++q) {
+            const int i = min(i0 + q, n - 2);
+            nbv[q] = b[(i + 1) * si];
+            fv[q] = fm[i * si];
+            sv[q] = sw[i * si];
+          }
+#pragma unroll
+          for (int q = 0; q < kVecStep; ++q) {
+            const int i = i0 + q;
+            if (i + 1 < n) {
+              const double nb = nbv[q] * scale;
+              const bool swp = sv[q] != 0.0;
+              b[i * si] = swp ? nb : cb;
+              cb = swp ? cb - fv[q] * nb : nb - fv[q] * cb;
+            }
+          }
         }
         // x = U^-1 y, in place, with |x|^2
         double x1 = cb * rd[(n - 1) * si], x2 = 0.0, nr = x1 * x1;
         b[(n - 1) * si] = x1;
-        for (int i = n - 2; i >= 0; --i) {
-          const double x = (b[i * si] - du[i * si] * x1 - dl[i * si] * x2) * rd[i * si];
-          b[i * si] = x;
-          nr += x * x;
-          x2 = x1;
-          x1 = x;
+        for (int i0 = n - 2; i0 >= 0; i0 -= kVecStep) {
+          double yv[kVecStep], uv[kVecStep], lv[kVecStep], rv[kVecStep];
+#pragma unroll
+          for (int q = 0; q < kVecStep; ++q) {
+            const int i = max(i0 - q, 0);
+            yv[q] = b[i * si];
+            uv[q] = du[i * si];
+            lv[q] = dl[i * si];
+            rv[q] = rd[i * si];
+          }
+#pragma unroll
+          for (int q = 0; q < kVecStep; ++q) {
+            const int i = i0 - q;
+            if (i >= 0) {
+              const double x = (yv[q] - uv[q] * x1 - lv[q] * x2) * rv[q];
+              b[i * si] = x;
+              nr += x * x;
+              x2 = x1;
+              x1 = x;
+            }
+          }
         }
         if (j > cstart[c]) {  // Gram-Schmidt against the cluster's earlier (unit) vectors
           for (int q = cstart[c]; q < j; ++q) {
@@ -602,7 +650,88 @@ __global__ __launch_bounds__(kInLds ? kVecLdsThreads : kTriVecThreads) void k_tr
       }
       for (int i = 0; i < n; ++i) X[(int64_t)j * n + i] = make_double2(b[i * si] * scale, 0.0);
     }
+  }}
+
+template <bool kInLds>
+__global__ __launch_bounds__(kInLds ? kVecLdsThreads : kTriVecThreads) void k_tri_vectors(
+    const UpdDesc *__restrict__ Ud, const int2 *__restrict__ list, int nsplit) {
+  extern __shared__ double tvs[];  // d, e (2 n), lambda (k), cluster starts (k + 1), lane arrays
+  __shared__ int ncl;
+  __shared__ double sh_tn[2][kTriVecThreads / 64];
+  const int2 ref = list[blockIdx.x];
+  const UpdDesc &u = Ud[ref.x];
+  const Sector s = u.sectors[ref.y];
+  const int n = s.n, k = u.kcount[ref.y];
+  if (k == 0 || vec_in_lds(n, k, nsplit) != kInLds) return;  // nothing kept, or the other variant's
+  double *d = tvs, *e = tvs + n, *l = tvs + 2 * n;
+  int *cstart = reinterpret_cast<int *>(tvs + 2 * n + k);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    d[i] = u.td[s.s_off + i];
+    e[i] = u.te[s.s_off + i];
   }
+  for (int j = threadIdx.x; j < k; j += blockDim.x) l[j] = u.lam[s.s_off + j];
+  __syncthreads();
+  double2 *X = s.left_small ? reinterpret_cast<double2 *>(u.V) + s.v_off
+                            : reinterpret_cast<double2 *>(u.W) + s.w_off;  // column j: X[j n + r]
+  // |T| bounds (row sums) by a block max
+  double tn = 0.0, tn1 = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double ei = i + 1 < n ? fabs(e[i]) : 0.0;
+    tn = fmax(tn, fabs(d[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0) + ei);
+    tn1 = fmax(tn1, fabs(d[i]) + 2.0 * ei);
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    tn = fmax(tn, __shfl_xor(tn, o));
+    tn1 = fmax(tn1, __shfl_xor(tn1, o));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh_tn[0][threadIdx.x >> 6] = tn;
+    sh_tn[1][threadIdx.x >> 6] = tn1;
+  }
+  __syncthreads();
+  tn = tn1 = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+    tn = fmax(tn, sh_tn[0][w]);
+    tn1 = fmax(tn1, sh_tn[1][w]);
+  }
+  // clusters of close eigenvalues (gap <= kClusterRel |T|) are orthogonalised
+  // explicitly; outside them inverse iteration loses at most
+  // eps |T| / gap <= 2e-10 orthogonality (LAPACK's stein uses 1e-3 |T|, which
+  // would chain most kept eigenvalues of a sector into one serial cluster)
+  const double ortol = kClusterRel * tn;
+  for (int j = threadIdx.x; j < k; j += blockDim.x)
+    cstart[j] = (j == 0 || fabs(l[j - 1] - l[j]) > ortol) ? 1 : 0;  // cluster head flags
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int c = 0;
+    for (int j = 0; j < k; ++j)
+      if (cstart[j]) cstart[c++] = j;  // compaction in place (c <= j)
+    cstart[c] = k;
+    ncl = c;
+  }
+  __syncthreads();
+  const double tiny = DBL_EPSILON * fmax(tn1, DBL_MIN);
+
+  int P, si;
+  int64_t sa;
+  double *base;
+  if (kInLds) {
+    P = vec_lds_lanes(n, k);
+    base = reinterpret_cast<double *>(reinterpret_cast<char *>(tvs) + vec_lds_head(n, k)) +
+           (int64_t)threadIdx.x * kVecArrays * n;
+    sa = n;
+    si = 1;
+  } else {
+    P = min(n, (int)blockDim.x);
+    base = u.triwork + s.tw_off + threadIdx.x;
+    sa = (int64_t)n * P;
+    si = P;
+  }
+  if ((int)threadIdx.x >= P) return;
+  if constexpr (kInLds)
+    vec_lanes(base, sa, si, P, n, ncl, cstart, d, e, l, tiny, gp(X));
+  else
+    vec_lanes(gp(base), sa, si, P, n, ncl, cstart, d, e, l, tiny, gp(X));
 }
 
 // largest n whose LDS variant holds at least one lane (k = n at most)

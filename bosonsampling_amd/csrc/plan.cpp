@@ -149,7 +149,7 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
         b.h.flops_gram += lower_only ? 4.0 * m * n * (n + 1) : 8.0 * m * n * n;
         b.h.flops_wmat += 8.0 * n * L * (double)R;
         if (!s.left_small) b.h.flops_vectors += 8.0 * L * (double)R * n;
-        b.h.flops_jacobi += 8.0 * n * n * n;
+        if (!lower_only) b.h.flops_jacobi += 8.0 * n * n * n;
       }
       int sid = (int)b.sectors.size();
       for (auto &p : lsel) {

@@ -248,34 +248,23 @@ class GroupRunner:
         return i, self.hosts[i]
 
     def run(self, updates, precision: int = C128):
-        """updates: list of (DeviceMPO, mode, U3 device table)."""
+        """updates: list of (DeviceMPO, mode, U3 device table).  Plans are
+        built on the host group by group, so that planning the next group
+        overlaps the device's work on the previous one."""
         if not updates:
             return
-        plans = []
-        for st, mode, U3 in updates:
-            ch = st.charge_host
-            plans.append(Plan(st.d, st.chi, ch[mode], ch[mode + 1], ch[mode + 2]))
-        # split into groups that fit the workspace bound
-        groups, cur, cur_bytes = [], [], 0
-        for k, p in enumerate(plans):
-            est = p.work_bytes
-            if cur and cur_bytes + est > self.max_bytes:
-                groups.append(cur)
-                cur, cur_bytes = [], 0
-            cur.append(k)
-            cur_bytes += est
-        if cur:
-            groups.append(cur)
         stream = _stream()
-        for grp in groups:
+        n_groups = 0
+
+        def launch(grp):
             n = len(grp)
-            arr = (C.c_void_p * n)(*[plans[k].address for k in grp])
+            arr = (C.c_void_p * n)(*[p.address for _, p in grp])
             hb, wb = C.c_size_t(), C.c_size_t()
             check(lib().bsmpo_group_query(n, arr, C.byref(hb), C.byref(wb)), "group_query")
             work = self._work(wb.value)
             slot, host = self._host(hb.value)
             io = (_capi.UpdateIO * n)()
-            for j, k in enumerate(grp):
+            for j, (k, _) in enumerate(grp):
                 st, mode, U3 = updates[k]
                 LR = st.edge if mode == st.M - 2 else st.Lambda[mode + 1]
                 io[j] = _capi.UpdateIO(st.Gamma[mode].data_ptr(), st.Gamma[mode + 1].data_ptr(),
@@ -304,7 +293,21 @@ class GroupRunner:
             ev = torch.cuda.Event()
             ev.record()
             self.events[slot] = ev
-        self.last_info = {"groups": len(groups), "updates": len(updates)}
+
+        cur, cur_bytes = [], 0
+        for k, (st, mode, U3) in enumerate(updates):
+            ch = st.charge_host
+            p = Plan(st.d, st.chi, ch[mode], ch[mode + 1], ch[mode + 2])
+            if cur and cur_bytes + p.work_bytes > self.max_bytes:
+                launch(cur)  # asynchronous: the next plans are built meanwhile
+                n_groups += 1
+                cur, cur_bytes = [], 0
+            cur.append((k, p))
+            cur_bytes += p.work_bytes
+        if cur:
+            launch(cur)
+            n_groups += 1
+        self.last_info = {"groups": n_groups, "updates": len(updates)}
         # host mirror of the new centre charges: one device-to-host copy
         rows = [st.charge[mode + 1] for st, mode, _ in updates]
         new = torch.stack(rows).cpu().numpy()
