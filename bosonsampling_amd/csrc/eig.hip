@@ -76,6 +76,7 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
 // 512 threads x 4 rows (n <= 2048); each launch skips the other size class.
 constexpr int kTriNB = kTriPanel;
 constexpr int kTriChunk = 64;  // trailing-update columns staged in LDS at a time
+constexpr int kTriCols = 4;    // trailing-update columns whose loads are issued together
 constexpr int kTriMaxN = 2048;
 constexpr int kTriFewSectors = 256;  // below this many sectors per launch: one block shape
 
@@ -349,16 +350,24 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
         __syncthreads();
         if (rv) {
           const int ncr = min(nc, r - c0 + 1);
-          for (int cc = 0; cc < ncr; ++cc) {
-            const int64_t o = (int64_t)(c0 + cc) * n + r;
-            double2 a = A[o];
+          // kTriCols columns at a time, their loads issued before the updates
+          for (int cb = 0; cb < ncr; cb += kTriCols) {
+            double2 a[kTriCols];
 #pragma unroll
-            for (int l = 0; l < kTriNB; ++l) {
-              const double2 x = cw[cc][l], y = cv[cc][l];
-              a.x -= vr[l].x * x.x - vr[l].y * x.y + wr[l].x * y.x - wr[l].y * y.y;
-              a.y -= vr[l].x * x.y + vr[l].y * x.x + wr[l].x * y.y + wr[l].y * y.x;
+            for (int q = 0; q < kTriCols; ++q)
+              a[q] = cb + q < ncr ? A[(int64_t)(c0 + cb + q) * n + r] : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int q = 0; q < kTriCols; ++q) {
+              const int cc = cb + q;
+              if (cc >= ncr) break;
+#pragma unroll
+              for (int l = 0; l < kTriNB; ++l) {
+                const double2 x = cw[cc][l], y = cv[cc][l];
+                a[q].x -= vr[l].x * x.x - vr[l].y * x.y + wr[l].x * y.x - wr[l].y * y.y;
+                a[q].y -= vr[l].x * x.y + vr[l].y * x.x + wr[l].x * y.y + wr[l].y * y.x;
+              }
+              A[(int64_t)(c0 + cc) * n + r] = a[q];
             }
-            A[o] = a;
           }
         }
       }
@@ -539,7 +548,8 @@ constexpr int kVecArrays = 6;
 constexpr int kVecLds = 40960;        // dynamic LDS of the LDS-resident variant
 constexpr int kVecLdsThreads = 64;
 constexpr int kVecSerial = 4;         // at most this many vectors per LDS lane
-constexpr int kVecStep = 8;           // solve steps whose loads are issued together
+constexpr int kVecStepLds = 8;        // solve steps whose loads are issued together (LDS)
+constexpr int kVecStepGlobal = 16;    // the same for arrays in global memory
 
 __host__ __device__ inline int vec_lds_head(int n, int k) {  // d, e, lambda, cluster starts (bytes)
   return (16 * n + 8 * k + 4 * (k + 1) + 15) / 16 * 16;
@@ -558,7 +568,7 @@ __device__ __forceinline__ double vec_start(int i, int j, int n) {
 }
 
 // One lane's clusters: factor arrays at base + a * sa + i * si (a = 0..5).
-template <typename PT>
+template <int kVecStep, typename PT>
 __device__ __forceinline__ void vec_lanes(PT base, int64_t sa, int si, int P, int n, int ncl,
                                           const int *cstart, const double *d, const double *e,
                                           const double *l, double tiny, gptr<double2> X) {
@@ -729,9 +739,9 @@ __global__ __launch_bounds__(kInLds ? kVecLdsThreads : kTriVecThreads) void k_tr
   }
   if ((int)threadIdx.x >= P) return;
   if constexpr (kInLds)
-    vec_lanes(base, sa, si, P, n, ncl, cstart, d, e, l, tiny, gp(X));
+    vec_lanes<kVecStepLds>(base, sa, si, P, n, ncl, cstart, d, e, l, tiny, gp(X));
   else
-    vec_lanes(gp(base), sa, si, P, n, ncl, cstart, d, e, l, tiny, gp(X));
+    vec_lanes<kVecStepGlobal>(gp(base), sa, si, P, n, ncl, cstart, d, e, l, tiny, gp(X));
 }
 
 // largest n whose LDS variant holds at least one lane (k = n at most)
@@ -746,21 +756,41 @@ static int vec_lds_nsplit() {
 constexpr int kBtChunk = 8;
 constexpr int kBtLds = 4096;  // complex elements of the reflector cache (64 KiB)
 template <int kBtRows>
-__global__ __launch_bounds__(kBtRows == 16 ? 1024 : 256) void k_tri_backtransform(
-    const UpdDesc *__restrict__ Ud, const int2 *__restrict__ list) {
+__global__ __launch_bounds__(kBtRows <= 16 ? 1024 : 256) void k_tri_backtransform(
+    const UpdDesc *__restrict__ Ud, const int2 *__restrict__ list, int nmin) {
   extern __shared__ double2 vsm[];  // reflector cache: per x n
   const int2 ref = list[blockIdx.x];
   const UpdDesc &u = Ud[ref.x];
   const Sector s = u.sectors[ref.y];
   const int n = s.n, k = u.kcount[ref.y];
-  if (k == 0) return;
+  if (k == 0 || n < nmin || n > 64 * kBtRows) return;  // nothing kept, or another size class
   const int per = max(1, min(kBtChunk, kBtLds / max(n, 1)));
-  double2 *X = s.left_small ? reinterpret_cast<double2 *>(u.V) + s.v_off
-                            : reinterpret_cast<double2 *>(u.W) + s.w_off;
-  const double2 *A = reinterpret_cast<const double2 *>(u.G) + s.g_off;
-  const double2 *tau = reinterpret_cast<const double2 *>(u.tau) + s.s_off;
+  gptr<double2> X = gp(s.left_small ? reinterpret_cast<double2 *>(u.V) + s.v_off
+                                     : reinterpret_cast<double2 *>(u.W) + s.w_off);
+  const gptr<const double2> A = gp(reinterpret_cast<const double2 *>(u.G) + s.g_off);
+  const gptr<const double2> tau = gp(reinterpret_cast<const double2 *>(u.tau) + s.s_off);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   constexpr int kBtVec = 1;  // n <= 64 * kBtRows
+  // the chunk ilo..ihi of reflector columns (zero above the unit entry); for
+  // n <= 512 it is read into registers one chunk ahead of its use
+  constexpr bool kAhead = kBtRows <= 8;
+  constexpr int kPre = kAhead ? kBtLds / 1024 : 1;  // staged elements per thread
+  double2 pre[kPre];
+  auto column = [&](int e, int ilo) {
+    const int i = ilo + e / n, r = e % n;
+    return r <= i ? make_double2(0.0, 0.0)
+                  : (r == i + 1 ? make_double2(1.0, 0.0) : gload2(A, (int64_t)i * n + r));
+  };
+  auto fetch = [&](int ihi) {
+    if constexpr (kAhead) {
+      const int ilo = max(0, ihi - per + 1), cnt = (ihi - ilo + 1) * n;
+#pragma unroll
+      for (int q = 0; q < kPre; ++q) {
+        const int e = threadIdx.x + q * (int)blockDim.x;
+        pre[q] = (ihi < 0 || e >= cnt) ? make_double2(0.0, 0.0) : column(e, ilo);
+      }
+    }
+  };
   for (int j0 = 0; j0 < k; j0 += nw * kBtVec) {
     double2 x[kBtVec][kBtRows];
     const int jb = j0 + wave * kBtVec;
@@ -772,15 +802,21 @@ __global__ __launch_bounds__(kBtRows == 16 ? 1024 : 256) void k_tri_backtransfor
         x[u][q] = (jb + u < k && r < n) ? X[(int64_t)(jb + u) * n + r] : make_double2(0.0, 0.0);
       }
     // reflectors n-2 .. 0, staged `per` at a time in LDS (one barrier pair per chunk)
+    fetch(n - 2);
     for (int ihi = n - 2; ihi >= 0; ihi -= per) {
       const int ilo = max(0, ihi - per + 1);
       __syncthreads();
-      for (int e = threadIdx.x; e < (ihi - ilo + 1) * n; e += blockDim.x) {
-        const int i = ilo + e / n, r = e % n;
-        vsm[e] = r <= i ? make_double2(0.0, 0.0)
-                        : (r == i + 1 ? make_double2(1.0, 0.0) : A[(int64_t)i * n + r]);
+      if constexpr (kAhead) {
+#pragma unroll
+        for (int q = 0; q < kPre; ++q) {
+          const int e = threadIdx.x + q * (int)blockDim.x;
+          if (e < (ihi - ilo + 1) * n) vsm[e] = pre[q];
+        }
+      } else {
+        for (int e = threadIdx.x; e < (ihi - ilo + 1) * n; e += blockDim.x) vsm[e] = column(e, ilo);
       }
       __syncthreads();
+      fetch(ihi - per);  // next chunk's loads in flight during this chunk
       for (int i = ihi; i >= ilo; --i) {
         const double2 ti = tau[i];
         if (ti.x == 0.0 && ti.y == 0.0) continue;  // uniform
@@ -938,14 +974,33 @@ int launch_tri_vectors(const GroupHeader &g, const char *w, hipStream_t stream) 
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, 0.0, 0.0);
   tk = timing_begin(ST_BACKTRANS, stream);
+  // one launch per sector size class (rows per lane); the classes above 512,
+  // few long-running sectors, go to the side stream to overlap the bulk
   const size_t lds = (size_t)kBtLds * 16;
-  if (g.tri_max_n <= 1024)
-    hipLaunchKernelGGL(k_tri_backtransform<16>, dim3(g.n_tri), dim3(1024), lds, stream,
-                       gdesc(g, w), glist(w, g.o_tri));
-  else
-    hipLaunchKernelGGL(k_tri_backtransform<32>, dim3(g.n_tri), dim3(256), lds, stream,
-                       gdesc(g, w), glist(w, g.o_tri));
+  hipStream_t big = stream;
+  SideStream *ss = nullptr;
+  if (g.tri_max_n > 512) {
+    int rc = side_stream(&ss);
+    if (rc) return rc;
+    BSMPO_HIP(hipEventRecord(ss->fork, stream));
+    BSMPO_HIP(hipStreamWaitEvent(ss->stream, ss->fork, 0));
+    big = ss->stream;
+    hipLaunchKernelGGL(k_tri_backtransform<16>, dim3(g.n_tri), dim3(1024), lds, big,
+                       gdesc(g, w), glist(w, g.o_tri), 513);
+    BSMPO_HIP(hipGetLastError());
+    if (g.tri_max_n > 1024) {
+      hipLaunchKernelGGL(k_tri_backtransform<32>, dim3(g.n_tri), dim3(256), lds, big,
+                         gdesc(g, w), glist(w, g.o_tri), 1025);
+      BSMPO_HIP(hipGetLastError());
+    }
+  }
+  hipLaunchKernelGGL(k_tri_backtransform<8>, dim3(g.n_tri), dim3(1024), lds, stream, gdesc(g, w),
+                     glist(w, g.o_tri), 1);
   BSMPO_HIP(hipGetLastError());
+  if (ss) {
+    BSMPO_HIP(hipEventRecord(ss->join, ss->stream));
+    BSMPO_HIP(hipStreamWaitEvent(stream, ss->join, 0));
+  }
   timing_end(tk, stream, 0.0, 0.0);
   return 0;
 }
