@@ -76,7 +76,7 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
 // 512 threads x 4 rows (n <= 2048); each launch skips the other size class.
 constexpr int kTriNB = kTriPanel;
 constexpr int kTriChunk = 64;  // trailing-update columns staged in LDS at a time
-constexpr int kTriCols = 4;    // trailing-update columns whose loads are issued together
+constexpr int kTriCols = 8;    // trailing-update columns whose loads are issued together
 constexpr int kTriMaxN = 2048;
 constexpr int kTriFewSectors = 256;  // below this many sectors per launch: one block shape
 
@@ -550,6 +550,7 @@ constexpr int kVecLdsThreads = 64;
 constexpr int kVecSerial = 4;         // at most this many vectors per LDS lane
 constexpr int kVecStepLds = 8;        // solve steps whose loads are issued together (LDS)
 constexpr int kVecStepGlobal = 16;    // the same for arrays in global memory
+constexpr int kVecGs = 8;             // cluster vectors projected out per pass
 
 __host__ __device__ inline int vec_lds_head(int n, int k) {  // d, e, lambda, cluster starts (bytes)
   return (16 * n + 8 * k + 4 * (k + 1) + 15) / 16 * 16;
@@ -647,11 +648,28 @@ __device__ __forceinline__ void vec_lanes(PT base, int64_t sa, int si, int P, in
             }
           }
         }
-        if (j > cstart[c]) {  // Gram-Schmidt against the cluster's earlier (unit) vectors
-          for (int q = cstart[c]; q < j; ++q) {
-            double dot = 0.0;
-            for (int i = 0; i < n; ++i) dot += X[(int64_t)q * n + i].x * b[i * si];
-            for (int i = 0; i < n; ++i) b[i * si] -= dot * X[(int64_t)q * n + i].x;
+        if (j > cstart[c]) {
+          // Gram-Schmidt against the cluster's earlier (unit) vectors, kVecGs
+          // at a time: all their projections in one pass over the rows (the
+          // loads of a row are independent), then one combined update
+          for (int qb = cstart[c]; qb < j; qb += kVecGs) {
+            const int nq = min(kVecGs, j - qb);
+            double dot[kVecGs];
+#pragma unroll
+            for (int t = 0; t < kVecGs; ++t) dot[t] = 0.0;
+            for (int i = 0; i < n; ++i) {
+              const double bi = b[i * si];
+#pragma unroll
+              for (int t = 0; t < kVecGs; ++t)
+                if (t < nq) dot[t] += X[(int64_t)(qb + t) * n + i].x * bi;
+            }
+            for (int i = 0; i < n; ++i) {
+              double bi = b[i * si];
+#pragma unroll
+              for (int t = 0; t < kVecGs; ++t)
+                if (t < nq) bi -= dot[t] * X[(int64_t)(qb + t) * n + i].x;
+              b[i * si] = bi;
+            }
           }
           nr = 0.0;
           for (int i = 0; i < n; ++i) nr += b[i * si] * b[i * si];
