@@ -37,6 +37,8 @@ __device__ __forceinline__ cx<T> cxfma(cx<T> acc, cx<T> a, cx<T> b) {
   return acc;
 }
 
+constexpr int kPairRows = 4;  // centre rows per batch of loads in phase 1
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_contract_pair(const UpdDesc *__restrict__ Ud,
                                                        const int2 *__restrict__ list, int nlist,
@@ -64,9 +66,10 @@ __global__ __launch_bounds__(256) void k_contract_pair(const UpdDesc *__restrict
       const double2 v = U[((A1 - t) * d + t) * d + (A1 - c)];
       bc_[t * kPairMaxB + c] = cx<T>{(T)v.x, (T)-v.y};
     }
-    // (1) P[q][e] over the centre blocks of the box
-    const double2 *G1 = reinterpret_cast<const double2 *>(u.Glc);
-    const double2 *G2 = reinterpret_cast<const double2 *>(u.Gcr);
+    // (1) P[q][e] over the centre blocks of the box; the rows of a block are
+    //     taken kPairRows at a time with their loads issued together
+    const gptr<const double2> G1 = gp(reinterpret_cast<const double2 *>(u.Glc));
+    const gptr<const double2> G2 = gp(reinterpret_cast<const double2 *>(u.Gcr));
     for (int idx = threadIdx.x; idx < B0 * B1 * E; idx += blockDim.x) {
       const int q = idx / E, e = idx % E;
       const int q0 = q / B1, q1 = q % B1;
@@ -75,16 +78,25 @@ __global__ __launch_bounds__(256) void k_contract_pair(const UpdDesc *__restrict
       if (cb >= 0) {
         const Block bk = u.cblk[cb];
         const int l = la.start + pi.l0 + e / tr, r = rb.start + pi.r0 + e % tr;
-        const double2 *gl = G1 + (int64_t)l * chi;
-        for (int c = bk.start; c < bk.end; ++c) {
-          double2 x = gl[c];
-          if (u.LC) {
-            const double w = u.LC[c];
-            x.x *= w;
-            x.y *= w;
+        const gptr<const double2> gl = G1 + (int64_t)l * chi;
+        for (int c0 = bk.start; c0 < bk.end; c0 += kPairRows) {
+          double2 x[kPairRows], y[kPairRows];
+#pragma unroll
+          for (int k = 0; k < kPairRows; ++k) {
+            const int c = min(c0 + k, bk.end - 1);
+            x[k] = gload2(gl, c);
+            y[k] = gload2(G2, (int64_t)c * chi + r);
           }
-          const double2 y = G2[(int64_t)c * chi + r];
-          acc = cxfma(acc, cx<T>{(T)x.x, (T)x.y}, cx<T>{(T)y.x, (T)y.y});
+#pragma unroll
+          for (int k = 0; k < kPairRows; ++k) {
+            if (c0 + k >= bk.end) break;
+            if (u.LC) {
+              const double w = u.LC[c0 + k];
+              x[k].x *= w;
+              x[k].y *= w;
+            }
+            acc = cxfma(acc, cx<T>{(T)x[k].x, (T)x[k].y}, cx<T>{(T)y[k].x, (T)y[k].y});
+          }
         }
       }
       P[idx] = acc;

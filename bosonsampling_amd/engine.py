@@ -344,13 +344,15 @@ def simulate_batch(states, meshes, input_modes, layers=None, precision=C128, run
 # ------------------------------------------------------------ initial state
 
 def autotune_eig_threshold(states, meshes, input_modes, layer, precision=C128, runner=None,
-                           candidates=(0, 32, 64, 96), grouped=True):
+                           candidates=(0, 32, 64, 96), grouped=True, tie=0.02):
     """Pick the split between the two sector eigensolver routes (one-sided
     Jacobi in LDS for Gram size n <= threshold, the tridiagonal route above)
     by timing one layer per candidate on clones of `states`, and set the
     fastest with bsmpo_set_eig_threshold.  grouped: the layer runs as grouped
     launches (simulate_batch), else one update at a time (simulate).  Every
     candidate gives the same results to rounding; only the time differs.
+    Candidates within `tie` (relative) of the fastest count as equal and the
+    smallest of them is taken, so that timing noise does not flip the choice.
     Returns (best, {thr: seconds})."""
     import time
     runner = runner or GroupRunner(states[0].device)
@@ -369,7 +371,8 @@ def autotune_eig_threshold(states, meshes, input_modes, layer, precision=C128, r
         torch.cuda.synchronize()
         times[int(thr)] = time.perf_counter() - t0
         del clones
-    best = min(times, key=times.get)
+    fastest = min(times.values())
+    best = min(t for t, sec in times.items() if sec <= (1.0 + tie) * fastest)
     check(lib().bsmpo_set_eig_threshold(best), "set_eig_threshold")
     return best, times
 
