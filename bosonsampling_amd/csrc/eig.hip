@@ -75,8 +75,6 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
 // one barrier.  Two block shapes: 256 threads x 2 rows (n <= 512) and
 // 512 threads x 4 rows (n <= 2048); each launch skips the other size class.
 constexpr int kTriNB = kTriPanel;
-constexpr int kTriChunk = 64;  // trailing-update columns staged in LDS at a time
-constexpr int kTriCols = 8;    // trailing-update columns whose loads are issued together
 constexpr int kTriMaxN = 2048;
 constexpr int kTriFewSectors = 256;  // below this many sectors per launch: one block shape
 
@@ -141,7 +139,6 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
   extern __shared__ double2 tsm[];
   __shared__ double2 red[2][16];
   __shared__ double red4[NW][4 * kTriNB];
-  __shared__ double2 cw[kTriChunk][kTriNB], cv[kTriChunk][kTriNB];  // conj(W(c, l)), conj(V(c, l))
   const int2 ref = list[blockIdx.x];
   const UpdDesc &u = Ud[ref.x];
   const Sector s = u.sectors[ref.y];
@@ -320,54 +317,50 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
       }
       __syncthreads();
     }
-    // (f) trailing update A(r, c) -= sum_l V(r, l) conj(W(c, l)) + W(r, l) conj(V(c, l)),
-    //     n > r >= c >= jn (lower triangle); rows over threads, columns staged
-    //     kTriChunk at a time
+    // (f) trailing update A(r, c) -= sum_l V(r, l) conj(W(c, l)) + W(r, l) conj(V(c, l))
+    //     on the lower triangle n > r >= c >= jn: a product with K = 2 pb on the
+    //     FP64 MFMA, 16 x 16 tiles round the waves.  Each tile is formed
+    //     transposed, U^T = [conj W  conj V](c, :) [V  W](r, :)^T, so that a
+    //     lane's output elements run down a column of A (contiguous rows).
     const int jn = j0 + pb;
-    for (int rbase = jn; rbase < n; rbase += NT) {
-      const int r = rbase + t;
-      const bool rv = r < n;
-      const int cend = min(n, rbase + NT);  // columns past the block's last row are upper
-      double2 vr[kTriNB], wr[kTriNB];
+    {
+      const int nts = (n - jn + 15) / 16;  // tiles per side
+      const int ntiles = nts * (nts + 1) / 2;
+      const int i16 = lane & 15, kq = lane >> 4;
+      for (int tix = wave; tix < ntiles; tix += NW) {
+        int ti = (int)((sqrt(8.0 * tix + 1.0) - 1.0) * 0.5);  // tile row: ti (ti + 1) / 2 <= tix
+        while ((ti + 1) * (ti + 2) / 2 <= tix) ++ti;
+        while (ti * (ti + 1) / 2 > tix) --ti;
+        const int tj = tix - ti * (ti + 1) / 2;
+        const int R0 = jn + 16 * ti, C0 = jn + 16 * tj;
+        const int ca = C0 + i16, rb = R0 + i16;  // operand rows of the a (columns of A) and b sides
+        dbl4 ur = {0.0, 0.0, 0.0, 0.0}, ui = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int l = 0; l < kTriNB; ++l) {
-        vr[l] = (rv && l < pb) ? Vp[(int64_t)l * n + r] : make_double2(0.0, 0.0);
-        wr[l] = (rv && l < pb) ? Wp[(int64_t)l * n + r] : make_double2(0.0, 0.0);
-      }
-      for (int c0 = jn; c0 < cend; c0 += kTriChunk) {
-        const int nc = min(kTriChunk, n - c0);
-        __syncthreads();
-        for (int q = t; q < kTriChunk * kTriNB; q += NT) {
-          const int cc = q / kTriNB, l = q % kTriNB;
-          double2 x = make_double2(0.0, 0.0), y = make_double2(0.0, 0.0);
-          if (cc < nc && l < pb) {
-            x = Wp[(int64_t)l * n + c0 + cc];
-            y = Vp[(int64_t)l * n + c0 + cc];
-          }
-          cw[cc][l] = make_double2(x.x, -x.y);
-          cv[cc][l] = make_double2(y.x, -y.y);
-        }
-        __syncthreads();
-        if (rv) {
-          const int ncr = min(nc, r - c0 + 1);
-          // kTriCols columns at a time, their loads issued before the updates
-          for (int cb = 0; cb < ncr; cb += kTriCols) {
-            double2 a[kTriCols];
-#pragma unroll
-            for (int q = 0; q < kTriCols; ++q)
-              a[q] = cb + q < ncr ? A[(int64_t)(c0 + cb + q) * n + r] : make_double2(0.0, 0.0);
-#pragma unroll
-            for (int q = 0; q < kTriCols; ++q) {
-              const int cc = cb + q;
-              if (cc >= ncr) break;
-#pragma unroll
-              for (int l = 0; l < kTriNB; ++l) {
-                const double2 x = cw[cc][l], y = cv[cc][l];
-                a[q].x -= vr[l].x * x.x - vr[l].y * x.y + wr[l].x * y.x - wr[l].y * y.y;
-                a[q].y -= vr[l].x * x.y + vr[l].y * x.x + wr[l].x * y.y + wr[l].y * y.x;
-              }
-              A[(int64_t)(c0 + cc) * n + r] = a[q];
+        for (int k0 = 0; k0 < 2 * kTriNB; k0 += 4) {
+          const int k = k0 + kq, l = k % kTriNB;
+          const bool vw = k < kTriNB;  // k < 8: V(r) conj(W(c)); k >= 8: W(r) conj(V(c))
+          cplx a{0.0, 0.0}, b{0.0, 0.0};
+          if (l < pb) {
+            if (ca < n) {
+              const double2 x = vw ? Wp[(int64_t)l * n + ca] : Vp[(int64_t)l * n + ca];
+              a = cplx{x.x, -x.y};
             }
+            if (rb < n) {
+              const double2 y = vw ? Vp[(int64_t)l * n + rb] : Wp[(int64_t)l * n + rb];
+              b = cplx{y.x, y.y};
+            }
+          }
+          cmfma_f64(ur, ui, a, b);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {  // element (r, c) = (R0 + i16, C0 + kq + 4 q)
+          const int r = R0 + i16, c = C0 + mfma64_row(kq, q);
+          if (r < n && c <= r) {
+            const int64_t o = (int64_t)c * n + r;
+            double2 av = A[o];
+            av.x -= ur[q];
+            av.y -= ui[q];
+            A[o] = av;
           }
         }
       }
