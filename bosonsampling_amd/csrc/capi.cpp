@@ -68,14 +68,10 @@ int run_group(const void *blob, int precision, void *d_work, size_t work_bytes, 
   BSMPO_HIP(hipMemcpyAsync(d_work, reinterpret_cast<const char *>(blob) + kGroupHostHead,
                            (size_t)g->staging_bytes, hipMemcpyHostToDevice, st));
   const char *w = reinterpret_cast<const char *>(d_work);
-  GroupHeader gg = *g;
-  // low precision: C and T in complex64; the contract-only entry point hands
-  // them to its caller in complex128, rounded to the same FP32 values
-  gg.compact = precision == BSMPO_C128 ? 0 : (contract_only ? 2 : 1);
-  int rc = launch_contract(gg, w, precision, st);
+  int rc = launch_contract(*g, w, precision, st);
   if (rc || contract_only) return rc;
-  if ((rc = launch_svd(gg, w, st))) return rc;
-  return launch_truncate(gg, w, st);
+  if ((rc = launch_svd(*g, w, st))) return rc;
+  return launch_truncate(*g, w, st);
 }
 
 // single-update entry points run as a group of one (host image reused per thread)

@@ -54,26 +54,6 @@ __device__ __forceinline__ cplx cload(const double *p, int64_t i) {
   const double2 v = reinterpret_cast<const double2 *>(p)[i];
   return cplx{v.x, v.y};
 }
-// C / T element i in the group's storage format: compact 1 = complex64;
-// 0 = complex128; 2 = complex128 holding values rounded to FP32 (the
-// contract-only entry point of a low-precision run, so that its values equal
-// those that the grouped run stores in complex64)
-__device__ __forceinline__ cplx cload_fmt(const double *p, int64_t i, int compact) {
-  if (compact == 1) {
-    const float2 v = reinterpret_cast<const float2 *>(p)[i];
-    return cplx{(double)v.x, (double)v.y};
-  }
-  const double2 v = reinterpret_cast<const double2 *>(p)[i];
-  return cplx{v.x, v.y};
-}
-__device__ __forceinline__ void cstore_fmt(double *p, int64_t i, double x, double y, int compact) {
-  if (compact == 1)
-    reinterpret_cast<float2 *>(p)[i] = make_float2((float)x, (float)y);
-  else if (compact == 2)
-    reinterpret_cast<double2 *>(p)[i] = make_double2((double)(float)x, (double)(float)y);
-  else
-    reinterpret_cast<double2 *>(p)[i] = make_double2(x, y);
-}
 __device__ __forceinline__ void cstore(double *p, int64_t i, cplx v) {
   reinterpret_cast<double2 *>(p)[i] = make_double2(v.x, v.y);
 }

@@ -27,8 +27,7 @@ __device__ __forceinline__ void cmfma_f32(flt4 &accr, flt4 &acci, float ar, floa
 }
 
 __global__ __launch_bounds__(256) void k_contract_f32(const UpdDesc *__restrict__ Ud,
-                                                      const int2 *__restrict__ list, int nlist,
-                                                      int compact) {
+                                                      const int2 *__restrict__ list, int nlist) {
   const int lane = threadIdx.x & 63;
   const int i16 = lane & 15, kq = lane >> 4;
   const int nwaves = gridDim.x * 4;
@@ -87,8 +86,8 @@ __global__ __launch_bounds__(256) void k_contract_f32(const UpdDesc *__restrict_
         const int row = mfma32_row(kq, r);
         if (row < item.nr) {
           const int64_t idx = s.c_off + (int64_t)(item.r0 + row) * s.R + item.c0 + i16;
-          cstore_fmt(u.C, idx, (double)accr[r], (double)acci[r], compact);
-          cstore_fmt(u.T, idx, (double)accr[r] * lr, (double)acci[r] * lr, compact);
+          cstore(u.C, idx, cplx{(double)accr[r], (double)acci[r]});
+          cstore(u.T, idx, cplx{(double)accr[r] * lr, (double)acci[r] * lr});
         }
       }
     }
@@ -126,7 +125,7 @@ __device__ __forceinline__ flt4 mfma3(bf16x4 xh, bf16x4 xl, bf16x4 yh, bf16x4 yl
 // the K = 16 steps are not padded at charge-block boundaries.
 __global__ __launch_bounds__(256) void k_contract_bf16x3(const UpdDesc *__restrict__ Ud,
                                                          const int2 *__restrict__ list,
-                                                         int nlist, int compact) {
+                                                         int nlist) {
   extern __shared__ int cmp[];  // per wave: rows[chi] then block index[chi]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i16 = lane & 15, kq = lane >> 4;
@@ -211,8 +210,8 @@ __global__ __launch_bounds__(256) void k_contract_bf16x3(const UpdDesc *__restri
         const int row = mfma32_row(kq, r);
         if (row < item.nr) {
           const int64_t idx = s.c_off + (int64_t)(item.r0 + row) * s.R + item.c0 + i16;
-          cstore_fmt(u.C, idx, (double)accr[r], (double)acci[r], compact);
-          cstore_fmt(u.T, idx, (double)accr[r] * lr, (double)acci[r] * lr, compact);
+          cstore(u.C, idx, cplx{(double)accr[r], (double)acci[r]});
+          cstore(u.T, idx, cplx{(double)accr[r] * lr, (double)acci[r] * lr});
         }
       }
     }
@@ -226,7 +225,7 @@ int launch_contract_lowp(const GroupHeader &g, const char *w, int precision,
   const int2 *list = reinterpret_cast<const int2 *>(w + g.o_items);
   if (precision == BSMPO_C64) {
     hipLaunchKernelGGL(k_contract_f32, dim3(grid_for(g.n_items)), dim3(256), 0, stream, desc, list,
-                       g.n_items, g.compact);
+                       g.n_items);
   } else if (precision == BSMPO_BF16X3) {
     const size_t lds = (size_t)4 * 2 * g.chi * sizeof(int);
     static bool attr = false;
@@ -237,7 +236,7 @@ int launch_contract_lowp(const GroupHeader &g, const char *w, int precision,
     }
     if (g.chi > 4096) return fail(-1, "bf16x3 contraction supports chi <= 4096");
     hipLaunchKernelGGL(k_contract_bf16x3, dim3(grid_for(g.n_items)), dim3(256), lds, stream, desc,
-                       list, g.n_items, g.compact);
+                       list, g.n_items);
   } else {
     return fail(-1, "unknown precision");
   }

@@ -41,8 +41,7 @@ constexpr int kPairRows = 4;  // centre rows per batch of loads in phase 1
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_contract_pair(const UpdDesc *__restrict__ Ud,
-                                                       const int2 *__restrict__ list, int nlist,
-                                                       int compact) {
+                                                       const int2 *__restrict__ list, int nlist) {
   __shared__ cx<T> P[kPairLds];
   __shared__ cx<T> al[kPairMaxB * kPairMaxB];  // alpha[t0][c0]
   __shared__ cx<T> bc_[kPairMaxB * kPairMaxB]; // conj(beta[t1][c1])
@@ -137,8 +136,9 @@ __global__ __launch_bounds__(256) void k_contract_pair(const UpdDesc *__restrict
         const int row = u.rowoff[(int64_t)s * u.n_lblk + pi.a] + lr;
         const int col = u.coloff[(int64_t)s * u.n_rblk + pi.b] + rr;
         const int64_t o = sc.c_off + (int64_t)row * sc.R + col;
-        cstore_fmt(u.C, o, (double)acc.x, (double)acc.y, compact);
-        cstore_fmt(u.T, o, (double)acc.x * wt, (double)acc.y * wt, compact);
+        const cplx c{(double)acc.x, (double)acc.y};
+        cstore(u.C, o, c);
+        cstore(u.T, o, cplx{c.x * wt, c.y * wt});
       }
     }
     __syncthreads();
@@ -152,10 +152,10 @@ int launch_contract_pair(const GroupHeader &g, const char *w, int precision, hip
   const int2 *list = reinterpret_cast<const int2 *>(w + g.o_pitems);
   if (precision == BSMPO_C128)
     hipLaunchKernelGGL(k_contract_pair<double>, dim3(grid), dim3(256), 0, stream, desc, list,
-                       g.n_pitems, g.compact);
+                       g.n_pitems);
   else
     hipLaunchKernelGGL(k_contract_pair<float>, dim3(grid), dim3(256), 0, stream, desc, list,
-                       g.n_pitems, g.compact);
+                       g.n_pitems);
   BSMPO_HIP(hipGetLastError());
   return 0;
 }

@@ -153,11 +153,34 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
   gptr<double2> tau = gp(reinterpret_cast<double2 *>(u.tau) + s.s_off);
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   int par = 0;
+  // The reduction works on G scaled by a power of two that brings its largest
+  // diagonal entry (its largest entry: G is PSD) into [1, 2); a sector of tiny
+  // weights would otherwise underflow in the squared column norms of the
+  // reflectors.  The scaling is exact.  It is applied as the first panel reads
+  // the matrix (every entry is rewritten, scaled, by then), and e[n - 1], not
+  // an off-diagonal, keeps the factor that maps the eigenvalues back (0 for a
+  // zero matrix, whose eigenvalues are all dropped).
+  double scl;
+  {
+    double mx = 0.0;
+    for (int i = t; i < n; i += NT) mx = fmax(mx, fabs(A[(int64_t)i * n + i].x));
+    for (int o = 32; o >= 1; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+    if (lane == 0) red[0][wave].x = mx;
+    __syncthreads();
+    mx = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) mx = fmax(mx, red[0][w].x);
+    const int ex = (mx > 0.0 && isfinite(mx)) ? ilogb(mx) : 0;
+    scl = ldexp(1.0, -ex);
+    if (t == 0) e[n - 1] = mx > 0.0 ? ldexp(1.0, ex) : 0.0;
+    __syncthreads();
+  }
   if (n == 1) {
-    if (t == 0) { d[0] = A[0].x; tau[0] = make_double2(0.0, 0.0); }
+    if (t == 0) { d[0] = A[0].x * scl; tau[0] = make_double2(0.0, 0.0); }
     return;
   }
   for (int j0 = 0; j0 < n - 1; j0 += kTriNB) {
+    if (j0 > 0) scl = 1.0;  // entries read from here on were written scaled
     const int pb = min(kTriNB, n - 1 - j0);
     for (int i = 0; i < pb; ++i) {
       const int j = j0 + i;
@@ -165,6 +188,8 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
       double xn2 = 0.0;
       for (int r = j + t; r < n; r += NT) {
         double2 a = A[(int64_t)j * n + r];
+        a.x *= scl;
+        a.y *= scl;
         for (int l = 0; l < i; ++l) {
           a = csub_mul(a, Vp[(int64_t)l * n + r], Wp[(int64_t)l * n + j]);
           a = csub_mul(a, Wp[(int64_t)l * n + r], Vp[(int64_t)l * n + j]);
@@ -221,8 +246,8 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
               rp = cfma2(rp, a[q], vs[min(s0 + q, n - 1)]);
               if (s0 + q < r) cp[q] = cfma2(cp[q], make_double2(a[q].x, -a[q].y), vr);
             }
-            atomicAdd(&ys[r].x, rp.x);
-            atomicAdd(&ys[r].y, rp.y);
+            atomicAdd(&ys[r].x, rp.x * scl);
+            atomicAdd(&ys[r].y, rp.y * scl);
           }
 #pragma unroll
           for (int q = 0; q < 8; ++q) a[q] = b[q];
@@ -231,8 +256,8 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
         const double2 cs = wave_colsum8(cp, lane);
         const int cq = s0 + (lane >> 3);
         if ((lane & 7) == 0 && cq < n) {
-          atomicAdd(&ys[cq].x, cs.x);
-          atomicAdd(&ys[cq].y, cs.y);
+          atomicAdd(&ys[cq].x, cs.x * scl);
+          atomicAdd(&ys[cq].y, cs.y * scl);
         }
       }
       __syncthreads();
@@ -358,8 +383,8 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
           if (r < n && c <= r) {
             const int64_t o = (int64_t)c * n + r;
             double2 av = A[o];
-            av.x -= ur[q];
-            av.y -= ui[q];
+            av.x = av.x * scl - ur[q];
+            av.y = av.y * scl - ui[q];
             A[o] = av;
           }
         }
@@ -501,10 +526,10 @@ __global__ __launch_bounds__(256) void k_bisect(const UpdDesc *__restrict__ Ud,
   }
   __syncthreads();
   if (j >= n) return;
-  const double lmax = sh_lmax;
-  const bool keep = lmax > 0.0 && l > kDropRel2Tri * lmax;
-  lam[s.s_off + j] = l;
-  S[s.s_off + j] = keep ? sqrt(l) : 0.0;
+  const double lmax = sh_lmax, unscale = e[n - 1];  // 0: a zero matrix
+  const bool keep = unscale > 0.0 && lmax > 0.0 && l > kDropRel2Tri * lmax;
+  lam[s.s_off + j] = l;  // of the scaled matrix, as the inverse iteration uses it
+  S[s.s_off + j] = keep ? sqrt(l * unscale) : 0.0;
   flags[s.s_off + j] = keep ? 1 : 0;
 }
 
@@ -644,7 +669,12 @@ __device__ __forceinline__ void vec_lanes(PT base, int64_t sa, int si, int P, in
         if (j > cstart[c]) {
           // Gram-Schmidt against the cluster's earlier (unit) vectors, kVecGs
           // at a time: all their projections in one pass over the rows (the
-          // loads of a row are independent), then one combined update
+          // loads of a row are independent), then one combined update.  Block
+          // classical Gram-Schmidt loses orthogonality like kappa^2 where the
+          // modified form loses kappa (and the iterate of a tight cluster is
+          // nearly inside the earlier vectors' span), so it runs twice
+          // ("twice is enough")
+          for (int pass = 0; pass < 2; ++pass)
           for (int qb = cstart[c]; qb < j; qb += kVecGs) {
             const int nq = min(kVecGs, j - qb);
             double dot[kVecGs];
@@ -715,13 +745,18 @@ __global__ __launch_bounds__(kInLds ? kVecLdsThreads : kTriVecThreads) void k_tr
     tn = fmax(tn, sh_tn[0][w]);
     tn1 = fmax(tn1, sh_tn[1][w]);
   }
-  // clusters of close eigenvalues (gap <= kClusterRel |T|) are orthogonalised
-  // explicitly; outside them inverse iteration loses at most
-  // eps |T| / gap <= 2e-10 orthogonality (LAPACK's stein uses 1e-3 |T|, which
-  // would chain most kept eigenvalues of a sector into one serial cluster)
-  const double ortol = kClusterRel * tn;
+  // clusters of close eigenvalues are orthogonalised explicitly.  Outside
+  // them inverse iteration leaves an overlap of about eps |T| / gap between
+  // vectors, which enters the truncated state weighted by the singular value
+  // sigma = sqrt(lambda).  The cluster test gap <= kClusterRel sqrt(|T| lambda)
+  // (lambda the larger of the pair) keeps that weighted overlap below
+  // 2e-10 sigma_max for every pair: kClusterRel |T| at the top of the
+  // spectrum, tighter below it, so that the many small kept eigenvalues, a
+  // small absolute gap apart, do not chain into one long serial cluster
+  // (LAPACK's stein uses 1e-3 |T| throughout)
   for (int j = threadIdx.x; j < k; j += blockDim.x)
-    cstart[j] = (j == 0 || fabs(l[j - 1] - l[j]) > ortol) ? 1 : 0;  // cluster head flags
+    cstart[j] = (j == 0 || fabs(l[j - 1] - l[j]) > kClusterRel * sqrt(tn * fmax(l[j - 1], 0.0)))
+                    ? 1 : 0;  // cluster head flags
   __syncthreads();
   if (threadIdx.x == 0) {
     int c = 0;

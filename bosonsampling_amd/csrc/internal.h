@@ -135,11 +135,6 @@ struct GroupHeader {
   int32_t n_pitems, pair_ok;  // pair_ok: every plan of the group supports the pair kernel
   int64_t o_pitems;
   double flops_pair_weight;
-  // set per run, not by group_build: 1 = C and T are stored as complex64 (the
-  // complex64 / BF16x3 contractions, whose values carry FP32 precision; the
-  // Gram route converts them to FP64 on load); 2 = complex128 holding the
-  // same FP32-rounded values (contract-only runs); 0 = complex128
-  int32_t compact, pad2;
 };
 constexpr int64_t kGroupHostHead = 1024;  // host blob: header, then the staging image
 static_assert(sizeof(GroupHeader) <= kGroupHostHead, "group header too large");

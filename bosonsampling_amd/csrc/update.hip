@@ -137,8 +137,7 @@ __global__ __launch_bounds__(256) void k_contract_c128(const UpdDesc *__restrict
 // ================================================================ Gram ==
 // G (n x n, column-major) = T T^H if L <= R else T^H T, T row-major L x R.
 __global__ __launch_bounds__(256) void k_gram(const UpdDesc *__restrict__ Ud,
-                                              const int2 *__restrict__ list, int nlist,
-                                              int compact) {
+                                              const int2 *__restrict__ list, int nlist) {
   const int lane = threadIdx.x & 63;
   const int i16 = lane & 15, kq = lane >> 4;
   const int nwaves = gridDim.x * 4;
@@ -147,7 +146,7 @@ __global__ __launch_bounds__(256) void k_gram(const UpdDesc *__restrict__ Ud,
     const UpdDesc &u = Ud[ref.x];
     const Item item = u.gram_items[ref.y];
     const Sector s = u.sectors[item.sector];
-    const double *Ts = u.T;  // element (i, j) of the sector at s.c_off + i R + j
+    const double2 *Ts = reinterpret_cast<const double2 *>(u.T) + s.c_off;
     const int p = item.r0 + i16, q = item.c0 + i16;
     const bool pv = i16 < item.nr, qv = i16 < item.nc;
     dbl4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
@@ -156,8 +155,8 @@ __global__ __launch_bounds__(256) void k_gram(const UpdDesc *__restrict__ Ud,
         const int k = k0 + kq;
         cplx a{0.0, 0.0}, b{0.0, 0.0};
         if (k < s.R) {
-          if (pv) a = cload_fmt(Ts, s.c_off + (int64_t)p * s.R + k, compact);
-          if (qv) b = cconj(cload_fmt(Ts, s.c_off + (int64_t)q * s.R + k, compact));
+          if (pv) { const double2 v = Ts[(int64_t)p * s.R + k]; a = cplx{v.x, v.y}; }
+          if (qv) { const double2 v = Ts[(int64_t)q * s.R + k]; b = cplx{v.x, -v.y}; }
         }
         cmfma_f64(accr, acci, a, b);
       }
@@ -166,8 +165,8 @@ __global__ __launch_bounds__(256) void k_gram(const UpdDesc *__restrict__ Ud,
         const int k = k0 + kq;
         cplx a{0.0, 0.0}, b{0.0, 0.0};
         if (k < s.L) {
-          if (pv) a = cconj(cload_fmt(Ts, s.c_off + (int64_t)k * s.R + p, compact));
-          if (qv) b = cload_fmt(Ts, s.c_off + (int64_t)k * s.R + q, compact);
+          if (pv) { const double2 v = Ts[(int64_t)k * s.R + p]; a = cplx{v.x, -v.y}; }
+          if (qv) { const double2 v = Ts[(int64_t)k * s.R + q]; b = cplx{v.x, v.y}; }
         }
         cmfma_f64(accr, acci, a, b);
       }
@@ -327,8 +326,7 @@ __global__ __launch_bounds__(256) void k_jacobi(const UpdDesc *__restrict__ Ud,
 
 // V = T Y diag(1/sigma) (L > R sectors); Y lives in the W area, n x n col-major.
 __global__ __launch_bounds__(256) void k_vectors(const UpdDesc *__restrict__ Ud,
-                                                 const int2 *__restrict__ list, int nlist,
-                                                 int compact) {
+                                                 const int2 *__restrict__ list, int nlist) {
   const int lane = threadIdx.x & 63;
   const int i16 = lane & 15, kq = lane >> 4;
   const int nwaves = gridDim.x * 4;
@@ -338,7 +336,7 @@ __global__ __launch_bounds__(256) void k_vectors(const UpdDesc *__restrict__ Ud,
     const Item item = u.v_items[ref.y];
     if (item.c0 >= u.kcount[item.sector]) continue;  // only kept vectors are needed
     const Sector s = u.sectors[item.sector];
-
+    const double2 *Tc = reinterpret_cast<const double2 *>(u.T);
     const double2 *Yc = reinterpret_cast<const double2 *>(u.W);
     const int i = item.r0 + i16, p = item.c0 + i16;
     const bool iv = i16 < item.nr, pv = i16 < item.nc;
@@ -347,7 +345,7 @@ __global__ __launch_bounds__(256) void k_vectors(const UpdDesc *__restrict__ Ud,
       const int k = k0 + kq;
       cplx a{0.0, 0.0}, b{0.0, 0.0};
       if (k < s.R) {
-        if (iv) a = cload_fmt(u.T, s.c_off + (int64_t)i * s.R + k, compact);
+        if (iv) { const double2 v = Tc[s.c_off + (int64_t)i * s.R + k]; a = cplx{v.x, v.y}; }
         if (pv) { const double2 v = Yc[s.w_off + (int64_t)p * s.n + k]; b = cplx{v.x, v.y}; }
       }
       cmfma_f64(accr, acci, a, b);
@@ -368,8 +366,7 @@ __global__ __launch_bounds__(256) void k_vectors(const UpdDesc *__restrict__ Ud,
 
 // W (n x R row-major) = V^H C, V L x n column-major, C L x R row-major.
 __global__ __launch_bounds__(256) void k_wmat(const UpdDesc *__restrict__ Ud,
-                                              const int2 *__restrict__ list, int nlist,
-                                              int compact) {
+                                              const int2 *__restrict__ list, int nlist) {
   const int lane = threadIdx.x & 63;
   const int i16 = lane & 15, kq = lane >> 4;
   const int nwaves = gridDim.x * 4;
@@ -380,7 +377,7 @@ __global__ __launch_bounds__(256) void k_wmat(const UpdDesc *__restrict__ Ud,
     if (item.r0 >= u.kcount[item.sector]) continue;  // only kept rows of W are needed
     const Sector s = u.sectors[item.sector];
     const double2 *Vc = reinterpret_cast<const double2 *>(u.V);
-
+    const double2 *Cc = reinterpret_cast<const double2 *>(u.C);
     const int p = item.r0 + i16, j = item.c0 + i16;
     const bool pv = i16 < item.nr, jv = i16 < item.nc;
     dbl4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
@@ -389,7 +386,7 @@ __global__ __launch_bounds__(256) void k_wmat(const UpdDesc *__restrict__ Ud,
       cplx a{0.0, 0.0}, b{0.0, 0.0};
       if (k < s.L) {
         if (pv) { const double2 v = Vc[s.v_off + (int64_t)p * s.L + k]; a = cplx{v.x, -v.y}; }
-        if (jv) b = cload_fmt(u.C, s.c_off + (int64_t)k * s.R + j, compact);
+        if (jv) { const double2 v = Cc[s.c_off + (int64_t)k * s.R + j]; b = cplx{v.x, v.y}; }
       }
       cmfma_f64(accr, acci, a, b);
     }
@@ -587,9 +584,9 @@ int launch_svd(const GroupHeader &g, const char *w, hipStream_t stream) {
   if (g.n_sec == 0) return 0;
   int tk = timing_begin(ST_GRAM, stream);
   hipLaunchKernelGGL(k_gram, dim3(grid_for(g.n_gram)), dim3(256), 0, stream, gdesc(g, w),
-                     glist(w, g.o_gram), g.n_gram, g.compact);
+                     glist(w, g.o_gram), g.n_gram);
   BSMPO_HIP(hipGetLastError());
-  timing_end(tk, stream, g.flops_gram, (g.compact == 1 ? 8.0 : 16.0) * g.c_elems + 16.0 * g.g_elems);
+  timing_end(tk, stream, g.flops_gram, 16.0 * (g.c_elems + g.g_elems));
   static bool attr_set = false;
   if (!attr_set) {
     BSMPO_HIP(hipFuncSetAttribute((const void *)k_jacobi,
@@ -620,17 +617,16 @@ int launch_truncate(const GroupHeader &g, const char *w, hipStream_t stream) {
   if (g.n_v) {
     tk = timing_begin(ST_VECTORS, stream);
     hipLaunchKernelGGL(k_vectors, dim3(grid_for(g.n_v)), dim3(256), 0, stream, gdesc(g, w),
-                       glist(w, g.o_v), g.n_v, g.compact);
+                       glist(w, g.o_v), g.n_v);
     BSMPO_HIP(hipGetLastError());
     timing_end(tk, stream, g.flops_vectors, 0.0);
   }
   if (g.n_w) {
     tk = timing_begin(ST_WMAT, stream);
     hipLaunchKernelGGL(k_wmat, dim3(grid_for(g.n_w)), dim3(256), 0, stream, gdesc(g, w),
-                       glist(w, g.o_w), g.n_w, g.compact);
+                       glist(w, g.o_w), g.n_w);
     BSMPO_HIP(hipGetLastError());
-    timing_end(tk, stream, g.flops_wmat,
-               16.0 * (g.v_elems + g.w_elems) + (g.compact == 1 ? 8.0 : 16.0) * g.c_elems);
+    timing_end(tk, stream, g.flops_wmat, 16.0 * (g.v_elems + g.c_elems + g.w_elems));
   }
   tk = timing_begin(ST_SCATTER, stream);
   hipLaunchKernelGGL(k_zero, dim3(256, g.n_upd), dim3(256), 0, stream, gdesc(g, w));

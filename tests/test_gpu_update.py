@@ -157,3 +157,28 @@ def test_gpu_initialize_matches_oracle(gpu, eig_route):
     _, Lam, ch = dev_st.to_host()
     for b in range(M - 1):
         assert np.allclose(np.sort(Lam[b])[::-1], np.sort(st.Lambda[b])[::-1], atol=1e-12)
+
+
+def test_tridiagonal_route_on_a_larger_mesh(gpu):
+    """Every sector on the tridiagonal route (threshold 0) against the default
+    routing on a mesh large enough to hold zero-weight and tiny-weight
+    sectors (M=24, six photons, d=7, chi=64, all 24 layers): the eigensolver
+    routes differ only by rounding, so trace, marginals and the output
+    probabilities agree to 1e-9."""
+    from bosonsampling_amd import engine, mesh
+    from bosonsampling_amd._capi import check, lib
+    M, d, chi, inputs, eta = 24, 7, 64, [0, 4, 8, 12, 16, 20], 0.6
+    mm, _ = mesh.haar_clements_mesh(M, 3)
+    out = {}
+    for thr in (96, 0):
+        check(lib().bsmpo_set_eig_threshold(thr), "set_eig_threshold")
+        st = engine.initialize(M, d, chi, inputs, eta)
+        prob, _ = engine.simulate(st, mm, inputs)
+        tr, marg = st.observables()
+        out[thr] = (prob, tr, marg)
+        print(f"threshold {thr}: trace {tr:.12f}", flush=True)
+    check(lib().bsmpo_set_eig_threshold(96), "set_eig_threshold")
+    (p96, t96, m96), (p0, t0, m0) = out[96], out[0]
+    assert np.isfinite(t0) and abs(t0 - t96) <= 1e-9
+    assert np.abs(m0 - m96).max() <= 1e-9
+    assert np.allclose(p0, p96, rtol=1e-9, atol=1e-14)
