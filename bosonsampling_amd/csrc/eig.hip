@@ -70,7 +70,7 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
 // Only the lower triangle of the (Hermitian) trailing matrix is read or
 // written: per column it is read once (the product A22 v, each entry below the
 // diagonal serving A v and A^H v alike); it is written once per panel.  V and W of the panel live in the sector's
-// tridiagonal work area (n x kTriNB each, column-major).  Six barriers per
+// tridiagonal work area (n x kTriNB each, column-major).  Five barriers per
 // column: block reductions alternate between two LDS buffers, so each costs
 // one barrier.  Two block shapes: 256 threads x 2 rows (n <= 512) and
 // 512 threads x 4 rows (n <= 2048); each launch skips the other size class.
@@ -260,14 +260,8 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
           atomicAdd(&ys[cq].y, cs.y * scl);
         }
       }
-      __syncthreads();
-      double2 acc[RPT];
-#pragma unroll
-      for (int k = 0; k < RPT; ++k) {
-        const int r = j + 1 + t + k * NT;
-        acc[k] = r < n ? ys[r] : make_double2(0.0, 0.0);
-      }
-      // (d) corrections for the panel's pending updates
+      // (d) corrections for the panel's pending updates: the per-wave parts of
+      //     W_l^H v and V_l^H v, before the barrier that completes A22 v
       if (i > 0) {
         for (int l = 0; l < i; ++l) {
           double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
@@ -295,7 +289,15 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
             red4[wave][4 * l + 3] = p3;
           }
         }
-        __syncthreads();
+      }
+      __syncthreads();
+      double2 acc[RPT];
+#pragma unroll
+      for (int k = 0; k < RPT; ++k) {
+        const int r = j + 1 + t + k * NT;
+        acc[k] = r < n ? ys[r] : make_double2(0.0, 0.0);
+      }
+      if (i > 0) {
         for (int l = 0; l < i; ++l) {
           double2 y = make_double2(0.0, 0.0), z = make_double2(0.0, 0.0);  // W_l^H v, V_l^H v
 #pragma unroll
