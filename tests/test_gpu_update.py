@@ -182,3 +182,39 @@ def test_tridiagonal_route_on_a_larger_mesh(gpu):
     assert np.isfinite(t0) and abs(t0 - t96) <= 1e-9
     assert np.abs(m0 - m96).max() <= 1e-9
     assert np.allclose(p0, p96, rtol=1e-9, atol=1e-14)
+
+
+def test_tridiagonal_route_is_scale_invariant(gpu):
+    """A site update is homogeneous: Glc, Gcr -> c Glc, c Gcr gives T -> c^2 T,
+    new singular values c^2 Lambda and the same bond charges and vectors.
+    With c = 1e-50 the Gram matrices hold entries ~1e-200, whose squares
+    underflow, so the tridiagonal route must work on an exactly rescaled
+    matrix.  Every sector takes that route (threshold 0)."""
+    import torch
+    from bosonsampling_amd import engine
+    from bosonsampling_amd._capi import check, lib
+    M, d, chi, inputs, eta, seed, layers, mode = 10, 6, 48, [0, 2, 4, 6, 8], 0.7, 1, 5, 4
+    st, mm = _oracle_state(M, d, chi, inputs, eta, seed, layers)
+    B = mm[layers, (mode - layers % 2) // 2]
+    c = 1e-50
+    check(lib().bsmpo_set_eig_threshold(0), "set_eig_threshold")
+    try:
+        outs = []
+        for scale in (1.0, c):
+            dev_st = _upload(st)
+            dev_st.Gamma[mode] *= scale
+            dev_st.Gamma[mode + 1] *= scale
+            U3 = engine.bs_tables(d, B, gpu)[0]
+            dev_st.update(mode, U3)
+            outs.append(dev_st.to_host())
+    finally:
+        check(lib().bsmpo_set_eig_threshold(96), "set_eig_threshold")
+    (G1, L1, ch1), (Gc, Lc, chc) = outs
+    assert np.array_equal(ch1[mode + 1], chc[mode + 1])
+    assert np.all(np.isfinite(Lc[mode])) and np.all(np.isfinite(Gc[mode]))
+    assert np.allclose(Lc[mode] / c ** 2, L1[mode], rtol=1e-10, atol=1e-10 * L1[mode].max())
+    # the new left site carries no Lambda factor: equal up to the gauge of each
+    # singular vector, so compare the projector onto each kept column block
+    P1 = G1[mode] @ G1[mode + 1]
+    Pc = Gc[mode] @ Gc[mode + 1]
+    assert np.abs(Pc / c ** 2 - P1).max() <= 1e-10 * np.abs(P1).max()
