@@ -400,16 +400,41 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
   }
 }
 
-// number of eigenvalues of the symmetric tridiagonal (d, e) below x; e2 = e^2
+// 1 / q from the hardware reciprocal and one Newton step (relative error a few
+// ulp): the Sturm recurrence below is a chain of these, and IEEE division is
+// about twice as long a dependency chain.  |q| >= pivmin keeps 1 / q finite.
+__device__ __forceinline__ double recip_nr(double q) {
+  const double r = __builtin_amdgcn_rcp(q);
+  return fma(fma(-q, r, 1.0), r, r);
+}
+
+// number of eigenvalues of the symmetric tridiagonal (d, e) below x; e2 = e^2.
+// The pivot guard (|q| >= pivmin, pivmin >= DBL_MIN max(1, max e^2)) keeps every
+// e^2 / q finite; the table loads go four rows at a time, off the chain.
 __device__ int sturm_below(const double *d, const double *e2, int n, double x, double pivmin) {
   int c = 0;
   double q = d[0] - x;
   if (fabs(q) < pivmin) q = -pivmin;
-  if (q < 0.0) ++c;
-  for (int i = 1; i < n; ++i) {
-    q = d[i] - x - e2[i - 1] / q;
+  c += q < 0.0;
+  int i = 1;
+  for (; i + 3 < n; i += 4) {
+    double dd[4], ff[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      dd[k] = d[i + k] - x;
+      ff[k] = e2[i + k - 1];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      q = dd[k] - ff[k] * recip_nr(q);
+      if (fabs(q) < pivmin) q = -pivmin;
+      c += q < 0.0;
+    }
+  }
+  for (; i < n; ++i) {
+    q = d[i] - x - e2[i - 1] * recip_nr(q);
     if (fabs(q) < pivmin) q = -pivmin;
-    if (q < 0.0) ++c;
+    c += q < 0.0;
   }
   return c;
 }

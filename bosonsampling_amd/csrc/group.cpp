@@ -191,6 +191,15 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
     const int32_t *jo = host_array<int32_t>(plans[u], h->o_jac_order);
     for (int i = h->jac_off[kTriClass]; i < h->jac_off[kTriClass + 1]; ++i) tri[nt++] = make_int2(u, jo[i]);
   }
+  // Tridiagonal-route sectors largest first across the group: blocks are
+  // dispatched in list order, so the longest column chains start at once and
+  // the short ones fill the slots they leave (longest-processing-time order).
+  {
+    auto size_of = [&](int2 r) {
+      return host_array<Sector>(plans[r.x], header(plans[r.x])->o_sectors)[r.y].n;
+    };
+    std::stable_sort(tri, tri + nt, [&](int2 x, int2 y) { return size_of(x) > size_of(y); });
+  }
   // LDS Jacobi list grouped by size class across updates
   int64_t nj = 0;
   for (int c = 0; c < kTriClass; ++c) {

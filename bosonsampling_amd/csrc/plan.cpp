@@ -241,6 +241,9 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
         }
       }
     }
+    if (c == kTriClass)  // largest first: the longest column chains start first
+      std::stable_sort(b.jac_order.begin() + b.h.jac_off[c], b.jac_order.end(),
+                       [&](int32_t x, int32_t y) { return b.sectors[x].n > b.sectors[y].n; });
   }
   b.h.jac_off[kJacClasses] = (int32_t)b.jac_order.size();
 
