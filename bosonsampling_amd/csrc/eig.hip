@@ -73,10 +73,13 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
 // tridiagonal work area (n x kTriNB each, column-major).  Five barriers per
 // column: block reductions alternate between two LDS buffers, so each costs
 // one barrier.  Two block shapes: 256 threads x 2 rows (n <= 512) and
-// 512 threads x 4 rows (n <= 2048); each launch skips the other size class.
+// 512 threads x 4 rows (n <= 2048); each launch takes its own segment of the
+// sector list, which is sorted largest first (n outside [nmin, nmax] is
+// skipped, a guard of the LDS sizing).
 constexpr int kTriNB = kTriPanel;
 constexpr int kTriMaxN = 2048;
 constexpr int kTriFewSectors = 256;  // below this many sectors per launch: one block shape
+constexpr int kTriSplit = 2;  // index into kTriSplitN: the block-shape boundary of a group (384)
 
 __device__ __forceinline__ double2 csub_mul(double2 a, double2 x, double2 y) {  // a - x conj(y)
   a.x -= x.x * y.x + x.y * y.y;
@@ -133,7 +136,7 @@ __device__ __forceinline__ double2 wave_colsum8(const double2 (&x)[8], int lane)
 }
 
 template <int NT, int RPT>
-__global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
+__global__ __launch_bounds__(NT, 2) void k_tridiag(const UpdDesc *__restrict__ Ud,
                                                 const int2 *__restrict__ list, int nmin, int nmax) {
   constexpr int NW = NT / 64;
   extern __shared__ double2 tsm[];
@@ -223,41 +226,68 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
       //     lane rows r >= c feed p(r) += A(r, c) v(c) and, below the
       //     diagonal, p(c) += conj(A(r, c)) v(r).  Both sums go to ys by LDS
       //     atomics (row sums per lane, column sums after a wave reduction).
-      for (int s0 = j + 1 + 8 * wave; s0 < n; s0 += 8 * NW) {
-        double2 cp[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) cp[q] = make_double2(0.0, 0.0);
-        // row blocks of 64, the next block's loads in flight while this one is summed
-        double2 a[8], b[8];
-        int r = s0 + lane;
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-          a[q] = (r < n && s0 + q <= r) ? A[(int64_t)(s0 + q) * n + r] : make_double2(0.0, 0.0);
-        for (int r0 = s0; r0 < n; r0 += 64) {  // wave-uniform
-          const int rn = r + 64;
+      //     A wave's items (strip s, row block r0 = s, s + 64, ...; then the
+      //     next strip) form one sequence whose loads run two items ahead,
+      //     across strip boundaries.
+      {
+        constexpr int stride = 8 * NW;
+        auto advance = [&](int &s, int &r0) {  // next item of this wave
+          r0 += 64;
+          if (r0 >= n) {
+            s += stride;
+            r0 = s;
+          }
+        };
+        auto fetch = [&](double2(&x)[8], int s, int r0) {
+          const int r = r0 + lane;
 #pragma unroll
           for (int q = 0; q < 8; ++q)
-            b[q] = (rn < n && s0 + q <= rn) ? A[(int64_t)(s0 + q) * n + rn] : make_double2(0.0, 0.0);
+            x[q] = (r < n && s + q <= r) ? A[(int64_t)(s + q) * n + r] : make_double2(0.0, 0.0);
+        };
+        double2 cp[8], a[8], b[8], c[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) cp[q] = make_double2(0.0, 0.0);
+        int sa = j + 1 + 8 * wave, ra = sa;  // item held in a
+        int sb = sa, rb = ra;
+        advance(sb, rb);                     // item held in b
+        int sc = sb, rc = rb;
+        advance(sc, rc);                     // item loaded into c
+        fetch(a, sa, ra);
+        fetch(b, sb, rb);
+        while (sa < n) {  // wave-uniform
+          fetch(c, sc, rc);
+          const int r = ra + lane;
           if (r < n) {
             const double2 vr = vs[r];
             double2 rp = make_double2(0.0, 0.0);
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-              rp = cfma2(rp, a[q], vs[min(s0 + q, n - 1)]);
-              if (s0 + q < r) cp[q] = cfma2(cp[q], make_double2(a[q].x, -a[q].y), vr);
+              rp = cfma2(rp, a[q], vs[min(sa + q, n - 1)]);
+              if (sa + q < r) cp[q] = cfma2(cp[q], make_double2(a[q].x, -a[q].y), vr);
             }
             atomicAdd(&ys[r].x, rp.x * scl);
             atomicAdd(&ys[r].y, rp.y * scl);
           }
+          if (sb != sa) {  // strip sa done: its column sums
+            const double2 cs = wave_colsum8(cp, lane);
+            const int cq = sa + (lane >> 3);
+            if ((lane & 7) == 0 && cq < n) {
+              atomicAdd(&ys[cq].x, cs.x * scl);
+              atomicAdd(&ys[cq].y, cs.y * scl);
+            }
 #pragma unroll
-          for (int q = 0; q < 8; ++q) a[q] = b[q];
-          r = rn;
-        }
-        const double2 cs = wave_colsum8(cp, lane);
-        const int cq = s0 + (lane >> 3);
-        if ((lane & 7) == 0 && cq < n) {
-          atomicAdd(&ys[cq].x, cs.x * scl);
-          atomicAdd(&ys[cq].y, cs.y * scl);
+            for (int q = 0; q < 8; ++q) cp[q] = make_double2(0.0, 0.0);
+          }
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            a[q] = b[q];
+            b[q] = c[q];
+          }
+          sa = sb;
+          ra = rb;
+          sb = sc;
+          rb = rc;
+          advance(sc, rc);
         }
       }
       // (d) corrections for the panel's pending updates: the per-wave parts of
@@ -354,40 +384,64 @@ __global__ __launch_bounds__(NT) void k_tridiag(const UpdDesc *__restrict__ Ud,
       const int nts = (n - jn + 15) / 16;  // tiles per side
       const int ntiles = nts * (nts + 1) / 2;
       const int i16 = lane & 15, kq = lane >> 4;
-      for (int tix = wave; tix < ntiles; tix += NW) {
-        int ti = (int)((sqrt(8.0 * tix + 1.0) - 1.0) * 0.5);  // tile row: ti (ti + 1) / 2 <= tix
-        while ((ti + 1) * (ti + 2) / 2 <= tix) ++ti;
-        while (ti * (ti + 1) / 2 > tix) --ti;
-        const int tj = tix - ti * (ti + 1) / 2;
-        const int R0 = jn + 16 * ti, C0 = jn + 16 * tj;
-        const int ca = C0 + i16, rb = R0 + i16;  // operand rows of the a (columns of A) and b sides
-        dbl4 ur = {0.0, 0.0, 0.0, 0.0}, ui = {0.0, 0.0, 0.0, 0.0};
+      // two tiles per pass (tix and tix + NW), every load of both issued
+      // before the first product, so that a pass waits on memory once
+      constexpr int kTT = 2;
+      for (int tix0 = wave; tix0 < ntiles; tix0 += kTT * NW) {
+        int R0[kTT], C0[kTT];
+        bool live[kTT];
+        double2 av[kTT][4];
+        cplx a[kTT][4], b[kTT][4];
 #pragma unroll
-        for (int k0 = 0; k0 < 2 * kTriNB; k0 += 4) {
-          const int k = k0 + kq, l = k % kTriNB;
-          const bool vw = k < kTriNB;  // k < 8: V(r) conj(W(c)); k >= 8: W(r) conj(V(c))
-          cplx a{0.0, 0.0}, b{0.0, 0.0};
-          if (l < pb) {
-            if (ca < n) {
-              const double2 x = vw ? Wp[(int64_t)l * n + ca] : Vp[(int64_t)l * n + ca];
-              a = cplx{x.x, -x.y};
-            }
-            if (rb < n) {
-              const double2 y = vw ? Vp[(int64_t)l * n + rb] : Wp[(int64_t)l * n + rb];
-              b = cplx{y.x, y.y};
+        for (int u = 0; u < kTT; ++u) {
+          const int tix = tix0 + u * NW;
+          live[u] = tix < ntiles;  // wave-uniform
+          const int tq = live[u] ? tix : 0;
+          int ti = (int)((sqrt(8.0 * tq + 1.0) - 1.0) * 0.5);  // tile row: ti (ti + 1) / 2 <= tix
+          while ((ti + 1) * (ti + 2) / 2 <= tq) ++ti;
+          while (ti * (ti + 1) / 2 > tq) --ti;
+          const int tj = tq - ti * (ti + 1) / 2;
+          R0[u] = jn + 16 * ti;
+          C0[u] = jn + 16 * tj;
+          const int ca = C0[u] + i16, rb = R0[u] + i16;  // operand rows: a side (columns of A), b side
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {  // element (r, c) = (R0 + i16, C0 + kq + 4 q)
+            const int r = R0[u] + i16, c = C0[u] + mfma64_row(kq, q);
+            av[u][q] = (live[u] && r < n && c <= r) ? A[(int64_t)c * n + r] : make_double2(0.0, 0.0);
+          }
+#pragma unroll
+          for (int k4 = 0; k4 < 4; ++k4) {
+            const int k = 4 * k4 + kq, l = k % kTriNB;
+            const bool vw = k < kTriNB;  // k < 8: V(r) conj(W(c)); k >= 8: W(r) conj(V(c))
+            a[u][k4] = cplx{0.0, 0.0};
+            b[u][k4] = cplx{0.0, 0.0};
+            if (live[u] && l < pb) {
+              if (ca < n) {
+                const double2 x = vw ? Wp[(int64_t)l * n + ca] : Vp[(int64_t)l * n + ca];
+                a[u][k4] = cplx{x.x, -x.y};
+              }
+              if (rb < n) {
+                const double2 y = vw ? Vp[(int64_t)l * n + rb] : Wp[(int64_t)l * n + rb];
+                b[u][k4] = cplx{y.x, y.y};
+              }
             }
           }
-          cmfma_f64(ur, ui, a, b);
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {  // element (r, c) = (R0 + i16, C0 + kq + 4 q)
-          const int r = R0 + i16, c = C0 + mfma64_row(kq, q);
-          if (r < n && c <= r) {
-            const int64_t o = (int64_t)c * n + r;
-            double2 av = A[o];
-            av.x = av.x * scl - ur[q];
-            av.y = av.y * scl - ui[q];
-            A[o] = av;
+        for (int u = 0; u < kTT; ++u) {
+          if (!live[u]) break;
+          dbl4 ur = {0.0, 0.0, 0.0, 0.0}, ui = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int k4 = 0; k4 < 4; ++k4) cmfma_f64(ur, ui, a[u][k4], b[u][k4]);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int r = R0[u] + i16, c = C0[u] + mfma64_row(kq, q);
+            if (r < n && c <= r) {
+              double2 x = av[u][q];
+              x.x = x.x * scl - ur[q];
+              x.y = x.y * scl - ui[q];
+              A[(int64_t)c * n + r] = x;
+            }
           }
         }
       }
@@ -985,31 +1039,33 @@ int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
     attr = true;
   }
   int tk = timing_begin(ST_TRIDIAG, stream);
-  // Few sectors (a single update): the larger block shape for every sector, so
-  // that each sector's column chain gets more threads.  Many sectors (a group
-  // across instances): 256-thread blocks for n <= 512, more blocks per CU.
-  // When both shapes run, the large-sector launch goes to a side stream so that
-  // its long column chains overlap the many short ones (fork / join by events).
-  const int split = g.n_tri < kTriFewSectors ? 0 : 512;
-  const bool both = split > 0 && g.tri_max_n > split;
-  hipStream_t big = stream;
+  // The list is sorted largest first.  Few sectors (a single update): the
+  // larger block shape for every sector, so that each column chain gets more
+  // threads.  Many sectors (a group across instances): the sectors above
+  // kTriSplitN[kTriSplit] (the head of the list) take the 512-thread shape and
+  // go first, on the launching stream; the rest take 256-thread blocks, more
+  // per CU, on a side stream forked before the first launch (fork / join by
+  // events), so that they fill the slots the long column chains leave.
+  const int nbig = g.n_tri < kTriFewSectors ? g.n_tri : g.tri_over[kTriSplit];
+  const int split = g.n_tri < kTriFewSectors ? kTriMaxN : kTriSplitN[kTriSplit];
+  const bool both = nbig > 0 && nbig < g.n_tri;
   SideStream *ss = nullptr;
   if (both) {
     int rc = side_stream(&ss);
     if (rc) return rc;
     BSMPO_HIP(hipEventRecord(ss->fork, stream));
     BSMPO_HIP(hipStreamWaitEvent(ss->stream, ss->fork, 0));
-    big = ss->stream;
   }
-  if (g.tri_max_n > split) {
-    hipLaunchKernelGGL((k_tridiag<512, 4>), dim3(g.n_tri), dim3(512), (size_t)g.tri_max_n * 32,
-                       big, gdesc(g, w), glist(w, g.o_tri), split + 1, kTriMaxN);
+  const int2 *tl = glist(w, g.o_tri);
+  if (nbig > 0) {
+    hipLaunchKernelGGL((k_tridiag<512, 4>), dim3(nbig), dim3(512), (size_t)g.tri_max_n * 32,
+                       stream, gdesc(g, w), tl, 1, kTriMaxN);
     BSMPO_HIP(hipGetLastError());
   }
-  if (split > 0) {
-    hipLaunchKernelGGL((k_tridiag<256, 2>), dim3(g.n_tri), dim3(256),
-                       (size_t)std::min(g.tri_max_n, 512) * 32, stream, gdesc(g, w),
-                       glist(w, g.o_tri), 1, split);
+  if (nbig < g.n_tri) {
+    hipLaunchKernelGGL((k_tridiag<256, 2>), dim3(g.n_tri - nbig), dim3(256),
+                       (size_t)std::min(split, g.tri_max_n) * 32,
+                       both ? ss->stream : stream, gdesc(g, w), tl + nbig, 1, split);
     BSMPO_HIP(hipGetLastError());
   }
   if (both) {
