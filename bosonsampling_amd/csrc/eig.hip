@@ -1103,30 +1103,37 @@ int launch_tri_vectors(const GroupHeader &g, const char *w, hipStream_t stream) 
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, 0.0, 0.0);
   tk = timing_begin(ST_BACKTRANS, stream);
-  // one launch per sector size class (rows per lane); the classes above 512,
-  // few long-running sectors, go to the side stream to overlap the bulk
+  // one launch per sector size class (rows per lane), each over its segment of
+  // the list (sorted largest first): the few long-running sectors above 512
+  // go first on the launching stream, the bulk on a side stream forked before
   const size_t lds = (size_t)kBtLds * 16;
-  hipStream_t big = stream;
+  const int2 *tl = glist(w, g.o_tri);
+  const int nbig = g.tri_over[kTriSplits - 1];  // n > 512
+  static_assert(kTriSplitN[kTriSplits - 1] == 512, "size class boundary of the back-transformation");
+  const bool both = nbig > 0 && nbig < g.n_tri;
   SideStream *ss = nullptr;
-  if (g.tri_max_n > 512) {
+  if (both) {
     int rc = side_stream(&ss);
     if (rc) return rc;
     BSMPO_HIP(hipEventRecord(ss->fork, stream));
     BSMPO_HIP(hipStreamWaitEvent(ss->stream, ss->fork, 0));
-    big = ss->stream;
-    hipLaunchKernelGGL(k_tri_backtransform<16>, dim3(g.n_tri), dim3(1024), lds, big,
-                       gdesc(g, w), glist(w, g.o_tri), 513);
+  }
+  if (nbig > 0) {
+    hipLaunchKernelGGL(k_tri_backtransform<16>, dim3(nbig), dim3(1024), lds, stream, gdesc(g, w), tl,
+                       513);
     BSMPO_HIP(hipGetLastError());
     if (g.tri_max_n > 1024) {
-      hipLaunchKernelGGL(k_tri_backtransform<32>, dim3(g.n_tri), dim3(256), lds, big,
-                         gdesc(g, w), glist(w, g.o_tri), 1025);
+      hipLaunchKernelGGL(k_tri_backtransform<32>, dim3(nbig), dim3(256), lds, stream, gdesc(g, w),
+                         tl, 1025);
       BSMPO_HIP(hipGetLastError());
     }
   }
-  hipLaunchKernelGGL(k_tri_backtransform<8>, dim3(g.n_tri), dim3(1024), lds, stream, gdesc(g, w),
-                     glist(w, g.o_tri), 1);
-  BSMPO_HIP(hipGetLastError());
-  if (ss) {
+  if (nbig < g.n_tri) {
+    hipLaunchKernelGGL(k_tri_backtransform<8>, dim3(g.n_tri - nbig), dim3(1024), lds,
+                       both ? ss->stream : stream, gdesc(g, w), tl + nbig, 1);
+    BSMPO_HIP(hipGetLastError());
+  }
+  if (both) {
     BSMPO_HIP(hipEventRecord(ss->join, ss->stream));
     BSMPO_HIP(hipStreamWaitEvent(stream, ss->join, 0));
   }
