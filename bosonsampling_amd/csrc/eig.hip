@@ -1002,28 +1002,35 @@ static inline const int2 *glist(const char *w, int64_t off) {
   return reinterpret_cast<const int2 *>(w + off);
 }
 
-// Per-device side stream (default priority, non-blocking) with a fork and a
-// join event, created on first use and kept for the life of the process.
+// Side stream (default priority, non-blocking) with a fork and a join event,
+// one per launching stream (so that groups run on different streams do not
+// share one), created on first use and kept for the life of the process.
 struct SideStream {
-  hipStream_t stream;
+  hipStream_t parent, stream;
   hipEvent_t fork, join;
 };
-static int side_stream(SideStream **out) {
+static int side_stream(hipStream_t parent, SideStream **out) {
+  constexpr int kMaxSides = 64;
   static std::mutex mu;
-  static SideStream per_dev[64];
-  static bool made[64] = {};
+  static SideStream sides[kMaxSides];
+  static int dev_of[kMaxSides];
+  static int n_sides = 0;
   int dev = 0;
   BSMPO_HIP(hipGetDevice(&dev));
-  if (dev < 0 || dev >= 64) return fail(-1, "device index out of range");
   std::lock_guard<std::mutex> lk(mu);
-  if (!made[dev]) {
-    SideStream &x = per_dev[dev];
-    BSMPO_HIP(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
-    BSMPO_HIP(hipEventCreateWithFlags(&x.fork, hipEventDisableTiming));
-    BSMPO_HIP(hipEventCreateWithFlags(&x.join, hipEventDisableTiming));
-    made[dev] = true;
-  }
-  *out = &per_dev[dev];
+  for (int i = 0; i < n_sides; ++i)
+    if (sides[i].parent == parent && dev_of[i] == dev) {
+      *out = &sides[i];
+      return 0;
+    }
+  if (n_sides == kMaxSides) return fail(-1, "too many launching streams for side streams");
+  SideStream &x = sides[n_sides];
+  x.parent = parent;
+  BSMPO_HIP(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
+  BSMPO_HIP(hipEventCreateWithFlags(&x.fork, hipEventDisableTiming));
+  BSMPO_HIP(hipEventCreateWithFlags(&x.join, hipEventDisableTiming));
+  dev_of[n_sides++] = dev;
+  *out = &x;
   return 0;
 }
 
@@ -1051,7 +1058,7 @@ int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
   const bool both = nbig > 0 && nbig < g.n_tri;
   SideStream *ss = nullptr;
   if (both) {
-    int rc = side_stream(&ss);
+    int rc = side_stream(stream, &ss);
     if (rc) return rc;
     BSMPO_HIP(hipEventRecord(ss->fork, stream));
     BSMPO_HIP(hipStreamWaitEvent(ss->stream, ss->fork, 0));
@@ -1113,7 +1120,7 @@ int launch_tri_vectors(const GroupHeader &g, const char *w, hipStream_t stream) 
   const bool both = nbig > 0 && nbig < g.n_tri;
   SideStream *ss = nullptr;
   if (both) {
-    int rc = side_stream(&ss);
+    int rc = side_stream(stream, &ss);
     if (rc) return rc;
     BSMPO_HIP(hipEventRecord(ss->fork, stream));
     BSMPO_HIP(hipStreamWaitEvent(ss->stream, ss->fork, 0));

@@ -214,14 +214,23 @@ class GroupRunner:
     The updates of one call must touch disjoint sites (e.g. all beam
     splitters of a brick-wall layer, across any number of DeviceMPO
     instances).  ``max_bytes`` bounds the device workspace of one group; a
-    call with more work is split into several groups.
+    call with more work is split into several groups.  The groups of one call
+    are independent, so with ``streams=2`` they alternate between the current
+    stream and a second one (each with its own workspace): one group's
+    contraction then fills the CUs that the other's long eigensolver chains
+    leave idle.  The call returns with the current stream joined to both.
     """
 
-    def __init__(self, device="cuda", max_bytes: int = 24 << 30, use_graph: bool = False):
+    def __init__(self, device="cuda", max_bytes: int = 24 << 30, use_graph: bool = False,
+                 streams: int = 2):
         self.device = torch.device(device)
         self.max_bytes = int(max_bytes)
         self.use_graph = bool(use_graph)
-        self.work = None
+        if streams not in (1, 2):
+            raise ValueError("GroupRunner: streams must be 1 or 2")
+        self.streams = 1 if use_graph else streams
+        self.aux = None                # second launching stream (streams=2)
+        self.works = [None, None]      # device workspace per launching stream
         self.hosts = [None, None]      # pinned host images, used alternately
         self.events = [None, None]     # consumption of each image by the device
         self.graphs = [None, None]     # graph that uploads each image (use_graph)
@@ -229,11 +238,19 @@ class GroupRunner:
         self.turn = 0
         self.last_info = {}
 
-    def _work(self, n):
-        if self.work is None or self.work.numel() < n:
-            self.work = None
-            self.work = torch.empty(n, dtype=torch.uint8, device=self.device)
-        return self.work
+    def _work(self, n, slot=0, stream=None):
+        w = self.works[slot]
+        if w is None or w.numel() < n:
+            self.works[slot] = None
+            w = torch.empty(n, dtype=torch.uint8, device=self.device)
+            if stream is not None:
+                w.record_stream(stream)  # freed only after that stream's work on it
+            self.works[slot] = w
+        return w
+
+    @property
+    def work(self):
+        return self.works[0]
 
     def _host(self, n):
         i = self.turn
@@ -253,15 +270,24 @@ class GroupRunner:
         overlaps the device's work on the previous one."""
         if not updates:
             return
-        stream = _stream()
+        main = torch.cuda.current_stream(self.device)
+        if self.streams == 2 and self.aux is None:
+            self.aux = torch.cuda.Stream(device=self.device)
+        forked = [False]
         n_groups = 0
 
-        def launch(grp):
+        def launch(grp, g):
             n = len(grp)
             arr = (C.c_void_p * n)(*[p.address for _, p in grp])
             hb, wb = C.c_size_t(), C.c_size_t()
             check(lib().bsmpo_group_query(n, arr, C.byref(hb), C.byref(wb)), "group_query")
-            work = self._work(wb.value)
+            side = self.streams == 2 and g % 2 == 1
+            ts = self.aux if side else main
+            if side and not forked[0]:
+                self.aux.wait_stream(main)  # the call's inputs, written on the current stream
+                forked[0] = True
+            stream = C.c_void_p(ts.cuda_stream)
+            work = self._work(wb.value, 1 if side else 0, ts if side else None)
             slot, host = self._host(hb.value)
             io = (_capi.UpdateIO * n)()
             for j, (k, _) in enumerate(grp):
@@ -291,7 +317,7 @@ class GroupRunner:
                                             C.c_void_p(work.data_ptr()), wb.value, stream),
                       "group_run")
             ev = torch.cuda.Event()
-            ev.record()
+            ev.record(ts)
             self.events[slot] = ev
 
         cur, cur_bytes = [], 0
@@ -299,14 +325,16 @@ class GroupRunner:
             ch = st.charge_host
             p = Plan(st.d, st.chi, ch[mode], ch[mode + 1], ch[mode + 2])
             if cur and cur_bytes + p.work_bytes > self.max_bytes:
-                launch(cur)  # asynchronous: the next plans are built meanwhile
+                launch(cur, n_groups)  # asynchronous: the next plans are built meanwhile
                 n_groups += 1
                 cur, cur_bytes = [], 0
             cur.append((k, p))
             cur_bytes += p.work_bytes
         if cur:
-            launch(cur)
+            launch(cur, n_groups)
             n_groups += 1
+        if forked[0]:
+            main.wait_stream(self.aux)
         self.last_info = {"groups": n_groups, "updates": len(updates)}
         # host mirror of the new centre charges: one device-to-host copy
         rows = [st.charge[mode + 1] for st, mode, _ in updates]
