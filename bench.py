@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--precision", default="c64", choices=["c128", "c64", "bf16x3"])
     ap.add_argument("--paths", default="plain,grouped")
     ap.add_argument("--group-gib", type=float, default=24.0, help="workspace bound of one group")
+    ap.add_argument("--streams", type=int, default=2, choices=[1, 2],
+                    help="launching streams the groups of a layer alternate between")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="budget of the CPU-oracle baseline sample (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -201,7 +203,8 @@ def main():
     roof = None
     if "grouped" in paths:
         states = [init.clone() for _ in range(B)]
-        runner = engine.GroupRunner(init.device, max_bytes=int(args.group_gib * (1 << 30)))
+        runner = engine.GroupRunner(init.device, max_bytes=int(args.group_gib * (1 << 30)),
+                                    streams=args.streams)
         t1 = time.time()
         tuned = None
         for layer in range(W):
@@ -239,6 +242,10 @@ def main():
         log(f"[bench] grouped {args.precision}: {n} updates in {el:.3f}s = {n / el:.2f} updates/s")
         stage_table(st_grp)
         roof = roofline(st_grp, args.precision)
+        roof["launch_streams"] = args.streams
+        if args.streams > 1:
+            roof["note"] = ("groups alternate between two launching streams: a launch's duration "
+                            "includes sharing the GPU with the other stream's stages")
 
     cpu = cpu_baseline(snap, d, chi, meshes[0], inputs, W, args.cpu_seconds) if snap else None
     workload = (f"lossy boson-sampling MPO site updates, M={M} N={N} "
@@ -270,7 +277,8 @@ def main():
         "config": {
             "workload": workload,
             "grouped": ({"instances": B, "precision": args.precision, "updates": g["updates"],
-                         "updates_per_s": g["rate"], "autotune": g.get("tuned")} if g else None),
+                         "updates_per_s": g["rate"], "autotune": g.get("tuned"),
+                         "launch_streams": args.streams} if g else None),
             "plain_fp64": ({"instances": 1, "updates": p["updates"],
                             "updates_per_s": p["rate"], "autotune": p.get("tuned")} if p else None),
             "grouped_over_plain": (g["rate"] / p["rate"]) if (g and p) else None,
