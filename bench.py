@@ -65,6 +65,8 @@ def parse():
                     help="MiB read and written by one elementwise kernel run first (PMC byte calibration)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="time the eigensolver route split on the last warmup layer (grouped path)")
+    ap.add_argument("--eig-threshold", type=int, default=-1,
+                    help="fixed eigensolver route split for both paths instead of the autotuned one")
     ap.add_argument("--verbose", action="store_true")
     return ap.parse_args()
 
@@ -169,9 +171,9 @@ def main():
             return len(todo)
 
         t1 = time.time()
-        plain_tuned = None
+        plain_tuned = fixed_threshold(args)
         for layer in range(W):
-            if args.autotune and layer == W - 1:
+            if args.autotune and args.eig_threshold < 0 and layer == W - 1:
                 best, times = engine.autotune_eig_threshold([st], [meshes[0]], inputs, layer,
                                                             precision=engine.C128, grouped=False)
                 plain_tuned = {"layer": layer, "eig_threshold": best,
@@ -206,9 +208,9 @@ def main():
         runner = engine.GroupRunner(init.device, max_bytes=int(args.group_gib * (1 << 30)),
                                     streams=args.streams)
         t1 = time.time()
-        tuned = None
+        tuned = fixed_threshold(args)
         for layer in range(W):
-            if args.autotune and layer == W - 1:
+            if args.autotune and args.eig_threshold < 0 and layer == W - 1:
                 # tune the eigensolver route split on the last warmup layer (clones)
                 best, times = engine.autotune_eig_threshold(states, meshes, inputs, layer,
                                                             precision=prec, runner=runner)
@@ -288,6 +290,16 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def fixed_threshold(args):
+    """Set the eigensolver route split given by --eig-threshold (recorded like
+    an autotuned one, without timings); None when it is to be autotuned."""
+    if args.eig_threshold < 0:
+        return None
+    from bosonsampling_amd import _capi
+    _capi.check(_capi.lib().bsmpo_set_eig_threshold(int(args.eig_threshold)), "set_eig_threshold")
+    return {"eig_threshold": int(args.eig_threshold), "fixed": True}
 
 
 def precision_check(g, p, precision, layers):

@@ -79,7 +79,6 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
 constexpr int kTriNB = kTriPanel;
 constexpr int kTriMaxN = 2048;
 constexpr int kTriFewSectors = 256;  // below this many sectors per launch: one block shape
-constexpr int kTriSplit = 2;  // index into kTriSplitN: the block-shape boundary of a group (384)
 
 __device__ __forceinline__ double2 csub_mul(double2 a, double2 x, double2 y) {  // a - x conj(y)
   a.x -= x.x * y.x + x.y * y.y;
@@ -1048,13 +1047,13 @@ int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
   int tk = timing_begin(ST_TRIDIAG, stream);
   // The list is sorted largest first.  Few sectors (a single update): the
   // larger block shape for every sector, so that each column chain gets more
-  // threads.  Many sectors (a group across instances): the sectors above
-  // kTriSplitN[kTriSplit] (the head of the list) take the 512-thread shape and
+  // threads.  Many sectors (a group across instances): the sectors above the
+  // split (bsmpo_set_tri_split, default 384; the head of the list) take the 512-thread shape and
   // go first, on the launching stream; the rest take 256-thread blocks, more
   // per CU, on a side stream forked before the first launch (fork / join by
   // events), so that they fill the slots the long column chains leave.
-  const int nbig = g.n_tri < kTriFewSectors ? g.n_tri : g.tri_over[kTriSplit];
-  const int split = g.n_tri < kTriFewSectors ? kTriMaxN : kTriSplitN[kTriSplit];
+  const int nbig = g.n_tri < kTriFewSectors ? g.n_tri : g.tri_nbig;
+  const int split = g.n_tri < kTriFewSectors ? kTriMaxN : g.tri_split;
   const bool both = nbig > 0 && nbig < g.n_tri;
   SideStream *ss = nullptr;
   if (both) {
@@ -1115,8 +1114,8 @@ int launch_tri_vectors(const GroupHeader &g, const char *w, hipStream_t stream) 
   // go first on the launching stream, the bulk on a side stream forked before
   const size_t lds = (size_t)kBtLds * 16;
   const int2 *tl = glist(w, g.o_tri);
-  const int nbig = g.tri_over[kTriSplits - 1];  // n > 512
-  static_assert(kTriSplitN[kTriSplits - 1] == 512, "size class boundary of the back-transformation");
+  const int nbig = g.tri_nbt;  // n > 512
+  static_assert(kTriBtSplit == 512, "size class boundary of the back-transformation");
   const bool both = nbig > 0 && nbig < g.n_tri;
   SideStream *ss = nullptr;
   if (both) {

@@ -199,10 +199,14 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
       return host_array<Sector>(plans[r.x], header(plans[r.x])->o_sectors)[r.y].n;
     };
     std::stable_sort(tri, tri + nt, [&](int2 x, int2 y) { return size_of(x) > size_of(y); });
-    for (int k = 0; k < kTriSplits; ++k) {
-      g.tri_over[k] = 0;
-      while (g.tri_over[k] < nt && size_of(tri[g.tri_over[k]]) > kTriSplitN[k]) ++g.tri_over[k];
-    }
+    auto count_above = [&](int m) {
+      int c = 0;
+      while (c < nt && size_of(tri[c]) > m) ++c;
+      return c;
+    };
+    g.tri_split = tri_split();
+    g.tri_nbig = count_above(g.tri_split);
+    g.tri_nbt = count_above(kTriBtSplit);
   }
   // LDS Jacobi list grouped by size class across updates
   int64_t nj = 0;

@@ -35,11 +35,9 @@ constexpr int kJacLimit[kJacClasses] = {16, 32, 64, 96, 1 << 30};
 constexpr int kTriClass = kJacClasses - 1;
 constexpr int kTriPanel = 8;  // column panel of the blocked Householder reduction
 constexpr int kTriVecThreads = 128;  // inverse-iteration threads per sector
-// sector sizes at which a group's tridiagonal-route list (sorted largest
-// first) may be split between the two Householder block shapes
-constexpr int kTriSplits = 4;
-constexpr int kTriSplitN[kTriSplits] = {256, 320, 384, 512};
+constexpr int kTriBtSplit = 512;  // back-transformation: sectors above take the 16-row class
 int eig_threshold();  // sectors with n > threshold take the tridiagonal route
+int tri_split();      // many-sector groups: Householder blocks of 512 threads for n > split
 double tie_rel();     // relative width of the tie band at the chi-th singular value
 
 struct PlanHeader {
@@ -139,7 +137,8 @@ struct GroupHeader {
   int32_t n_pitems, pair_ok;  // pair_ok: every plan of the group supports the pair kernel
   int64_t o_pitems;
   double flops_pair_weight;
-  int32_t tri_over[kTriSplits];  // tridiagonal-route sectors with n > kTriSplitN[i]
+  int32_t tri_split, tri_nbig;  // the block-shape split in effect; sectors with n > tri_split
+  int32_t tri_nbt, pad2;        // sectors with n > kTriBtSplit
 };
 constexpr int64_t kGroupHostHead = 1024;  // host blob: header, then the staging image
 static_assert(sizeof(GroupHeader) <= kGroupHostHead, "group header too large");

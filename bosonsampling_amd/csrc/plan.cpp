@@ -19,8 +19,10 @@ namespace bsmpo {
 
 static thread_local std::string g_last_error;
 static int g_eig_threshold = 96;
+static int g_tri_split = 384;
 static double g_tie_rel = 1e-6;
 int eig_threshold() { return g_eig_threshold; }
+int tri_split() { return g_tri_split; }
 double tie_rel() { return g_tie_rel; }
 
 void set_error(const std::string &msg) { g_last_error = msg; }
@@ -445,6 +447,12 @@ int bsmpo_set_tie_rel(double rel) {
 int bsmpo_set_eig_threshold(int n) {
   if (n < 0 || n > 96) return fail(-1, "eig threshold must be in [0, 96]");
   g_eig_threshold = n;
+  return 0;
+}
+
+int bsmpo_set_tri_split(int n) {
+  if (n < 1 || n > 512) return fail(-1, "tridiagonal block-shape split must be in [1, 512]");
+  g_tri_split = n;
   return 0;
 }
 

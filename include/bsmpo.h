@@ -47,6 +47,17 @@ int bsmpo_abi_version(void);
 int bsmpo_set_eig_threshold(int n);
 
 /*
+ * Householder block-shape split (process-wide, affects groups built
+ * afterwards): in a grouped launch with many tridiagonal-route sectors, those
+ * with n > split run on 512-thread blocks, launched first, the rest on
+ * 256-thread blocks on a side stream.  Default 384, allowed [1, 512]; every
+ * value gives the same results to rounding.  No counterpart in the reference
+ * (a launch-geometry knob of this build's eigensolver, like
+ * bsmpo_set_eig_threshold).
+ */
+int bsmpo_set_tri_split(int n);
+
+/*
  * Truncation tie band (process-wide, affects plans built afterwards): the
  * kept set is every singular value above t (1 + rel), t the chi-th largest,
  * then the values within [t (1 - rel), t (1 + rel)] in concatenated order

@@ -57,6 +57,49 @@ def test_group_single_update_chunks(gpu):
         assert np.abs(x.observables()[1] - y.observables()[1]).max() <= 1e-12
 
 
+def test_tridiagonal_block_shapes_agree(gpu):
+    """A many-sector group on the tridiagonal route: the sector list (sorted
+    largest first) split between the 512-thread Householder blocks, launched
+    first, and 256-thread blocks on a side stream gives the state of the run
+    with one block shape.  Split 512 puts every sector of this case on the
+    256-thread shape; split 8 sends the sectors with n > 8 to the 512-thread
+    shape.  The shapes differ only in their reduction order."""
+    from bosonsampling_amd import engine
+    from bosonsampling_amd._capi import check, lib
+    M, d, chi, inputs, eta = CASE
+    seeds = list(range(20, 28))
+    meshes = _meshes(M, seeds)
+    ref = mpo.initialize(M, d, chi, inputs, eta)
+    runs = {}
+    check(lib().bsmpo_set_eig_threshold(0), "set_eig_threshold")
+    try:
+        for split in (512, 8):
+            check(lib().bsmpo_set_tri_split(split), "set_tri_split")
+            print(f"tridiagonal block-shape split {split}, {len(seeds)} instances", flush=True)
+            sts = [engine.DeviceMPO.from_host(ref.Gamma, ref.Lambda, ref.charge, d) for _ in seeds]
+            engine.simulate_batch(sts, meshes, inputs, runner=engine.GroupRunner(streams=1))
+            runs[split] = sts
+    finally:
+        check(lib().bsmpo_set_tri_split(384), "set_tri_split")
+        check(lib().bsmpo_set_eig_threshold(96), "set_eig_threshold")
+    # the last layer's group has many sectors on both sides of the split
+    layer = M - 1
+    sizes = []
+    for st in runs[8]:
+        ch = st.charge_host
+        for _, mode in engine.layer_modes(M, layer, max(inputs)):
+            sec = engine.Plan(d, chi, ch[mode], ch[mode + 1], ch[mode + 2]).sectors()[0]
+            sizes += list(np.minimum(sec[:, 2], sec[:, 3]))
+    sizes = np.array(sizes)
+    assert len(sizes) >= 256 and (sizes > 8).sum() > 0 and (sizes <= 8).sum() > 0
+    for x, y in zip(runs[512], runs[8]):
+        tx, mx = x.observables()
+        ty, my = y.observables()
+        assert abs(tx - ty) <= 1e-10 * abs(tx)
+        assert np.abs(mx - my).max() <= 1e-10
+        assert all(np.array_equal(a, b) for a, b in zip(x.charge_host, y.charge_host))
+
+
 @pytest.mark.parametrize("prec,tol", [(1, 1e-6), (2, 1e-4)], ids=["c64", "bf16x3"])
 def test_lowp_contraction_error(gpu, prec, tol):
     import torch
