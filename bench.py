@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--precision", default="c64", choices=["c128", "c64", "bf16x3"])
     ap.add_argument("--paths", default="plain,grouped")
     ap.add_argument("--group-gib", type=float, default=24.0, help="workspace bound of one group")
-    ap.add_argument("--streams", type=int, default=2, choices=[1, 2],
+    ap.add_argument("--streams", type=int, default=2, choices=[1, 2, 3, 4],
                     help="launching streams the groups of a layer alternate between")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="budget of the CPU-oracle baseline sample (0 = skip)")
@@ -246,8 +246,8 @@ def main():
         roof = roofline(st_grp, args.precision)
         roof["launch_streams"] = args.streams
         if args.streams > 1:
-            roof["note"] = ("groups alternate between two launching streams: a launch's duration "
-                            "includes sharing the GPU with the other stream's stages")
+            roof["note"] = (f"groups go round {args.streams} launching streams: a launch's duration "
+                            "includes sharing the GPU with the other streams' stages")
 
     cpu = cpu_baseline(snap, d, chi, meshes[0], inputs, W, args.cpu_seconds) if snap else None
     workload = (f"lossy boson-sampling MPO site updates, M={M} N={N} "

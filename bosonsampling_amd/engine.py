@@ -215,10 +215,10 @@ class GroupRunner:
     splitters of a brick-wall layer, across any number of DeviceMPO
     instances).  ``max_bytes`` bounds the device workspace of one group; a
     call with more work is split into several groups.  The groups of one call
-    are independent, so with ``streams=2`` they alternate between the current
-    stream and a second one (each with its own workspace): one group's
-    contraction then fills the CUs that the other's long eigensolver chains
-    leave idle.  The call returns with the current stream joined to both.
+    are independent, so with ``streams`` > 1 they go round the current stream
+    and streams - 1 more (each with its own workspace): one group's
+    contraction then fills the CUs that another's long eigensolver chains
+    leave idle.  The call returns with the current stream joined to all.
     """
 
     def __init__(self, device="cuda", max_bytes: int = 24 << 30, use_graph: bool = False,
@@ -226,11 +226,11 @@ class GroupRunner:
         self.device = torch.device(device)
         self.max_bytes = int(max_bytes)
         self.use_graph = bool(use_graph)
-        if streams not in (1, 2):
-            raise ValueError("GroupRunner: streams must be 1 or 2")
-        self.streams = 1 if use_graph else streams
-        self.aux = None                # second launching stream (streams=2)
-        self.works = [None, None]      # device workspace per launching stream
+        if streams not in (1, 2, 3, 4):
+            raise ValueError("GroupRunner: streams must be in 1..4")
+        self.streams = 1 if use_graph else int(streams)
+        self.aux = []                  # launching streams besides the current one
+        self.works = [None] * self.streams  # device workspace per launching stream
         self.hosts = [None, None]      # pinned host images, used alternately
         self.events = [None, None]     # consumption of each image by the device
         self.graphs = [None, None]     # graph that uploads each image (use_graph)
@@ -271,9 +271,9 @@ class GroupRunner:
         if not updates:
             return
         main = torch.cuda.current_stream(self.device)
-        if self.streams == 2 and self.aux is None:
-            self.aux = torch.cuda.Stream(device=self.device)
-        forked = [False]
+        while len(self.aux) < self.streams - 1:
+            self.aux.append(torch.cuda.Stream(device=self.device))
+        forked = [False] * len(self.aux)
         n_groups = 0
 
         def launch(grp, g):
@@ -281,13 +281,14 @@ class GroupRunner:
             arr = (C.c_void_p * n)(*[p.address for _, p in grp])
             hb, wb = C.c_size_t(), C.c_size_t()
             check(lib().bsmpo_group_query(n, arr, C.byref(hb), C.byref(wb)), "group_query")
-            side = self.streams == 2 and g % 2 == 1
-            ts = self.aux if side else main
-            if side and not forked[0]:
-                self.aux.wait_stream(main)  # the call's inputs, written on the current stream
-                forked[0] = True
+            slot_s = g % self.streams
+            side = slot_s > 0
+            ts = self.aux[slot_s - 1] if side else main
+            if side and not forked[slot_s - 1]:
+                ts.wait_stream(main)  # the call's inputs, written on the current stream
+                forked[slot_s - 1] = True
             stream = C.c_void_p(ts.cuda_stream)
-            work = self._work(wb.value, 1 if side else 0, ts if side else None)
+            work = self._work(wb.value, slot_s, ts if side else None)
             slot, host = self._host(hb.value)
             io = (_capi.UpdateIO * n)()
             for j, (k, _) in enumerate(grp):
@@ -333,8 +334,9 @@ class GroupRunner:
         if cur:
             launch(cur, n_groups)
             n_groups += 1
-        if forked[0]:
-            main.wait_stream(self.aux)
+        for a, f in zip(self.aux, forked):
+            if f:
+                main.wait_stream(a)
         self.last_info = {"groups": n_groups, "updates": len(updates)}
         # host mirror of the new centre charges: one device-to-host copy
         rows = [st.charge[mode + 1] for st, mode, _ in updates]

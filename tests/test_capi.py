@@ -109,3 +109,24 @@ def test_site_update_rejects_without_gpu():
     from bosonsampling_amd._capi import BsmpoError
     with pytest.raises(BsmpoError):
         engine.DeviceMPO(4, 3, 8)
+
+
+def test_launch_knobs_validate_their_range():
+    """The process-wide launch knobs reject values outside their range with
+    an error code and a message, and leave the setting as it was."""
+    from bosonsampling_amd._capi import lib
+    L = lib()
+    assert L.bsmpo_set_tri_split(0) < 0 and b"split" in L.bsmpo_last_error()
+    assert L.bsmpo_set_tri_split(513) < 0
+    assert L.bsmpo_set_eig_threshold(97) < 0 and b"threshold" in L.bsmpo_last_error()
+    assert L.bsmpo_set_tri_split(384) == 0
+    assert L.bsmpo_set_eig_threshold(96) == 0
+
+
+def test_group_runner_stream_count():
+    """GroupRunner takes 1..4 launching streams; a graph-replay runner uses one."""
+    from bosonsampling_amd import engine
+    with pytest.raises(ValueError):
+        engine.GroupRunner("cpu", streams=5)
+    assert engine.GroupRunner("cpu", streams=3).streams == 3
+    assert engine.GroupRunner("cpu", streams=3, use_graph=True).streams == 1
