@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--group-gib", type=float, default=24.0, help="workspace bound of one group")
     ap.add_argument("--streams", type=int, default=2, choices=[1, 2, 3, 4],
                     help="launching streams the groups of a layer alternate between")
+    ap.add_argument("--first-frac", type=float, default=0.25,
+                    help="workspace share of a layer's first group (planned while the device idles)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="budget of the CPU-oracle baseline sample (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -206,7 +208,7 @@ def main():
     if "grouped" in paths:
         states = [init.clone() for _ in range(B)]
         runner = engine.GroupRunner(init.device, max_bytes=int(args.group_gib * (1 << 30)),
-                                    streams=args.streams)
+                                    streams=args.streams, first_frac=args.first_frac)
         t1 = time.time()
         tuned = fixed_threshold(args)
         for layer in range(W):

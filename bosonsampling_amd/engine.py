@@ -222,9 +222,10 @@ class GroupRunner:
     """
 
     def __init__(self, device="cuda", max_bytes: int = 24 << 30, use_graph: bool = False,
-                 streams: int = 2):
+                 streams: int = 2, first_frac: float = 0.25):
         self.device = torch.device(device)
         self.max_bytes = int(max_bytes)
+        self.first_frac = float(first_frac)  # workspace share of a call's first group
         self.use_graph = bool(use_graph)
         if streams not in (1, 2, 3, 4):
             raise ValueError("GroupRunner: streams must be in 1..4")
@@ -328,7 +329,10 @@ class GroupRunner:
         for k, (st, mode, U3) in enumerate(updates):
             ch = st.charge_host
             p = Plan(st.d, st.chi, ch[mode], ch[mode + 1], ch[mode + 2])
-            if cur and cur_bytes + p.work_bytes > self.max_bytes:
+            # the first group of a call is planned while the device idles (the
+            # previous call ended on the charge read-back), so it is kept small
+            cap = self.max_bytes * (self.first_frac if n_groups == 0 else 1.0)
+            if cur and cur_bytes + p.work_bytes > cap:
                 launch(cur, n_groups)  # asynchronous: the next plans are built meanwhile
                 n_groups += 1
                 cur, cur_bytes = [], 0
