@@ -87,17 +87,19 @@ def stage_table(stages):
 HBM_BOUND = ("topk", "scatter", "observables", "bs_table", "tridiag")
 
 
-def measured_traffic(kernel, workload, instances, precision, eig_threshold):
+def measured_traffic(kernel, workload, instances, precision, eig_threshold, launches):
     """HBM bytes per launch of `kernel` measured on this workload by two
     rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE, scaled by a probe kernel
     of known size) and recorded in profiles/<kernel>_*_traffic.json; None if
-    no record matches."""
+    no record matches (the same grouping: a per-launch figure holds only for
+    the same number of launches)."""
     import glob
     for path in sorted(glob.glob(os.path.join(HERE, "profiles", f"{kernel}_*_traffic.json"))):
         with open(path) as f:
             rec = json.load(f)
         if (rec.get("kernel"), rec.get("workload"), rec.get("instances"), rec.get("precision"),
-                rec.get("eig_threshold")) == (kernel, workload, instances, precision, eig_threshold):
+                rec.get("eig_threshold"), rec.get("launches")) == (kernel, workload, instances,
+                                                                   precision, eig_threshold, launches):
             return rec["traffic_bytes_per_launch"], os.path.relpath(path, HERE)
     return None, None
 
@@ -260,7 +262,7 @@ def main():
     if roof is not None:
         thr = (g or {}).get("tuned") or {}
         roof["traffic"], src = measured_traffic(roof["kernel"], workload, B, args.precision,
-                                                thr.get("eig_threshold", 96))
+                                                thr.get("eig_threshold", 96), roof["launches"])
         if src:
             roof["traffic_source"] = src
     main_path = g or p
