@@ -1048,10 +1048,11 @@ int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
   // The list is sorted largest first.  Few sectors (a single update): the
   // larger block shape for every sector, so that each column chain gets more
   // threads.  Many sectors (a group across instances): the sectors above the
-  // split (bsmpo_set_tri_split, default 384; the head of the list) take the 512-thread shape and
-  // go first, on the launching stream; the rest take 256-thread blocks, more
-  // per CU, on a side stream forked before the first launch (fork / join by
-  // events), so that they fill the slots the long column chains leave.
+  // split (bsmpo_set_tri_split, default 384; the head of the list) take the
+  // 512-thread shape and go first, on the launching stream; the rest take
+  // 256-thread blocks, more per CU, on a side stream forked before the first
+  // launch (fork / join by events), so that they fill the slots the long
+  // column chains leave.
   const int nbig = g.n_tri < kTriFewSectors ? g.n_tri : g.tri_nbig;
   const int split = g.n_tri < kTriFewSectors ? kTriMaxN : g.tri_split;
   const bool both = nbig > 0 && nbig < g.n_tri;
