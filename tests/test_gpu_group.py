@@ -57,6 +57,30 @@ def test_group_single_update_chunks(gpu):
         assert np.abs(x.observables()[1] - y.observables()[1]).max() <= 1e-12
 
 
+def test_launching_streams_agree(gpu):
+    """Groups spread over two launching streams (several groups per layer,
+    the first one capped) give the state of the one-stream run."""
+    from bosonsampling_amd import engine
+    M, d, chi, inputs, eta = CASE
+    seeds = list(range(30, 36))
+    meshes = _meshes(M, seeds)
+    ref = mpo.initialize(M, d, chi, inputs, eta)
+    runs = {}
+    for ns in (1, 2):
+        print(f"launching streams {ns}, {len(seeds)} instances", flush=True)
+        sts = [engine.DeviceMPO.from_host(ref.Gamma, ref.Lambda, ref.charge, d) for _ in seeds]
+        runner = engine.GroupRunner(max_bytes=2 << 20, streams=ns)
+        engine.simulate_batch(sts, meshes, inputs, runner=runner)
+        assert runner.last_info["groups"] >= 3
+        runs[ns] = sts
+    for x, y in zip(runs[1], runs[2]):
+        tx, mx = x.observables()
+        ty, my = y.observables()
+        assert abs(tx - ty) <= 1e-12 * abs(tx)
+        assert np.abs(mx - my).max() <= 1e-12
+        assert all(np.array_equal(a, b) for a, b in zip(x.charge_host, y.charge_host))
+
+
 def test_tridiagonal_block_shapes_agree(gpu):
     """A many-sector group on the tridiagonal route: the sector list (sorted
     largest first) split between the 512-thread Householder blocks, launched
