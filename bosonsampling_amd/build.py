@@ -15,8 +15,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbsmpo.so")
-SOURCES = ["plan.cpp", "group.cpp", "capi.cpp", "adjoint.cpp", "update.hip", "contract_lowp.hip",
-           "contract_pair.hip", "eig.hip", "observables.hip", "adjoint.hip"]
+SOURCES = ["plan.cpp", "group.cpp", "capi.cpp", "adjoint.cpp", "update.hip", "contract_tile.hip",
+           "eig.hip", "observables.hip", "adjoint.hip"]
 ARCH = os.environ.get("BSMPO_ARCH", "gfx950")
 
 
@@ -34,17 +34,21 @@ def build(verbose: bool = False) -> str:
     common = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
               "-Wno-unused-command-line-argument", "-I", os.path.join(HERE, "..", "include")]
-    for src in SOURCES:
+    def compile_one(src):
         obj = os.path.join(build_dir, src + ".o")
-        lang = ["-x", "hip"]
-        cmd = [hipcc()] + common + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [hipcc()] + common + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             sys.stderr.write(r.stdout + r.stderr)
             raise RuntimeError(f"hipcc failed on {src}")
-        objs.append(obj)
+        return obj
+
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", "8"))))
+    with ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -106,10 +106,9 @@ struct UpdDesc {
   const Sector *sectors;
   const int32_t *rows, *cols;
   const Block *lblk, *cblk, *rblk;
-  const Item *items, *gram_items, *w_items, *v_items;
+  const Item *gram_items, *w_items, *v_items;
   const int32_t *charges, *sv_sector;
   double *C, *T, *G, *V, *W, *S, *lam, *td, *te, *tau, *triwork;
-  float *Cf;  // complex64 staging of the FP32 / BF16x3 contraction (unused for c128)
   int32_t *flag, *slot, *kcount;
   double *Glc, *Gcr;
   const double *LR, *U3, *LL, *LC;
@@ -119,9 +118,9 @@ struct UpdDesc {
   int32_t d, chi, edge, n_sectors, n_cblk, pad0;
   int64_t sv_total;
   double tie_rel;
-  // pair formulation of the contraction
-  const PairItem *pitems;
-  const int32_t *tau_sector, *cblk_of, *rowoff, *coloff;
+  // tile formulation of the contraction
+  const TileItem *titems;
+  const int32_t *tau_sector, *cblk_of, *rowoff, *coloff, *row_blk, *col_blk;
   int32_t n_lblk, n_rblk;
 };
 
@@ -133,7 +132,6 @@ struct DevPlan {
   const Block *lblk;
   const Block *cblk;
   const Block *rblk;
-  const Item *items;
   const Item *gram_items;
   const Item *w_items;
   const Item *v_items;
@@ -151,7 +149,6 @@ __host__ __device__ inline DevPlan dev_plan(const PlanHeader &h, const void *wor
   p.lblk = reinterpret_cast<const Block *>(b + h.o_lblk);
   p.cblk = reinterpret_cast<const Block *>(b + h.o_cblk);
   p.rblk = reinterpret_cast<const Block *>(b + h.o_rblk);
-  p.items = reinterpret_cast<const Item *>(b + h.o_items);
   p.gram_items = reinterpret_cast<const Item *>(b + h.o_gram_items);
   p.w_items = reinterpret_cast<const Item *>(b + h.o_w_items);
   p.v_items = reinterpret_cast<const Item *>(b + h.o_v_items);

@@ -24,15 +24,17 @@ int layout(int n, const void *const *plans, Layout &L) {
   std::memset(&g, 0, sizeof(g));
   g.magic = kGroupMagic;
   g.n_upd = n;
-  int64_t n_items = 0, n_gram = 0, n_tri = 0, n_sec = 0, n_v = 0, n_w = 0, n_jac = 0, n_pit = 0;
-  g.pair_ok = 1;
+  int64_t n_tiles = 0, n_gram = 0, n_tri = 0, n_sec = 0, n_v = 0, n_w = 0, n_jac = 0;
+  int64_t tile_cnt[kTileClasses] = {};
   for (int u = 0; u < n; ++u) {
     const PlanHeader *h = header(plans[u]);
     if (!h || h->magic != kPlanMagic) return fail(-1, "group: entry is not a bsmpo plan");
-    n_items += h->n_items;
-    n_pit += h->n_pitems;
-    if (!h->pair_ok) g.pair_ok = 0;
-    g.flops_pair_weight += h->flops_pair_weight;
+    n_tiles += h->n_titems;
+    for (int c = 0; c < kTileClasses; ++c) tile_cnt[c] += h->tile_off[c + 1] - h->tile_off[c];
+    g.flops_weight += h->flops_weight;
+    g.mfma_k4 += h->mfma_k4;
+    g.mfma_k32 += h->mfma_k32;
+    g.d_max = std::max(g.d_max, h->d);
     n_gram += h->n_gram_items;
     n_v += h->n_v_items;
     n_w += h->n_w_items;
@@ -57,11 +59,11 @@ int layout(int n, const void *const *plans, Layout &L) {
     g.flops_tridiag += h->flops_tridiag;
     g.bytes_tridiag += h->bytes_tridiag;
   }
-  if (n_items > INT32_MAX || n_gram > INT32_MAX || n_w > INT32_MAX || n_v > INT32_MAX ||
-      n_pit > INT32_MAX)
+  if (n_tiles > INT32_MAX || n_gram > INT32_MAX || n_w > INT32_MAX || n_v > INT32_MAX)
     return fail(-1, "group: too many work items");
-  g.n_items = (int32_t)n_items;
-  g.n_pitems = (int32_t)n_pit;
+  g.n_tiles = (int32_t)n_tiles;
+  g.tile_off[0] = 0;
+  for (int c = 0; c < kTileClasses; ++c) g.tile_off[c + 1] = g.tile_off[c] + (int32_t)tile_cnt[c];
   g.n_gram = (int32_t)n_gram;
   g.n_tri = (int32_t)n_tri;
   g.n_sec = (int32_t)n_sec;
@@ -74,8 +76,7 @@ int layout(int n, const void *const *plans, Layout &L) {
     return at;
   };
   g.o_desc = place((int64_t)sizeof(UpdDesc) * n);
-  g.o_items = place(8 * n_items);
-  g.o_pitems = place(8 * n_pit);
+  g.o_tiles = place(8 * n_tiles);
   g.o_gram = place(8 * n_gram);
   g.o_jac = place(8 * n_jac);
   g.o_tri = place(8 * n_tri);
@@ -116,15 +117,16 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
   char *img = host_blob + kGroupHostHead;  // image of d_work[0, staging_bytes)
   std::memset(img, 0, (size_t)g.staging_bytes);
   UpdDesc *desc = reinterpret_cast<UpdDesc *>(img + g.o_desc);
-  int2 *items = reinterpret_cast<int2 *>(img + g.o_items);
-  int2 *pit = reinterpret_cast<int2 *>(img + g.o_pitems);
+  int2 *tiles = reinterpret_cast<int2 *>(img + g.o_tiles);
+  int32_t tile_fill[kTileClasses];
+  for (int c = 0; c < kTileClasses; ++c) tile_fill[c] = g.tile_off[c];
   int2 *gram = reinterpret_cast<int2 *>(img + g.o_gram);
   int2 *jac = reinterpret_cast<int2 *>(img + g.o_jac);
   int2 *tri = reinterpret_cast<int2 *>(img + g.o_tri);
   int2 *sec = reinterpret_cast<int2 *>(img + g.o_sec);
   int2 *vl = reinterpret_cast<int2 *>(img + g.o_v);
   int2 *wl = reinterpret_cast<int2 *>(img + g.o_w);
-  int64_t ni = 0, ng = 0, nt = 0, ns = 0, nv = 0, nw = 0, np = 0;
+  int64_t ng = 0, nt = 0, ns = 0, nv = 0, nw = 0;
   for (int u = 0; u < n; ++u) {
     const PlanHeader *h = header(plans[u]);
     std::memcpy(img + L.plan_off[u], reinterpret_cast<const char *>(plans[u]) + h->dev_begin,
@@ -138,7 +140,6 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
     D.lblk = reinterpret_cast<const Block *>(pb + h->o_lblk);
     D.cblk = reinterpret_cast<const Block *>(pb + h->o_cblk);
     D.rblk = reinterpret_cast<const Block *>(pb + h->o_rblk);
-    D.items = reinterpret_cast<const Item *>(pb + h->o_items);
     D.gram_items = reinterpret_cast<const Item *>(pb + h->o_gram_items);
     D.w_items = reinterpret_cast<const Item *>(pb + h->o_w_items);
     D.v_items = reinterpret_cast<const Item *>(pb + h->o_v_items);
@@ -155,7 +156,6 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
     D.te = reinterpret_cast<double *>(wb + h->w_te);
     D.tau = reinterpret_cast<double *>(wb + h->w_tau);
     D.triwork = reinterpret_cast<double *>(wb + h->w_triwork);
-    D.Cf = nullptr;
     D.flag = reinterpret_cast<int32_t *>(wb + h->w_flag);
     D.slot = reinterpret_cast<int32_t *>(wb + h->w_slot);
     D.kcount = reinterpret_cast<int32_t *>(wb + h->w_kcount);
@@ -165,6 +165,7 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
     D.U3 = io[u].U3;
     D.LL = io[u].LL;
     D.LC = io[u].LC;
+    if (io[u].LC) g.has_lc = 1;
     D.charge_c = io[u].charge_c;
     D.lambda_c = io[u].lambda_c;
     D.num_lambda = io[u].num_lambda;
@@ -175,15 +176,17 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
     D.n_cblk = h->n_cblk;
     D.sv_total = h->sv_total;
     D.tie_rel = h->tie_rel;
-    D.pitems = reinterpret_cast<const PairItem *>(pb + h->o_pitems);
+    D.titems = reinterpret_cast<const TileItem *>(pb + h->o_titems);
     D.tau_sector = reinterpret_cast<const int32_t *>(pb + h->o_tau_sector);
     D.cblk_of = reinterpret_cast<const int32_t *>(pb + h->o_cblk_of);
     D.rowoff = reinterpret_cast<const int32_t *>(pb + h->o_rowoff);
     D.coloff = reinterpret_cast<const int32_t *>(pb + h->o_coloff);
+    D.row_blk = reinterpret_cast<const int32_t *>(pb + h->o_row_blk);
+    D.col_blk = reinterpret_cast<const int32_t *>(pb + h->o_col_blk);
     D.n_lblk = h->n_lblk;
     D.n_rblk = h->n_rblk;
-    for (int i = 0; i < h->n_items; ++i) items[ni++] = make_int2(u, i);
-    for (int i = 0; i < h->n_pitems; ++i) pit[np++] = make_int2(u, i);
+    for (int c = 0; c < kTileClasses; ++c)
+      for (int i = h->tile_off[c]; i < h->tile_off[c + 1]; ++i) tiles[tile_fill[c]++] = make_int2(u, i);
     for (int i = 0; i < h->n_gram_items; ++i) gram[ng++] = make_int2(u, i);
     for (int i = 0; i < h->n_v_items; ++i) vl[nv++] = make_int2(u, i);
     for (int i = 0; i < h->n_w_items; ++i) wl[nw++] = make_int2(u, i);

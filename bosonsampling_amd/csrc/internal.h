@@ -40,9 +40,12 @@ int eig_threshold();  // sectors with n > threshold take the tridiagonal route
 int tri_split();      // many-sector groups: Householder blocks of 512 threads for n > split
 double tie_rel();     // relative width of the tie band at the chi-th singular value
 
+constexpr int kTileClasses = 4;
+constexpr int kTileElems[kTileClasses] = {1024, 2048, 4096, 8192};
+
 struct PlanHeader {
   int32_t magic, d, chi, n_sectors;
-  int32_t n_items, n_lblk, n_cblk, n_rblk;
+  int32_t n_items, n_lblk, n_cblk, n_rblk;  // n_items: contraction tile items
   int32_t n_gram_items, n_w_items, n_v_items, pad0;
   int64_t c_elems, g_elems, sv_total, v_elems, w_elems;
   int64_t rows_total, cols_total;
@@ -56,7 +59,7 @@ struct PlanHeader {
   // host byte offsets of the device-copied region and total blob size
   int64_t dev_begin, blob_bytes;
   // byte offsets RELATIVE TO dev_begin (i.e. inside the device copy)
-  int64_t o_sectors, o_rows, o_cols, o_lblk, o_cblk, o_rblk, o_items;
+  int64_t o_sectors, o_rows, o_cols, o_lblk, o_cblk, o_rblk;
   int64_t o_gram_items, o_w_items, o_v_items, o_charges, o_sv_sector, o_jac_order;
   int64_t dev_bytes;  // size of the device-copied region
   // sectors grouped by Jacobi size class: jac_order[jac_off[c] .. jac_off[c+1])
@@ -67,27 +70,37 @@ struct PlanHeader {
   int64_t w_td, w_te, w_tau, w_kcount, w_triwork, w_total;
   int32_t eig_threshold, pad_t;
   double tie_rel;
-  // pair formulation of the contraction (see PairItem)
-  int32_t n_pitems, pair_ok;  // pair_ok = 0: a box side exceeds kPairMaxB, use the tile kernel
-  int64_t o_pitems, o_tau_sector, o_cblk_of, o_rowoff, o_coloff;
-  double flops_pair_weight;   // 8 sum_pairs nL nR B0 B1 (B0 + B1)
+  // MFMA tile formulation of the contraction (see TileItem)
+  int32_t n_titems, tile_off[kTileClasses + 1];
+  int64_t o_titems, o_tau_sector, o_cblk_of, o_rowoff, o_coloff, o_row_blk, o_col_blk;
+  double flops_weight;  // BS transforms: 8 sum_pairs nL nR B0 B1 (B0 + B1)
+  double mfma_k4;       // sum over tile items and their centre blocks of K padded to 4
+  double mfma_k32;      // the same with K padded to 32 (BF16x3 steps)
 };
 
-// Pair formulation of the sector contraction.  For a left block a (charge cl)
-// and a right block b (cr <= cl) every tau and every centre charge cc of the
-// box cr <= q <= cl take part, and the BS weight factorises over the two
-// charge components:
-//   u(cl, cc, cr; tau) = alpha[tau0][cc0] conj(beta[tau1][cc1]),
-//   alpha[t][c] = U3[cl0 - t][t - cr0][cl0 - c], beta the same on component 1.
-// So C_tau[a, b] = sum_cc u P_cc with P_cc = Glc[a, cc] Gcr[cc, b] is
-//   C[t0][t1] = sum_c0 alpha[t0][c0] sum_c1 conj(beta[t1][c1]) P[c0][c1],
-// computed once per (a, b) for every tau.  A PairItem is a tile of tl x tr
-// elements of one (a, b) pair.
-constexpr int kPairMaxB = 16;     // largest box side handled (d <= 16 always fits)
-constexpr int kPairLds = 3072;    // B0 B1 tl tr <= kPairLds (LDS elements per item)
-struct PairItem {
-  int32_t a, b, l0, r0, tl, tr, pad0, pad1;
+// Tile formulation of the sector contraction.  A TileItem is one pass over
+// a 16 x 16 tile of (left row, right column) elements: rows r0.., columns
+// c0.. of the update (nr, nc of them exist); the pass stores rows
+// [s0, s0 + rp) of the tile.  [lo, hi] is the union box of the pass's
+// elements' boxes cr <= q <= cl (lo = min right charge over the tile's
+// columns, hi = max left charge over the pass's rows); the left blocks of
+// the pass's rows are a_lo .. a_lo + na - 1, the right blocks of the tile's
+// columns b_lo .. b_lo + nb - 1.  The kernel keeps, in LDS, the products P_q
+// of every union-box position for the pass's rows (nbox x rp x 16 elements)
+// and the output offset tables (nbox x (12 + 4 (na + nb)) bytes); a tile
+// whose union box is too large for 16 rows in the largest LDS class is split
+// into passes of rp < 16 rows.  Items are sorted into LDS classes: class c
+// holds passes whose need (elements plus tables in 8-byte units) is at most
+// kTileElems[c].
+constexpr int kMaxD = 16;  // the reference's limit too: 256-entry centre-charge tables
+                           // (src/update_MPO_kernel.cu:236-238)
+struct TileItem {
+  int32_t r0, c0;
+  int8_t nr, nc, s0, rp, lo0, lo1, hi0, hi1;
+  int16_t a_lo, b_lo;
+  int8_t na, nb, pad0, pad1;
 };
+static_assert(sizeof(TileItem) == 24, "TileItem layout");
 
 struct Sector {
   int32_t t0, t1, L, R;
@@ -107,7 +120,7 @@ struct Block {
   int32_t c0, c1, start, end;
 };
 
-// contraction / GEMM work item: a 16-row x 16-col tile of a sector output
+// GEMM work item of the Gram / W / V stages: a 16-row x 16-col tile of a sector matrix
 struct Item {
   int32_t sector;
   int32_t blk;   // left-block index (contraction) or unused
@@ -125,18 +138,19 @@ struct Item {
 constexpr int32_t kGroupMagic = 0x4253475a;  // "BSGZ"
 struct GroupHeader {
   int32_t magic, n_upd, chi, pad0;
-  int32_t n_items, n_gram, n_tri, n_sec, n_v, n_w;
+  int32_t n_gram, n_tri, n_sec, n_v, n_w, pad4;
   int32_t jac_off[kTriClass + 1];
   int32_t jac_max_n[kTriClass];
   int32_t tri_max_n, pad1;
-  int64_t o_desc, o_items, o_gram, o_jac, o_tri, o_sec, o_v, o_w;
+  int64_t o_desc, o_gram, o_jac, o_tri, o_sec, o_v, o_w;
   int64_t staging_bytes, work_bytes, max_sv, host_bytes;
   double flops_alg, flops_gram, flops_jacobi, flops_wmat, flops_vectors, bytes_alg16;
   double c_elems, g_elems, v_elems, w_elems, sv_total;
   double flops_tridiag, bytes_tridiag;
-  int32_t n_pitems, pair_ok;  // pair_ok: every plan of the group supports the pair kernel
-  int64_t o_pitems;
-  double flops_pair_weight;
+  int32_t n_tiles, d_max;
+  int32_t tile_off[kTileClasses + 1], has_lc;  // has_lc: some update weights centre rows (LC)
+  int64_t o_tiles;
+  double flops_weight, mfma_k4, mfma_k32;
   int32_t tri_split, tri_nbig;  // the block-shape split in effect; sectors with n > tri_split
   int32_t tri_nbt, pad2;        // sectors with n > kTriBtSplit
 };
@@ -174,10 +188,6 @@ void timing_end(int token, hipStream_t stream, double flops, double bytes);
 // launchers implemented in the .hip files; all enqueue on `stream`.
 int launch_bs_table(int d, int nb, const double *d_B, double *d_U3, hipStream_t stream);
 int launch_contract(const GroupHeader &g, const char *d_work, int precision, hipStream_t stream);
-int launch_contract_lowp(const GroupHeader &g, const char *d_work, int precision,
-                         hipStream_t stream);
-int launch_contract_pair(const GroupHeader &g, const char *d_work, int precision,
-                         hipStream_t stream);
 int launch_svd(const GroupHeader &g, const char *d_work, hipStream_t stream);
 int launch_truncate(const GroupHeader &g, const char *d_work, hipStream_t stream);
 

@@ -38,12 +38,12 @@ struct Built {
   std::vector<Sector> sectors;
   std::vector<int32_t> rows, cols;
   std::vector<Block> lblk, cblk, rblk;
-  std::vector<Item> items, gram_items, w_items, v_items;
+  std::vector<Item> gram_items, w_items, v_items;
   std::vector<int32_t> charges;  // CL, CC, CR: 3 x chi x 2
   std::vector<int32_t> sv_sector;  // sector of every singular-value slot
   std::vector<int32_t> jac_order;  // sector indices grouped by Jacobi class
-  std::vector<PairItem> pitems;    // pair-formulation work items
-  std::vector<int32_t> tau_sector, cblk_of, rowoff, coloff;
+  std::vector<TileItem> titems;    // contraction tiles, by LDS class
+  std::vector<int32_t> tau_sector, cblk_of, rowoff, coloff, row_blk, col_blk;
 };
 
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
@@ -76,7 +76,9 @@ void add_tiles(std::vector<Item> &items, int sector, int blk, int r_begin, int r
 
 int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *CR, int only_t0,
           int only_t1, Built &b) {
-  if (d < 1 || d > 32) return fail(-1, "d must be in [1, 32]");
+  if (d < 1 || d > kMaxD)
+    return fail(-1, "d must be in [1, 16] (the sector contraction keeps one BS box side per "
+                    "register array; the reference's kernel has the same bound)");
   if (chi < 1) return fail(-1, "chi must be positive");
   std::memset(&b.h, 0, sizeof(PlanHeader));
   int rc;
@@ -154,10 +156,6 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
         if (!lower_only) b.h.flops_jacobi += 8.0 * n * n * n;
       }
       int sid = (int)b.sectors.size();
-      for (auto &p : lsel) {
-        const Block &bl = b.lblk[p.first];
-        add_tiles(b.items, sid, p.first, p.second, p.second + (bl.end - bl.start), 0, R);
-      }
       if (lower_only) {  // 16 x 16 tiles on or below the diagonal
         for (int r = 0; r < s.n; r += 16)
           for (int c = 0; c <= r; c += 16)
@@ -249,7 +247,7 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
   }
   b.h.jac_off[kJacClasses] = (int32_t)b.jac_order.size();
 
-  // pair formulation: tau -> sector, centre charge -> block, block offsets in sectors
+  // contraction tables: tau -> sector, centre charge -> block, block offsets in sectors
   {
     const int D1 = d + 1;
     const int ns = (int)b.sectors.size(), nl = (int)b.lblk.size(), nr = (int)b.rblk.size();
@@ -275,27 +273,117 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
           o += b.rblk[c].end - b.rblk[c].start;
         }
     }
-    b.h.pair_ok = 1;
     for (int a = 0; a < nl; ++a) {
       const Block &la = b.lblk[a];
       for (int c = 0; c < nr; ++c) {
         const Block &rb = b.rblk[c];
         if (!(la.c0 >= rb.c0 && la.c1 >= rb.c1)) continue;
-        if (only_t0 >= 0 && !(la.c0 >= only_t0 && only_t0 >= rb.c0 && la.c1 >= only_t1 &&
-                              only_t1 >= rb.c1))
-          continue;
         const int B0 = la.c0 - rb.c0 + 1, B1 = la.c1 - rb.c1 + 1;
-        if (B0 > kPairMaxB || B1 > kPairMaxB) b.h.pair_ok = 0;
-        const int na = la.end - la.start, nb = rb.end - rb.start;
-        const int emax = std::max(1, std::min(64, kPairLds / (B0 * B1)));
-        const int tr = std::min(nb, std::min(8, emax));
-        const int tl = std::min(na, std::max(1, emax / tr));
-        for (int l0 = 0; l0 < na; l0 += tl)
-          for (int r0 = 0; r0 < nb; r0 += tr)
-            b.pitems.push_back(PairItem{a, c, l0, r0, std::min(tl, na - l0), std::min(tr, nb - r0), 0, 0});
-        b.h.flops_pair_weight += 8.0 * na * nb * (double)(B0 * B1) * (B0 + B1);
+        b.h.flops_weight += 8.0 * (la.end - la.start) * (rb.end - rb.start) *
+                            (double)(B0 * B1) * (B0 + B1);
       }
     }
+    // contraction tiles: 16 x 16 (left row, right column) tiles, visited in
+    // 8 x 8 super-tiles so that neighbouring items share operand panels
+    b.row_blk.assign((size_t)chi, -1);
+    b.col_blk.assign((size_t)chi, -1);
+    int nLv = 0, nRv = 0;
+    for (int a = 0; a < nl; ++a) {
+      for (int i = b.lblk[a].start; i < b.lblk[a].end; ++i) b.row_blk[i] = a;
+      nLv = std::max(nLv, b.lblk[a].end);
+    }
+    for (int c = 0; c < nr; ++c) {
+      for (int i = b.rblk[c].start; i < b.rblk[c].end; ++i) b.col_blk[i] = c;
+      nRv = std::max(nRv, b.rblk[c].end);
+    }
+    std::vector<TileItem> cls[kTileClasses];
+    const int kLast = kTileElems[kTileClasses - 1];
+    struct Pass {
+      TileItem it;
+      int need;
+      double k4, k32;
+      bool any;
+    };
+    // one pass over rows [r0 + s0, r0 + s0 + rp) of the tile (r0, c0)
+    auto make_pass = [&](int r0, int c0, int tnr, int tnc, int s0, int rp, int lo0, int lo1,
+                         int c_lo, int c_hi) {
+      Pass ps{};
+      int hi0 = -1, hi1 = -1, a_lo = nl, a_hi = -1;
+      for (int i = r0 + s0; i < r0 + std::min(tnr, s0 + rp); ++i) {
+        const int a = b.row_blk[i];
+        if (a < 0) continue;
+        hi0 = std::max(hi0, b.lblk[a].c0);
+        hi1 = std::max(hi1, b.lblk[a].c1);
+        a_lo = std::min(a_lo, a);
+        a_hi = std::max(a_hi, a);
+      }
+      for (int a = a_lo; a <= a_hi && !ps.any; ++a)
+        for (int c = c_lo; c <= c_hi && !ps.any; ++c) {
+          const Block &la = b.lblk[a], &rb = b.rblk[c];
+          if (!(la.c0 >= rb.c0 && la.c1 >= rb.c1)) continue;
+          if (only_t0 >= 0 && !(la.c0 >= only_t0 && only_t0 >= rb.c0 && la.c1 >= only_t1 &&
+                                only_t1 >= rb.c1))
+            continue;
+          ps.any = true;
+        }
+      if (!ps.any) return ps;
+      for (int q0 = lo0; q0 <= hi0; ++q0)
+        for (int q1 = lo1; q1 <= hi1; ++q1) {
+          const int cb = b.cblk_of[(size_t)q0 * D1 + q1];
+          if (cb < 0) continue;
+          const int n = b.cblk[cb].end - b.cblk[cb].start;
+          ps.k4 += (n + 3) / 4 * 4;
+          ps.k32 += (n + 31) / 32 * 32;
+        }
+      const int nbox = (hi0 - lo0 + 1) * (hi1 - lo1 + 1);
+      const int na = a_hi - a_lo + 1, nbk = c_hi - c_lo + 1;
+      ps.need = nbox * rp * 16 + (nbox * (12 + 4 * (na + nbk)) + 7) / 8;
+      ps.it = TileItem{r0, c0, (int8_t)tnr, (int8_t)tnc, (int8_t)s0, (int8_t)rp,
+                       (int8_t)lo0, (int8_t)lo1, (int8_t)hi0, (int8_t)hi1,
+                       (int16_t)a_lo, (int16_t)c_lo, (int8_t)na, (int8_t)nbk, 0, 0};
+      return ps;
+    };
+    std::vector<Pass> passes;
+    for (int R0 = 0; R0 < nLv; R0 += 128)
+      for (int C0 = 0; C0 < nRv; C0 += 128)
+        for (int r0 = R0; r0 < std::min(R0 + 128, nLv); r0 += 16)
+          for (int c0 = C0; c0 < std::min(C0 + 128, nRv); c0 += 16) {
+            const int tnr = std::min(16, nLv - r0), tnc = std::min(16, nRv - c0);
+            int lo0 = d, lo1 = d, c_lo = nr, c_hi = -1;
+            for (int j = c0; j < c0 + tnc; ++j) {
+              const int c = b.col_blk[j];
+              if (c < 0) continue;
+              lo0 = std::min(lo0, b.rblk[c].c0);
+              lo1 = std::min(lo1, b.rblk[c].c1);
+              c_lo = std::min(c_lo, c);
+              c_hi = std::max(c_hi, c);
+            }
+            if (c_hi < 0) continue;
+            for (int rp = 16; rp >= 1; rp /= 2) {
+              passes.clear();
+              bool fits = true;
+              for (int s0 = 0; s0 < tnr && fits; s0 += rp) {
+                Pass ps = make_pass(r0, c0, tnr, tnc, s0, rp, lo0, lo1, c_lo, c_hi);
+                if (!ps.any) continue;
+                if (ps.need > kLast) fits = false;
+                passes.push_back(ps);
+              }
+              if (fits) break;
+              if (rp == 1) return fail(-1, "contraction tile does not fit the LDS classes");
+            }
+            for (const Pass &ps : passes) {
+              int c = 0;
+              while (ps.need > kTileElems[c]) ++c;
+              cls[c].push_back(ps.it);
+              b.h.mfma_k4 += ps.k4;
+              b.h.mfma_k32 += ps.k32;
+            }
+          }
+    for (int c = 0; c < kTileClasses; ++c) {
+      b.h.tile_off[c] = (int32_t)b.titems.size();
+      b.titems.insert(b.titems.end(), cls[c].begin(), cls[c].end());
+    }
+    b.h.tile_off[kTileClasses] = (int32_t)b.titems.size();
   }
 
   b.charges.resize((size_t)6 * chi);
@@ -308,7 +396,8 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
   h.d = d;
   h.chi = chi;
   h.n_sectors = (int32_t)b.sectors.size();
-  h.n_items = (int32_t)b.items.size();
+  h.n_items = (int32_t)b.titems.size();
+  h.n_titems = h.n_items;
   h.n_lblk = (int32_t)b.lblk.size();
   h.n_cblk = (int32_t)b.cblk.size();
   h.n_rblk = (int32_t)b.rblk.size();
@@ -339,19 +428,19 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
   h.o_lblk = place(sizeof(Block) * b.lblk.size());
   h.o_cblk = place(sizeof(Block) * b.cblk.size());
   h.o_rblk = place(sizeof(Block) * b.rblk.size());
-  h.o_items = place(sizeof(Item) * b.items.size());
   h.o_gram_items = place(sizeof(Item) * b.gram_items.size());
   h.o_w_items = place(sizeof(Item) * b.w_items.size());
   h.o_v_items = place(sizeof(Item) * b.v_items.size());
   h.o_charges = place(sizeof(int32_t) * b.charges.size());
   h.o_sv_sector = place(sizeof(int32_t) * b.sv_sector.size());
   h.o_jac_order = place(sizeof(int32_t) * b.jac_order.size());
-  h.o_pitems = place(sizeof(PairItem) * b.pitems.size());
+  h.o_titems = place(sizeof(TileItem) * b.titems.size());
   h.o_tau_sector = place(sizeof(int32_t) * b.tau_sector.size());
   h.o_cblk_of = place(sizeof(int32_t) * b.cblk_of.size());
   h.o_rowoff = place(sizeof(int32_t) * b.rowoff.size());
   h.o_coloff = place(sizeof(int32_t) * b.coloff.size());
-  h.n_pitems = (int32_t)b.pitems.size();
+  h.o_row_blk = place(sizeof(int32_t) * b.row_blk.size());
+  h.o_col_blk = place(sizeof(int32_t) * b.col_blk.size());
   h.dev_bytes = o;
   h.blob_bytes = h.dev_begin + h.dev_bytes;
 
@@ -490,18 +579,19 @@ int bsmpo_plan_build(int d, int chi, const int32_t *h_CL, const int32_t *h_CC,
   put(dev, b.h.o_lblk, b.lblk);
   put(dev, b.h.o_cblk, b.cblk);
   put(dev, b.h.o_rblk, b.rblk);
-  put(dev, b.h.o_items, b.items);
   put(dev, b.h.o_gram_items, b.gram_items);
   put(dev, b.h.o_w_items, b.w_items);
   put(dev, b.h.o_v_items, b.v_items);
   put(dev, b.h.o_charges, b.charges);
   put(dev, b.h.o_sv_sector, b.sv_sector);
   put(dev, b.h.o_jac_order, b.jac_order);
-  put(dev, b.h.o_pitems, b.pitems);
+  put(dev, b.h.o_titems, b.titems);
   put(dev, b.h.o_tau_sector, b.tau_sector);
   put(dev, b.h.o_cblk_of, b.cblk_of);
   put(dev, b.h.o_rowoff, b.rowoff);
   put(dev, b.h.o_coloff, b.coloff);
+  put(dev, b.h.o_row_blk, b.row_blk);
+  put(dev, b.h.o_col_blk, b.col_blk);
   return 0;
 }
 

@@ -1,8 +1,6 @@
 // gfx950 kernels of the beam-splitter site update (complex128 path).
 //
 //   k_bs_table     Fock BS elements, FP64            (src/u_kernel.cu:37-70)
-//   k_contract     grouped sector contraction on FP64 MFMA, LR epilogue
-//                  (src/update_MPO_kernel.cu:193-734, Rank.py:114-118)
 //   k_gram         G = T T^H or T^H T per sector on FP64 MFMA
 //   k_jacobi       one-sided Jacobi on G (LDS-resident when it fits)
 //   k_vectors      V = T Y diag(1/sigma) for sectors with L > R
@@ -57,81 +55,6 @@ int launch_bs_table(int d, int nb, const double *d_B, double *d_U3, hipStream_t 
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, 0.0, 16.0 * nb * d * d * d);
   return 0;
-}
-
-// ========================================================= contraction ==
-// One wave per work item = a 16 x 16 tile of C_tau whose rows lie in one
-// left charge block cl.  The BS weight u(cl, ch, cr; tau) depends on the
-// column's cr and on the centre block ch only, so it is folded into the B
-// operand as it is loaded:  B'[c][j] = Gcr[c][col_j] * u(cl, ch(c), cr_j).
-__global__ __launch_bounds__(256) void k_contract_c128(const UpdDesc *__restrict__ Ud,
-                                                       const int2 *__restrict__ list, int nlist) {
-  const int lane = threadIdx.x & 63;
-  const int i16 = lane & 15, kq = lane >> 4;
-  const int nwaves = gridDim.x * 4;
-  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < nlist; it += nwaves) {
-    const int2 ref = list[it];
-    const UpdDesc &u = Ud[ref.x];
-    const int d = u.d, chi = u.chi;
-    const int32_t *CR = u.charges + 4 * chi;
-    const Item item = u.items[ref.y];
-    const Sector s = u.sectors[item.sector];
-    const Block bl = u.lblk[item.blk];
-    const bool rv = i16 < item.nr;
-    const bool cv = i16 < item.nc;
-    const int grow = rv ? u.rows[s.row_off + item.r0 + i16] : 0;
-    const int gcol = cv ? u.cols[s.col_off + item.c0 + i16] : 0;
-    const int cr0 = cv ? CR[2 * gcol] : (1 << 20);
-    const int cr1 = cv ? CR[2 * gcol + 1] : (1 << 20);
-    // smallest column charge of the tile (columns are sorted: lane 0 has cr0)
-    const int mincr0 = __shfl(cr0, 0);
-    int mincr1 = cr1;
-    for (int o = 8; o >= 1; o >>= 1) mincr1 = min(mincr1, __shfl_xor(mincr1, o));
-    const double2 *G1 = reinterpret_cast<const double2 *>(u.Glc);
-    const double2 *G2 = reinterpret_cast<const double2 *>(u.Gcr);
-    const double2 *U = reinterpret_cast<const double2 *>(u.U3);
-    const double *LC = u.LC;
-    dbl4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
-    for (int cb = 0; cb < u.n_cblk; ++cb) {
-      const Block bc = u.cblk[cb];
-      if (bc.c0 > bl.c0 || bc.c1 > bl.c1) continue;
-      if (bc.c0 < mincr0 || bc.c1 < mincr1) continue;
-      cplx w{0.0, 0.0};
-      if (cv && bc.c0 >= cr0 && bc.c1 >= cr1) {
-        const double2 a = U[((bl.c0 - s.t0) * d + (s.t0 - cr0)) * d + (bl.c0 - bc.c0)];
-        const double2 b = U[((bl.c1 - s.t1) * d + (s.t1 - cr1)) * d + (bl.c1 - bc.c1)];
-        w = cmulc(cplx{a.x, a.y}, cplx{b.x, b.y});
-      }
-      for (int k0 = bc.start; k0 < bc.end; k0 += 4) {
-        const int k = k0 + kq;
-        const bool kv = k < bc.end;
-        cplx a{0.0, 0.0}, g{0.0, 0.0};
-        if (kv && rv) {
-          const double2 v = G1[(int64_t)grow * chi + k];
-          a = cplx{v.x, v.y};
-        }
-        if (kv && cv) {
-          const double2 v = G2[(int64_t)k * chi + gcol];
-          const double lc = LC ? LC[k] : 1.0;
-          g = cplx{v.x * lc, v.y * lc};
-        }
-        cmfma_f64(accr, acci, a, cmul(g, w));
-      }
-    }
-    if (cv) {
-      const double lr = u.LR[gcol];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = mfma64_row(kq, r);
-        if (row < item.nr) {
-          const int64_t idx = s.c_off + (int64_t)(item.r0 + row) * s.R + item.c0 + i16;
-          const double wt = u.LL ? lr * u.LL[u.rows[s.row_off + item.r0 + row]] : lr;
-          cstore(u.C, idx, cplx{accr[r], acci[r]});
-          cstore(u.T, idx, cplx{accr[r] * wt, acci[r] * wt});
-        }
-      }
-    }
-  }
 }
 
 // ================================================================ Gram ==
@@ -560,24 +483,6 @@ static inline const UpdDesc *gdesc(const GroupHeader &g, const char *w) {
 }
 static inline const int2 *glist(const char *w, int64_t off) {
   return reinterpret_cast<const int2 *>(w + off);
-}
-
-int launch_contract(const GroupHeader &g, const char *w, int precision, hipStream_t stream) {
-  if (g.n_items == 0) return 0;
-  const int tk = timing_begin(ST_CONTRACT, stream);
-  if (g.pair_ok && (precision == BSMPO_C128 || precision == BSMPO_C64)) {
-    int rc = launch_contract_pair(g, w, precision, stream);
-    if (rc) return rc;
-  } else if (precision == BSMPO_C128) {
-    hipLaunchKernelGGL(k_contract_c128, dim3(grid_for(g.n_items)), dim3(256), 0, stream,
-                       gdesc(g, w), glist(w, g.o_items), g.n_items);
-  } else {
-    int rc = launch_contract_lowp(g, w, precision, stream);
-    if (rc) return rc;
-  }
-  BSMPO_HIP(hipGetLastError());
-  timing_end(tk, stream, g.flops_alg, g.bytes_alg16);
-  return 0;
 }
 
 int launch_svd(const GroupHeader &g, const char *w, hipStream_t stream) {

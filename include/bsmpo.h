@@ -68,11 +68,14 @@ int bsmpo_set_tri_split(int n);
  */
 int bsmpo_set_tie_rel(double rel);
 
-/* Contraction arithmetic of the sector GEMMs. */
+/* Contraction arithmetic of the sector GEMMs (the 16 x 16 block products
+ * P_q = Glc Gcr of the sector contraction; the BS-weight transforms that
+ * follow run on the vector ALUs in the same precision). */
 enum bsmpo_precision {
-  BSMPO_C128 = 0,    /* complex128 on FP64 MFMA                             */
-  BSMPO_C64 = 1,     /* complex64 on FP32 MFMA, FP32 accumulation            */
-  BSMPO_BF16X3 = 2   /* split-BF16x3 compensated products, FP32 accumulation */
+  BSMPO_C128 = 0,    /* complex128 on v_mfma_f64_16x16x4f64                   */
+  BSMPO_C64 = 1,     /* complex64 on v_mfma_f32_16x16x4f32, FP32 accumulation */
+  BSMPO_BF16X3 = 2   /* split-BF16x3 (hi hi + hi lo + lo hi) on gfx950's
+                        v_mfma_f32_16x16x32_bf16, FP32 accumulation          */
 };
 
 /* ------------------------------------------------------------- timing -- */
@@ -116,7 +119,9 @@ int bsmpo_bs_table(int d, int nb, const double *d_B, double *d_U3, void *stream)
  * and the contraction work list.  tau0 < 0 plans every sector of the update;
  * tau0, tau1 >= 0 plans that one sector only (the update_MPO drop-in).
  * Replaces the Aligner of src/mpo_sort.py:39-418 and the tau loop bounds of
- * src/beam_splitter_parallel/Rank.py:60-105.
+ * src/beam_splitter_parallel/Rank.py:60-105.  d <= 16 (the local Fock
+ * cutoff; the reference's contraction kernel has the same bound: 256-entry
+ * centre-charge tables, src/update_MPO_kernel.cu:236-238).
  */
 int bsmpo_plan_query(int d, int chi, const int32_t *h_CL, const int32_t *h_CC,
                      const int32_t *h_CR, int tau0, int tau1, size_t *plan_bytes,

@@ -14,6 +14,12 @@ against fixed data on the GPU box, where the oracle is not re-derived.
                  and entropies per layer, final marginals, two pattern
                  probabilities, and one site update's input and output
                  (layer 4, mode 2) as gauge-invariant data
+  m32_sp8.npz    M=32, single photons in modes 0,4,...,28, eta=0.7, d=9,
+                 chi=256, Haar seed 1 Clements mesh (496 beam splitters):
+                 trace and entropies per layer, final marginals and three
+                 pattern probabilities (takes ~15 min of CPU)
+
+`python tests/golden/make_golden.py m32` writes only m32_sp8.npz.
 """
 import os
 import sys
@@ -66,7 +72,32 @@ def m8_sp4():
         upd_out_pair=after.Gamma[mode] @ after.Gamma[mode + 1])
 
 
+def m32_sp8():
+    M, d, chi, inputs, eta = 32, 9, 256, list(range(0, 32, 4)), 0.7
+    mm, _ = mesh.haar_clements_mesh(M, 1)
+    st = mpo.initialize(M, d, chi, inputs, eta)
+    prob, EE = [mpo.trace(st)], [mpo.entanglement_entropy(st.Lambda)]
+    for layer in range(M):
+        mpo.simulate(st, mm, inputs, layers=[layer])
+        prob.append(mpo.trace(st))
+        EE.append(mpo.entanglement_entropy(st.Lambda))
+        print(f"m32_sp8 layer {layer}: trace {prob[-1]:.12f}", flush=True)
+    patterns = np.zeros((3, M), np.int64)
+    patterns[0, inputs] = 1                    # every photon back in its input mode
+    patterns[1, [m + 1 for m in inputs]] = 1   # shifted by one mode
+    # patterns[2]: vacuum, P = (1 - eta)^8 before truncation
+    np.savez_compressed(
+        os.path.join(HERE, "m32_sp8.npz"),
+        mesh=mm, prob=np.array(prob), EE=np.stack(EE, 1), marginals=mpo.marginals(st),
+        patterns=patterns,
+        pattern_prob=np.array([mpo.pattern_probability(st, p) for p in patterns]))
+
+
 if __name__ == "__main__":
+    if sys.argv[1:] == ["m32"]:
+        m32_sp8()
+        sys.exit(0)
     bs_tables()
     m8_sp4()
+    m32_sp8()
     print("wrote", sorted(f for f in os.listdir(HERE) if f.endswith(".npz")))
