@@ -35,6 +35,7 @@ SIGNATURES = {
     "bsmpo_contract": (C.c_int, [_i, _i, _vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i, _vp,
                                  _sz, _vp]),
     "bsmpo_observables": (C.c_int, [_i, _i, _i, _dp, _dp, _i32p, _dp, _dp, _dp, _dp, _vp]),
+    "bsmpo_entanglement_entropy": (C.c_int, [_i, _i, _dp, _dp, _vp]),
     "bsmpo_group_query": (C.c_int, [_i, _vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "bsmpo_group_build": (C.c_int, [_i, _vp, _vp, _vp, _vp, _sz]),
     "bsmpo_group_run": (C.c_int, [_vp, _i, _vp, _sz, _vp]),

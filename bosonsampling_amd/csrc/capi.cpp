@@ -12,6 +12,7 @@ namespace bsmpo {
 int launch_observables(int M, int d, int chi, const double *Gamma, const double *Lambda,
                        const int32_t *charge, double *R, double *L, double *marg, double *trace,
                        hipStream_t stream);
+int launch_entropy(int M, int chi, const double *Lambda, double *S, hipStream_t stream);
 }
 
 namespace bsmpo {
@@ -254,6 +255,12 @@ int bsmpo_observables(int M, int d, int chi, const double *d_Gamma, const double
     return fail(-1, "observables: NULL pointer");
   return launch_observables(M, d, chi, d_Gamma, d_Lambda, d_charge, d_R, d_L, d_marg, d_trace,
                             (hipStream_t)stream);
+}
+
+int bsmpo_entanglement_entropy(int M, int chi, const double *d_Lambda, double *d_S,
+                               void *stream) {
+  if (!d_Lambda || !d_S) return fail(-1, "entanglement_entropy: NULL pointer");
+  return launch_entropy(M, chi, d_Lambda, d_S, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -128,6 +128,40 @@ __global__ __launch_bounds__(256) void k_marginals(int M, int d, int chi, const 
   }
 }
 
+// Entanglement entropy of every inner bond (src/summary.py:3-8,28-30):
+// S_b = -sum_x p_x log2 p_x with p = Lambda_b^2 / sum_x Lambda_b^2; terms with
+// p = 0 add nothing and an all-zero bond gives 0 (the reference's nansum over
+// 0 log 0 and 0 / 0 terms).  One block per bond.
+__global__ __launch_bounds__(256) void k_entropy(int chi, const double *Lambda, double *S) {
+  __shared__ double red[4];
+  const double *lam = Lambda + (int64_t)blockIdx.x * chi;
+  auto block_sum = [&](double v) {
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return red[0] + red[1] + red[2] + red[3];
+  };
+  double t = 0.0;
+  for (int x = threadIdx.x; x < chi; x += blockDim.x) t += lam[x] * lam[x];
+  const double tot = block_sum(t);
+  double h = 0.0;
+  if (tot > 0.0)
+    for (int x = threadIdx.x; x < chi; x += blockDim.x) {
+      const double p = lam[x] * lam[x] / tot;
+      if (p > 0.0) h -= p * log2(p);
+    }
+  const double s = block_sum(h);
+  if (threadIdx.x == 0) S[blockIdx.x] = s;
+}
+
+int launch_entropy(int M, int chi, const double *Lambda, double *S, hipStream_t stream) {
+  if (M < 2 || chi < 1) return fail(-1, "entanglement_entropy: bad sizes");
+  hipLaunchKernelGGL(k_entropy, dim3(M - 1), dim3(256), 0, stream, chi, Lambda, S);
+  BSMPO_HIP(hipGetLastError());
+  return 0;
+}
+
 int launch_observables(int M, int d, int chi, const double *Gamma, const double *Lambda,
                        const int32_t *charge, double *R, double *L, double *marg, double *trace,
                        hipStream_t stream) {

@@ -194,15 +194,13 @@ class DeviceMPO:
         return float(tr.item()), marg.cpu().numpy()
 
     def entanglement_entropy(self):
-        """src/summary.py:3-8: S = -sum p log2 p, p = Lambda^2 / sum Lambda^2."""
-        lam = self.Lambda.cpu().numpy()
-        sq = lam ** 2
-        tot = sq.sum(axis=1, keepdims=True)
-        with np.errstate(divide="ignore", invalid="ignore"):
-            p = sq / tot
-            out = -np.nansum(p * np.log2(p), axis=1)
-        out[tot[:, 0] <= 0] = np.nan
-        return out
+        """Entropy of every inner bond on the device (src/summary.py:3-8):
+        S = -sum p log2 p, p = Lambda^2 / sum Lambda^2; returns M-1 floats."""
+        S = torch.empty(self.M - 1, dtype=torch.float64, device=self.device)
+        check(lib().bsmpo_entanglement_entropy(self.M, self.chi, C.c_void_p(self.Lambda.data_ptr()),
+                                               C.c_void_p(S.data_ptr()), _stream()),
+              "entanglement_entropy")
+        return S.cpu().numpy()
 
 
 # ---------------------------------------------------------- grouped updates

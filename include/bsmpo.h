@@ -263,6 +263,15 @@ int bsmpo_observables(int M, int d, int chi, const double *d_Gamma, const double
                       const int32_t *d_charge, double *d_R, double *d_L, double *d_marg,
                       double *d_trace, void *stream);
 
+/*
+ * Entanglement entropy of every inner bond of an M-site MPO:
+ *   d_S[b] = -sum_x p_x log2 p_x,  p = Lambda[b]^2 / sum_x Lambda[b]^2,
+ * b = 0..M-2 (d_Lambda device (M-1) x chi float64, d_S device M-1 float64);
+ * an all-zero bond gives 0.  Replaces src/summary.py:3-8,28-30
+ * (EntanglementEntropy, EntropyFromColumn, ColumnSumToOne).
+ */
+int bsmpo_entanglement_entropy(int M, int chi, const double *d_Lambda, double *d_S, void *stream);
+
 /* ------------------------------------------------------ reverse mode -- */
 /*
  * The reference has no derivative code.  These entry points give the

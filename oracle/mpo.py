@@ -484,13 +484,14 @@ def pattern_probability(state: MPOState, pattern) -> float:
 
 
 def entanglement_entropy(Lambda: np.ndarray) -> np.ndarray:
-    """src/summary.py:3-8,28-30: S = -sum p log2 p, p = Lambda^2 / sum Lambda^2."""
+    """src/summary.py:3-8,28-30: S = -sum p log2 p, p = Lambda^2 / sum Lambda^2.
+    Zero entries add nothing (nansum over 0 log 0) and an all-zero bond gives
+    0 (the reference's nansum over the 0 / 0 column)."""
     out = np.zeros(Lambda.shape[0])
     sq = np.asarray(Lambda, np.float64) ** 2
     for i in range(Lambda.shape[0]):
         tot = sq[i].sum()
         if tot <= 0:
-            out[i] = np.nan
             continue
         p = sq[i] / tot
         with np.errstate(divide="ignore", invalid="ignore"):

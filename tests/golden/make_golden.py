@@ -19,7 +19,13 @@ against fixed data on the GPU box, where the oracle is not re-derived.
                  trace and entropies per layer, final marginals and three
                  pattern probabilities (takes ~15 min of CPU)
 
-`python tests/golden/make_golden.py m32` writes only m32_sp8.npz.
+  synth_d13.npz  one site update at d=13 on synthetic inputs (tests/synth.py,
+                 seed 5) at chi=1024 and chi=2048, whose largest sectors
+                 have n = 614 and n = 1228: the kept charges and singular
+                 values and X^H (Gamma0 Gamma1) Y for fixed random X, Y
+                 (gauge invariant; ~2 min of CPU)
+
+`python tests/golden/make_golden.py m32` (or `synth`) writes only that file.
 """
 import os
 import sys
@@ -93,11 +99,46 @@ def m32_sp8():
         pattern_prob=np.array([mpo.pattern_probability(st, p) for p in patterns]))
 
 
+SYNTH_CASES = {1024: ((4, 12), (2, 10), (0, 8)), 2048: ((4, 12), (2, 10), (0, 8))}
+SYNTH_SEED = 5
+
+
+def synth_projectors(chi, k=8):
+    rng = np.random.default_rng(99)
+    X = rng.standard_normal((chi, k)) + 1j * rng.standard_normal((chi, k))
+    Y = rng.standard_normal((chi, k)) + 1j * rng.standard_normal((chi, k))
+    return X, Y
+
+
+def synth_bs(d):
+    return fock.bs_table(d, mesh.haar_unitary(2, 3))
+
+
+def synth_d13():
+    sys.path.insert(0, os.path.dirname(HERE))
+    from synth import synthetic_update
+    d, out = 13, {}
+    for chi, (lq, cq, rq) in SYNTH_CASES.items():
+        CL, CC, CR, Glc, Gcr, LR = synthetic_update(d, chi, lq, cq, rq, SYNTH_SEED)
+        ch, lam, G0, G1, num = mpo.site_update(d, chi, synth_bs(d), LR, CL, CC, CR, Glc, Gcr)
+        X, Y = synth_projectors(chi)
+        out[f"charge_{chi}"] = ch
+        out[f"lambda_{chi}"] = lam
+        out[f"num_{chi}"] = num
+        out[f"proj_{chi}"] = (X.conj().T @ G0) @ (G1 @ Y)
+        print(f"synth_d13 chi={chi}: kept {num}", flush=True)
+    np.savez_compressed(os.path.join(HERE, "synth_d13.npz"), **out)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["m32"]:
         m32_sp8()
         sys.exit(0)
+    if sys.argv[1:] == ["synth"]:
+        synth_d13()
+        sys.exit(0)
     bs_tables()
     m8_sp4()
     m32_sp8()
+    synth_d13()
     print("wrote", sorted(f for f in os.listdir(HERE) if f.endswith(".npz")))

@@ -1,0 +1,195 @@
+"""GPU parity at the benchmark's sizes (d = 9 and 13, chi = 256 to 2048).
+
+- the plain complex128 path on the M=32, d=9, chi=256 workload against the
+  oracle's fixture (src/beam_splitter_parallel/Full.py:169-202 layer loop,
+  src/summary.py:11-26 trace), 1e-10 relative;
+- one site update at d=13, chi=1024 from a state of the M=64 workload, against
+  the oracle run on the same input (sectors with n > 512);
+- synthetic updates at d=13, chi=1024 and chi=2048 (sectors with n > 512 and
+  n > 1024) against the oracle's fixture;
+- grouped complex128 against sequential plain runs of two M=64, chi=1024
+  instances (src/beam_splitter_parallel/Full.py:149-166 dispatches one update
+  per rank; the grouped run must give the same state);
+- complex64 and split-BF16x3 grouped runs against plain complex128 at
+  M=64, chi=1024 and chi=2048.
+
+Integer outputs are compared bit-exactly; floating outputs on gauge-invariant
+quantities (singular values, two-site products, trace, marginals).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import fock, mpo
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TOL = 1e-10
+
+
+def _set_tie(rel):
+    from bosonsampling_amd._capi import check, lib
+    check(lib().bsmpo_set_tie_rel(rel), "set_tie_rel")
+
+
+@pytest.mark.timeout(600)
+def test_m32_plain_matches_oracle_fixture(gpu):
+    """M=32, photons in modes 0,4,..,28, eta=0.7, d=9, chi=256, Haar seed 1
+    (496 beam splitters): per-layer trace, final marginals, entropies and
+    pattern probabilities of the plain complex128 path."""
+    from bosonsampling_amd import engine
+    z = np.load(os.path.join(GOLDEN, "m32_sp8.npz"))
+    M, d, chi, inputs, eta = 32, 9, 256, list(range(0, 32, 4)), 0.7
+    print("m32 plain c128 run", flush=True)
+    st = engine.initialize(M, d, chi, inputs, eta, device=gpu)
+    prob, EE = engine.simulate(st, z["mesh"], inputs)
+    tr, marg = st.observables()
+    ref = z["prob"]
+    assert np.abs(prob - ref).max() <= TOL * np.abs(ref).max(), np.abs(prob - ref).max()
+    assert abs(tr - ref[-1]) <= TOL * abs(ref[-1])
+    assert np.abs(marg - z["marginals"]).max() <= TOL * abs(ref[-1])
+    assert np.abs(EE - z["EE"]).max() <= 1e-9
+    G, L, ch = st.to_host()
+    host = mpo.MPOState(d, chi, G, L, ch, np.ones(chi))
+    for pat, pref in zip(z["patterns"], z["pattern_prob"]):
+        p = mpo.pattern_probability(host, pat)
+        assert abs(p - pref) <= TOL * abs(ref[-1]), (pat, p, pref)
+
+
+def _bench_state(gpu, chi, layers, seed=100):
+    from bosonsampling_amd import engine, mesh
+    M, d, inputs = 64, 13, list(range(0, 60, 5))
+    mm, _ = mesh.haar_clements_mesh(M, seed)
+    st = engine.initialize(M, d, chi, inputs, 0.6, device=gpu)
+    engine.simulate(st, mm, inputs, layers=range(layers), observe=False)
+    return st, mm, inputs
+
+
+@pytest.mark.timeout(600)
+def test_bench_state_update_matches_oracle(gpu):
+    """One update of layer 16 of the M=64, d=13, chi=1024 workload (seed 100),
+    at the mode whose largest sector is largest (n > 512: the 16-row
+    back-transformation class), GPU against the oracle on the same input."""
+    from bosonsampling_amd import engine
+    M, d, chi, layer = 64, 13, 1024, 16
+    print("bench state to layer 16", flush=True)
+    st, mm, inputs = _bench_state(gpu, chi, layer)
+    todo = engine.layer_modes(M, layer, max(inputs))
+    best = None
+    for i, mode in todo:
+        sec = st.plan(mode).sectors()[0]
+        n = int(np.minimum(sec[:, 2], sec[:, 3]).max())
+        if best is None or n > best[0]:
+            best = (n, i, mode)
+    n, i, mode = best
+    assert n > 512, n
+    print(f"update mode {mode}: largest sector n = {n}", flush=True)
+    B = mm[layer, i]
+    G, L, ch = st.to_host()
+    host = mpo.MPOState(d, chi, G, L, ch, np.ones(chi))
+    st.update(mode, engine.bs_tables(d, B, gpu)[0])
+    G2, L2, ch2 = st.to_host()
+    mpo.apply_update(host, mode, fock.bs_table(d, B))
+    assert np.array_equal(ch2[mode + 1], host.charge[mode + 1])
+    lam = host.Lambda[mode]
+    assert np.abs(L2[mode] - lam).max() <= TOL * lam.max()
+    P_gpu = G2[mode] @ G2[mode + 1]
+    P_ref = host.Gamma[mode] @ host.Gamma[mode + 1]
+    assert np.abs(P_gpu - P_ref).max() <= TOL * np.abs(P_ref).max()
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("chi,nmin", [(1024, 512), (2048, 1024)])
+def test_synthetic_large_sectors_match_fixture(gpu, chi, nmin):
+    """Synthetic d=13 update (tests/synth.py, seed 5) whose largest sectors
+    exceed n = 512 / 1024 (back-transformation classes of 16 and 32 rows per
+    lane group, inverse iteration with its arrays in global memory): kept
+    charges bit-exact, singular values and X^H (Gamma0 Gamma1) Y to 1e-10."""
+    import torch
+    from bosonsampling_amd import engine
+    from golden.make_golden import SYNTH_CASES, SYNTH_SEED, synth_bs, synth_projectors
+    from synth import synthetic_update
+    z = np.load(os.path.join(GOLDEN, "synth_d13.npz"))
+    d = 13
+    CL, CC, CR, Glc, Gcr, LR = synthetic_update(d, chi, *SYNTH_CASES[chi], SYNTH_SEED)
+    # a 3-site MPO holding the update's inputs on sites 0 and 1
+    G = np.zeros((3, chi, chi), np.complex128)
+    G[0], G[1] = Glc, Gcr
+    Lam = np.zeros((2, chi))
+    Lam[1] = LR
+    charge = np.full((4, chi, 2), d, np.int32)
+    charge[0], charge[1], charge[2] = CL, CC, CR
+    st = engine.DeviceMPO.from_host(G, Lam, charge, d, gpu)
+    sec = st.plan(0).sectors()[0]
+    n = int(np.minimum(sec[:, 2], sec[:, 3]).max())
+    print(f"synthetic chi={chi}: largest sector n = {n}", flush=True)
+    assert n > nmin
+    U3 = torch.from_numpy(synth_bs(d)).to(gpu)
+    st.update(0, U3)
+    G2, L2, ch2 = st.to_host()
+    assert np.array_equal(ch2[1], z[f"charge_{chi}"])
+    lam = z[f"lambda_{chi}"]
+    assert np.abs(L2[0] - lam).max() <= TOL * lam.max()
+    X, Y = synth_projectors(chi)
+    proj = (X.conj().T @ G2[0]) @ (G2[1] @ Y)
+    ref = z[f"proj_{chi}"]
+    assert np.abs(proj - ref).max() <= TOL * np.abs(ref).max()
+
+
+@pytest.mark.timeout(600)
+def test_grouped_equals_sequential_m64(gpu):
+    """Two instances of the M=64, d=13, chi=1024 workload (seeds 100, 101)
+    through layer 17: grouped complex128 (one launch per stage for every
+    update of a layer of both instances) against each instance run alone."""
+    from bosonsampling_amd import engine, mesh
+    M, d, chi, inputs = 64, 13, 1024, list(range(0, 60, 5))
+    meshes = [mesh.haar_clements_mesh(M, s)[0] for s in (100, 101)]
+    init = engine.initialize(M, d, chi, inputs, 0.6, device=gpu)
+    seq = [init.clone() for _ in meshes]
+    for st, mm in zip(seq, meshes):
+        print("sequential instance", flush=True)
+        engine.simulate(st, mm, inputs, layers=range(18), observe=False)
+    grp = [init.clone() for _ in meshes]
+    print("grouped instances", flush=True)
+    engine.simulate_batch(grp, meshes, inputs, layers=range(18), precision=engine.C128)
+    for a, b in zip(seq, grp):
+        ta, ma = a.observables()
+        tb, mb = b.observables()
+        assert np.array_equal(a.charge.cpu().numpy(), b.charge.cpu().numpy())
+        assert abs(ta - tb) <= 1e-12 * abs(ta)
+        assert np.abs(ma - mb).max() <= 1e-12 * abs(ta)
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("chi,layers", [(1024, 18), (2048, 14)])
+def test_low_precision_against_c128_m64(gpu, chi, layers):
+    """M=64, d=13, seed 100: the grouped complex64 and split-BF16x3
+    contractions against the plain complex128 run after the same layers.
+    Output photon-number marginals and trace: complex64 within 1e-5 (the
+    target); BF16x3 within 1e-3 (measured ~1e-5).  The tie band of the
+    truncation is set above each contraction's rounding noise (1e-6 for
+    complex64, 1e-4 for BF16x3) on both sides of the comparison."""
+    from bosonsampling_amd import engine, mesh
+    M, d, inputs = 64, 13, list(range(0, 60, 5))
+    mm, _ = mesh.haar_clements_mesh(M, 100)
+    init = engine.initialize(M, d, chi, inputs, 0.6, device=gpu)
+    errs = {}
+    try:
+        for prec, band, bound in ((engine.C64, 1e-6, 1e-5), (engine.BF16X3, 1e-4, 1e-3)):
+            _set_tie(band)
+            ref = init.clone()
+            print(f"chi={chi} plain c128 (band {band})", flush=True)
+            engine.simulate(ref, mm, inputs, layers=range(layers), observe=False)
+            low = init.clone()
+            print(f"chi={chi} grouped precision {prec}", flush=True)
+            engine.simulate_batch([low], [mm], inputs, layers=range(layers), precision=prec)
+            tr, mr = ref.observables()
+            tl, ml = low.observables()
+            errs[prec] = (float(np.abs(ml - mr).max()), abs(tl - tr))
+            print(f"chi={chi} precision {prec}: marginal error {errs[prec][0]:.2e}, "
+                  f"trace error {errs[prec][1]:.2e}, trace {tr:.6f}", flush=True)
+            assert errs[prec][0] <= bound and errs[prec][1] <= bound
+    finally:
+        _set_tie(1e-6)

@@ -218,3 +218,22 @@ def test_tridiagonal_route_is_scale_invariant(gpu):
     P1 = G1[mode] @ G1[mode + 1]
     Pc = Gc[mode] @ Gc[mode + 1]
     assert np.abs(Pc / c ** 2 - P1).max() <= 1e-10 * np.abs(P1).max()
+
+
+def test_entanglement_entropy_on_device(gpu):
+    """bsmpo_entanglement_entropy against the oracle (pinned by closed forms
+    in test_oracle.py) on random bonds, a one-value bond and an all-zero bond."""
+    import torch
+    from bosonsampling_amd import engine
+    rng = np.random.default_rng(12)
+    M, chi, d = 6, 300, 5
+    lam = rng.random((M - 1, chi)) * (rng.random((M - 1, chi)) < 0.6)
+    lam[1] = 0.0
+    lam[1, 17] = 2.5
+    lam[3] = 0.0
+    st = engine.DeviceMPO(M, d, chi, gpu)
+    st.Lambda.copy_(torch.from_numpy(lam))
+    S = st.entanglement_entropy()
+    ref = mpo.entanglement_entropy(lam)
+    assert np.abs(S - ref).max() <= 1e-12
+    assert S[1] == 0.0 and S[3] == 0.0

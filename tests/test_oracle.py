@@ -230,3 +230,37 @@ def test_select_top_ties_and_order():
     assert kept == [(0, 0), (2, 0)]
     kept = mpo.select_top([np.array([3.0]), np.array([5.0])], 10)
     assert kept == [(0, 0), (1, 0)]
+
+
+# --------------------------------------------------------------- entropy --
+# src/summary.py:3-8,28-30.  Closed forms pin the base (log2), the squaring of
+# Lambda and the normalisation: a restated formula with ln, with Lambda in
+# place of Lambda^2 or without the column normalisation fails one of them.
+
+def test_entropy_equal_values_is_log2_k():
+    for k in (1, 2, 3, 7, 64):
+        lam = np.zeros((1, 80))
+        lam[0, :k] = 0.37
+        assert abs(mpo.entanglement_entropy(lam)[0] - np.log2(k)) < 1e-13
+
+
+def test_entropy_two_values_is_binary_entropy():
+    a, b = 0.9, 0.2
+    p = a * a / (a * a + b * b)        # weights are Lambda^2, normalised
+    h = -(p * np.log2(p) + (1 - p) * np.log2(1 - p))
+    lam = np.array([[a, 0.0, b, 0.0]])
+    assert abs(mpo.entanglement_entropy(lam)[0] - h) < 1e-13
+    # Lambda (not squared) would give a different value
+    q = a / (a + b)
+    assert abs(h + (q * np.log2(q) + (1 - q) * np.log2(1 - q))) > 1e-2
+
+
+def test_entropy_single_value_scale_and_zero_bond():
+    lam = np.array([[0.0, 3.0, 0.0], [0.0, 0.0, 0.0]])
+    S = mpo.entanglement_entropy(lam)
+    assert S[0] == 0.0
+    assert S[1] == 0.0                  # nansum over 0/0 in the reference
+    rng = np.random.default_rng(4)
+    lam = rng.random((3, 50))
+    assert np.allclose(mpo.entanglement_entropy(7.5 * lam), mpo.entanglement_entropy(lam),
+                       rtol=0, atol=1e-13)
