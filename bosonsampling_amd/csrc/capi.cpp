@@ -75,8 +75,48 @@ int run_group(const void *blob, int precision, void *d_work, size_t work_bytes, 
   return launch_truncate(*g, w, st);
 }
 
-// single-update entry points run as a group of one (host image reused per thread)
-thread_local std::vector<char> t_blob;
+// Single-update entry points run as a group of one.  The host image goes up
+// with an asynchronous copy, so an image buffer is reused only after the copy
+// that read it has completed: each thread keeps two pinned buffers, each with
+// the event recorded after its last upload.
+struct ImageSlot {
+  char *buf = nullptr;
+  int64_t cap = 0;
+  hipEvent_t done = nullptr;
+  bool pending = false;
+};
+struct ThreadImages {
+  ImageSlot slot[2];
+  int turn = 0;
+  ~ThreadImages() {
+    for (ImageSlot &s : slot) {
+      if (s.pending) hipEventSynchronize(s.done);
+      if (s.done) hipEventDestroy(s.done);
+      if (s.buf) hipHostFree(s.buf);
+    }
+  }
+};
+thread_local ThreadImages t_images;
+
+int image_for(int64_t bytes, ImageSlot **out) {
+  ImageSlot &s = t_images.slot[t_images.turn];
+  t_images.turn ^= 1;
+  if (s.pending) {
+    BSMPO_HIP(hipEventSynchronize(s.done));  // the last upload from this buffer has run
+    s.pending = false;
+  }
+  if (!s.done) BSMPO_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+  if (s.cap < bytes) {
+    if (s.buf) BSMPO_HIP(hipHostFree(s.buf));
+    s.buf = nullptr;
+    s.cap = 0;
+    const int64_t cap = bytes + bytes / 4;
+    BSMPO_HIP(hipHostMalloc((void **)&s.buf, (size_t)cap, hipHostMallocDefault));
+    s.cap = cap;
+  }
+  *out = &s;
+  return 0;
+}
 
 int single(const void *h_plan, const UpdateIO &io, void *d_work, size_t work_bytes, int precision,
            hipStream_t st, bool contract_only) {
@@ -85,10 +125,15 @@ int single(const void *h_plan, const UpdateIO &io, void *d_work, size_t work_byt
   int rc = group_query(1, plans, &hb, &wb);
   if (rc) return rc;
   if ((int64_t)work_bytes < wb) return fail(-1, "workspace too small for this plan");
-  if ((int64_t)t_blob.size() < hb) t_blob.resize((size_t)hb);
-  if ((rc = group_build(1, plans, &io, reinterpret_cast<char *>(d_work), t_blob.data(), hb)))
+  ImageSlot *img = nullptr;
+  if ((rc = image_for(hb, &img))) return rc;
+  if ((rc = group_build(1, plans, &io, reinterpret_cast<char *>(d_work), img->buf, hb,
+                        contract_only)))
     return rc;
-  return run_group(t_blob.data(), precision, d_work, work_bytes, st, contract_only);
+  rc = run_group(img->buf, precision, d_work, work_bytes, st, contract_only);
+  // the upload was enqueued first in run_group: the event marks its completion
+  if (hipEventRecord(img->done, st) == hipSuccess) img->pending = true;
+  return rc;
 }
 
 }  // namespace

@@ -13,10 +13,15 @@ namespace {
 
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
+// The plan copies follow the work lists at o_plans, the per-update scratch
+// areas follow the staging region at o_ws, each in update order and aligned
+// like the other regions (no per-update offset arrays: building a group
+// allocates nothing).
 struct Layout {
   GroupHeader g;
-  std::vector<int64_t> plan_off, ws_off;
 };
+
+inline int64_t region(int64_t bytes) { return align_up(std::max<int64_t>(bytes, 8), 256); }
 
 int layout(int n, const void *const *plans, Layout &L) {
   if (n <= 0) return fail(-1, "group: no updates");
@@ -83,13 +88,13 @@ int layout(int n, const void *const *plans, Layout &L) {
   g.o_sec = place(8 * n_sec);
   g.o_v = place(8 * n_v);
   g.o_w = place(8 * n_w);
-  L.plan_off.resize(n);
-  for (int u = 0; u < n; ++u) L.plan_off[u] = place(header(plans[u])->dev_bytes);
+  g.o_plans = o;
+  for (int u = 0; u < n; ++u) o += region(header(plans[u])->dev_bytes);
   g.staging_bytes = o;
-  L.ws_off.resize(n);
+  g.o_ws = o;
   for (int u = 0; u < n; ++u) {
     const PlanHeader *h = header(plans[u]);
-    L.ws_off[u] = place(h->w_total - h->w_C);
+    o += region(h->w_total - h->w_C);
   }
   g.work_bytes = o;
   g.host_bytes = kGroupHostHead + g.staging_bytes;
@@ -108,7 +113,7 @@ int group_query(int n, const void *const *plans, int64_t *host_bytes, int64_t *w
 }
 
 int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_work,
-                char *host_blob, int64_t host_bytes) {
+                char *host_blob, int64_t host_bytes, bool contract_only) {
   Layout L;
   int rc = layout(n, plans, L);
   if (rc) return rc;
@@ -127,12 +132,15 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
   int2 *vl = reinterpret_cast<int2 *>(img + g.o_v);
   int2 *wl = reinterpret_cast<int2 *>(img + g.o_w);
   int64_t ng = 0, nt = 0, ns = 0, nv = 0, nw = 0;
+  int64_t plan_at = g.o_plans, ws_at = g.o_ws;
   for (int u = 0; u < n; ++u) {
     const PlanHeader *h = header(plans[u]);
-    std::memcpy(img + L.plan_off[u], reinterpret_cast<const char *>(plans[u]) + h->dev_begin,
+    std::memcpy(img + plan_at, reinterpret_cast<const char *>(plans[u]) + h->dev_begin,
                 (size_t)h->dev_bytes);
-    char *pb = d_work + L.plan_off[u];           // device address of the plan copy
-    char *wb = d_work + L.ws_off[u] - h->w_C;     // so that wb + h->w_X is the scratch X
+    char *pb = d_work + plan_at;           // device address of the plan copy
+    char *wb = d_work + ws_at - h->w_C;    // so that wb + h->w_X is the scratch X
+    plan_at += region(h->dev_bytes);
+    ws_at += region(h->w_total - h->w_C);
     UpdDesc &D = desc[u];
     D.sectors = reinterpret_cast<const Sector *>(pb + h->o_sectors);
     D.rows = reinterpret_cast<const int32_t *>(pb + h->o_rows);
@@ -187,12 +195,19 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
     D.n_rblk = h->n_rblk;
     for (int c = 0; c < kTileClasses; ++c)
       for (int i = h->tile_off[c]; i < h->tile_off[c + 1]; ++i) tiles[tile_fill[c]++] = make_int2(u, i);
+    if (contract_only) continue;  // the SVD-stage lists stay empty (zero)
     for (int i = 0; i < h->n_gram_items; ++i) gram[ng++] = make_int2(u, i);
     for (int i = 0; i < h->n_v_items; ++i) vl[nv++] = make_int2(u, i);
     for (int i = 0; i < h->n_w_items; ++i) wl[nw++] = make_int2(u, i);
     for (int i = 0; i < h->n_sectors; ++i) sec[ns++] = make_int2(u, i);
     const int32_t *jo = host_array<int32_t>(plans[u], h->o_jac_order);
     for (int i = h->jac_off[kTriClass]; i < h->jac_off[kTriClass + 1]; ++i) tri[nt++] = make_int2(u, jo[i]);
+  }
+  if (contract_only) {  // a group that only runs the contraction stage
+    g.n_gram = g.n_tri = g.n_sec = g.n_v = g.n_w = 0;
+    for (int c = 0; c <= kTriClass; ++c) g.jac_off[c] = 0;
+    std::memcpy(host_blob, &g, sizeof(GroupHeader));
+    return 0;
   }
   // Tridiagonal-route sectors largest first across the group: blocks are
   // dispatched in list order, so the longest column chains start at once and

@@ -142,7 +142,7 @@ struct GroupHeader {
   int32_t jac_off[kTriClass + 1];
   int32_t jac_max_n[kTriClass];
   int32_t tri_max_n, pad1;
-  int64_t o_desc, o_gram, o_jac, o_tri, o_sec, o_v, o_w;
+  int64_t o_desc, o_gram, o_jac, o_tri, o_sec, o_v, o_w, o_plans, o_ws;
   int64_t staging_bytes, work_bytes, max_sv, host_bytes;
   double flops_alg, flops_gram, flops_jacobi, flops_wmat, flops_vectors, bytes_alg16;
   double c_elems, g_elems, v_elems, w_elems, sv_total;
@@ -202,8 +202,10 @@ struct UpdateIO {
   int edge;
 };
 int group_query(int n, const void *const *plans, int64_t *host_bytes, int64_t *work_bytes);
+// contract_only: build only what the contraction stage reads (the SVD-stage
+// work lists stay empty); no heap allocation in either case
 int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_work,
-                char *host_blob, int64_t host_bytes);
+                char *host_blob, int64_t host_bytes, bool contract_only = false);
 inline const GroupHeader *group_header(const void *blob) {
   return reinterpret_cast<const GroupHeader *>(blob);
 }
