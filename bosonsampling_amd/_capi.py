@@ -34,6 +34,8 @@ SIGNATURES = {
                                     _vp, _sz, _vp]),
     "bsmpo_contract": (C.c_int, [_i, _i, _vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i, _vp,
                                  _sz, _vp]),
+    "bsmpo_contract_noalloc": (C.c_int, [_i, _i, _vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i,
+                                         _vp]),
     "bsmpo_observables": (C.c_int, [_i, _i, _i, _dp, _dp, _i32p, _dp, _dp, _dp, _dp, _vp]),
     "bsmpo_entanglement_entropy": (C.c_int, [_i, _i, _dp, _dp, _vp]),
     "bsmpo_group_query": (C.c_int, [_i, _vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),

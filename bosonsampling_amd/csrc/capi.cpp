@@ -136,6 +136,32 @@ int single(const void *h_plan, const UpdateIO &io, void *d_work, size_t work_byt
   return rc;
 }
 
+// ------------------------------------------------ allocation-free variant --
+// bsmpo_contract_noalloc keeps its staging (update descriptor, work lists and
+// the plan's device part) in this file-scope array instead of a caller
+// workspace.  Host code never passes `scratch` itself to a kernel or copy:
+// each device's own copy is reached through the address hipGetSymbolAddress
+// returns for it, looked up once per device and kept in g_scratch_ptr.
+constexpr int64_t kNoAllocBytes = 8 << 20;  // plans up to chi = 2048 at d = 13 fit
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) char scratch[kNoAllocBytes];
+std::mutex g_noalloc_mu;                   // guards the two arrays below
+void *g_scratch_ptr[kMaxDevices];          // device address of `scratch`, per device
+alignas(256) char g_noalloc_image[kGroupHostHead + kNoAllocBytes];  // host image
+
+int scratch_address(void **out) {
+  int dev = 0;
+  BSMPO_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(-1, "contract_noalloc: device index out of range");
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    BSMPO_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -197,6 +223,48 @@ int bsmpo_contract(int d, int chi, const void *h_plan, const double *d_Glc, cons
   io.C_override = d_C;
   io.T_override = d_T;
   return single(h_plan, io, d_work, work_bytes, precision, (hipStream_t)stream, true);
+}
+
+int bsmpo_contract_noalloc(int d, int chi, const void *h_plan, const double *d_Glc,
+                           const double *d_Gcr, const double *d_LR, const double *d_U3,
+                           double *d_C, double *d_T, const double *d_LL, const double *d_LC,
+                           int precision, void *stream) {
+  int rc = validate_plan(h_plan, d, chi);
+  if (rc) return rc;
+  if (!d_Glc || !d_Gcr || !d_LR || !d_U3 || !d_C || !d_T)
+    return fail(-1, "contract_noalloc: NULL pointer");
+  if (precision != BSMPO_C128 && precision != BSMPO_C64 && precision != BSMPO_BF16X3)
+    return fail(-1, "contract_noalloc: unknown precision");
+  if (timing_on()) return fail(-1, "contract_noalloc: disable stage timing first (it creates events)");
+  const void *plans[1] = {h_plan};
+  int64_t hb = 0, wb = 0;
+  if ((rc = group_query(1, plans, &hb, &wb))) return rc;
+  if (hb - kGroupHostHead > kNoAllocBytes)
+    return fail(-1, "contract_noalloc: this plan needs " + std::to_string(hb - kGroupHostHead) +
+                        " bytes of staging, more than the library's 8 MiB scratch (use "
+                        "bsmpo_contract with a workspace)");
+  hipStream_t st = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  void *dev_scratch = nullptr;
+  if ((rc = scratch_address(&dev_scratch))) return rc;
+  UpdateIO io{};
+  io.Glc = const_cast<double *>(d_Glc);
+  io.Gcr = const_cast<double *>(d_Gcr);
+  io.LR = d_LR;
+  io.U3 = d_U3;
+  io.LL = d_LL;
+  io.LC = d_LC;
+  io.C_override = d_C;
+  io.T_override = d_T;
+  if ((rc = group_build(1, plans, &io, reinterpret_cast<char *>(dev_scratch), g_noalloc_image, hb,
+                        true)))
+    return rc;
+  const GroupHeader *g = group_header(g_noalloc_image);
+  BSMPO_HIP(hipMemcpyAsync(dev_scratch, g_noalloc_image + kGroupHostHead, (size_t)g->staging_bytes,
+                           hipMemcpyHostToDevice, st));
+  // the host image is shared by every call: it may be rewritten once the copy has read it
+  BSMPO_HIP(hipStreamSynchronize(st));
+  return launch_contract(*g, reinterpret_cast<const char *>(dev_scratch), precision, st);
 }
 
 int bsmpo_site_update(int d, int chi, const void *h_plan, double *d_Glc, double *d_Gcr,

@@ -190,6 +190,27 @@ int bsmpo_contract(int d, int chi, const void *h_plan, const double *d_Glc,
                    double *d_C, double *d_T, const double *d_LL, const double *d_LC,
                    int precision, void *d_work, size_t work_bytes, void *stream);
 
+/*
+ * bsmpo_contract without a workspace: the same contraction and outputs, with
+ * the staging it needs (update descriptor, work lists and the plan's device
+ * part) kept in an 8 MiB array the library holds on each device it runs on
+ * (until the process ends).  It calls no allocating routine (no hipMalloc,
+ * hipFree, pinned or pageable heap allocation on the success path), which
+ * suits callers that cannot allocate: tight serving loops and harnesses that
+ * forbid allocation.  Plans whose staging exceeds 8 MiB are rejected (< 0,
+ * with the size in bsmpo_last_error); chi <= 2048 at d = 13 fits.  The call
+ * synchronises `stream` once, after uploading the plan, because the host
+ * image it uploads from is shared by all calls.  Calls that overlap on one
+ * device race on its array: issue them one at a time (the kernels of one
+ * call must finish before the next call's upload overwrites the staging,
+ * which the synchronisation gives on a single stream).  Stage timing must be
+ * off.  No counterpart in the reference.
+ */
+int bsmpo_contract_noalloc(int d, int chi, const void *h_plan, const double *d_Glc,
+                           const double *d_Gcr, const double *d_LR, const double *d_U3,
+                           double *d_C, double *d_T, const double *d_LL, const double *d_LC,
+                           int precision, void *stream);
+
 /* Copies the device offsets of the packed per-sector buffers into h_out:
  * for sector s: h_out[4 s + 0..3] = tau0, tau1, L, R, and the packed C/T
  * element offset is returned in h_coff[s].  Host only. */
