@@ -5,21 +5,25 @@ Workload: lossy boson-sampling MPO, M=64 modes, N=12 single photons in modes
 0,5,...,55, d=13, chi=1024, eta=0.6; Haar unitaries (seeds 100, 101, ...)
 as Clements meshes of 2016 beam splitters each.
 
-A "step" is one brick-wall layer.  The first --warmup layers run untimed
-(they bring the bond dimension up to chi); the next --steps layers are
-timed, for two paths on the same inputs:
+A "step" is one brick-wall layer.  The timed layers start at layer
+max(--start, --warmup) (default 16: by then the bond dimension has reached
+chi on the inner bonds); the --warmup layers before them run untimed, and so
+do the layers before those (state preparation).  --steps layers are timed,
+for two paths on the same inputs:
   plain    one instance, complex128 contraction, every update launched on its
            own (the reference's one-update-at-a-time structure)
   grouped  --instances independent instances; every update of a layer across
            all of them in one grouped run (one launch per stage), contraction
-           in --precision (default complex64 on FP32 MFMA), SVD / truncation /
-           BS elements / norm in FP64
+           in --precision (default complex64 on the FP32 MFMA), SVD /
+           truncation / BS elements / norm in FP64
 `value` is the grouped path's BS updates/s over its timed layers; the plain
-path's rate and the ratio are in `config`.
+path's rate and the ratio are in `config`, with the mean kept bond dimension
+at the start and end of the timed layers.
 
 Also on the JSON line:
   roofline      the grouped run's dominant kernel: algorithmic work per launch
-                / its average HIP-event time over the timed region
+                / its average HIP-event time over the timed region; `stages`
+                holds the same figures for every stage
   cpu_baseline  the FP64 CPU oracle timed on a bounded sample of the same
                 updates (first updates of the first timed layer)
 """
@@ -37,15 +41,17 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 METRIC = "BS updates/s + % MFMA/HBM peak, M=64 χ=1024: FP32-grouped vs plain FP64"
-PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}   # MI355X dense
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}   # MI355X dense (MFMA; FP64 vector too)
 PEAK_HBM_GBS = 8000.0
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2, help="timed layers")
-    ap.add_argument("--warmup", type=int, default=16, help="untimed layers before the timed ones")
+    ap.add_argument("--steps", type=int, default=4, help="timed layers")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed layers just before the timed ones")
+    ap.add_argument("--start", type=int, default=16,
+                    help="first timed layer (at least --warmup); earlier layers prepare the state")
     ap.add_argument("--modes", type=int, default=64)
     ap.add_argument("--photons", type=int, default=12)
     ap.add_argument("--stride", type=int, default=5)
@@ -84,44 +90,70 @@ def stage_table(stages):
                 f"{fl / max(ms, 1e-9) / 1e9:8.3f} TFLOP/s {by / max(ms, 1e-9) / 1e6:9.2f} GB/s")
 
 
-HBM_BOUND = ("topk", "scatter", "observables", "bs_table", "tridiag")
+# Roof of each stage: ("mfma" | "valu" | "hbm", peak key); None = latency-bound
+# chains with no roof (Jacobi sweeps, bisection, inverse iteration).
+STAGE_ROOF = {"contract": ("mfma", None), "gram": ("mfma", "f64"), "wmat": ("mfma", "f64"),
+              "vectors": ("mfma", "f64"), "backtransform": ("valu", "f64"),
+              "tridiag": ("hbm", None), "topk": ("hbm", None), "scatter": ("hbm", None),
+              "observables": ("hbm", None), "bs_table": ("hbm", None),
+              "jacobi": None, "bisect": None, "trivec": None}
 
 
-def measured_traffic(kernel, workload, instances, precision, eig_threshold, launches):
-    """HBM bytes per launch of `kernel` measured on this workload by two
-    rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE, scaled by a probe kernel
-    of known size) and recorded in profiles/<kernel>_*_traffic.json; None if
-    no record matches (the same grouping: a per-launch figure holds only for
-    the same number of launches)."""
+def stage_entry(name, ms, fl, by, cnt, precision):
+    """Algorithmic work of one stage over the timed region against its roof."""
+    e = {"ms": round(ms, 3), "launches": cnt, "avg_launch_ms": round(ms / cnt, 4)}
+    roof = STAGE_ROOF.get(name)
+    t = ms / 1e3
+    if roof is None or t <= 0:
+        e["bound"] = "latency"
+        return e
+    kind, key = roof
+    if kind == "hbm":
+        ach = by / t / 1e9
+        e.update({"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                  "frac": ach / PEAK_HBM_GBS, "work_per_launch": by / cnt})
+    else:
+        if key is None:  # the contraction: the precision's MFMA peak
+            key = {"c128": "f64", "c64": "f32", "bf16x3": "bf16"}[precision]
+        ach = fl / t / 1e12
+        peak = PEAK_TFLOPS[key]
+        e.update({"bound": kind, "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+                  "frac": ach / peak, "work_per_launch": fl / cnt})
+    return e
+
+
+def roofline(stages, precision):
+    table = {s: stage_entry(s, *v, precision) for s, v in stages.items() if v[3] > 0}
+    dom = max(table, key=lambda s: table[s]["ms"])
+    d = table[dom]
+    roof = {k: d.get(k) for k in ("bound", "achieved", "peak", "unit", "frac")}
+    roof.update({"traffic": None, "kernel": dom, "launches": d["launches"],
+                 "avg_launch_ms": d["avg_launch_ms"], "stages": table})
+    return roof
+
+
+def measured_traffic(kernel, shape, precision, eig_threshold, alg_bytes_per_launch):
+    """HBM bytes per launch of `kernel`: the ratio of measured to algorithmic
+    bytes recorded in profiles/<kernel>_*_traffic.json for this workload shape,
+    precision and eigensolver split (two rocprofv3 --pmc passes, FETCH_SIZE and
+    WRITE_SIZE, scaled by a probe kernel of known size), times this run's
+    algorithmic bytes per launch; None if no record matches."""
     import glob
     for path in sorted(glob.glob(os.path.join(HERE, "profiles", f"{kernel}_*_traffic.json"))):
         with open(path) as f:
             rec = json.load(f)
-        if (rec.get("kernel"), rec.get("workload"), rec.get("instances"), rec.get("precision"),
-                rec.get("eig_threshold"), rec.get("launches")) == (kernel, workload, instances,
-                                                                   precision, eig_threshold, launches):
-            return rec["traffic_bytes_per_launch"], os.path.relpath(path, HERE)
+        if (rec.get("kernel"), rec.get("shape"), rec.get("precision"),
+                rec.get("eig_threshold")) == (kernel, shape, precision, eig_threshold):
+            return rec["traffic_over_algorithmic"] * alg_bytes_per_launch, os.path.relpath(path, HERE)
     return None, None
 
 
-def roofline(stages, precision):
-    dom = max((s for s in stages if stages[s][3] > 0), key=lambda s: stages[s][0])
-    ms, fl, by, cnt = stages[dom]
-    t = ms / cnt / 1e3
-    if dom in HBM_BOUND or fl == 0.0:
-        ach = by / cnt / t / 1e9 if by else None
-        roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": (ach / PEAK_HBM_GBS) if ach else None, "traffic": None}
-    else:
-        peak = PEAK_TFLOPS["f64"]
-        if dom == "contract" and precision != "c128":
-            peak = PEAK_TFLOPS["f32"] if precision == "c64" else PEAK_TFLOPS["bf16"]
-        ach = fl / cnt / t / 1e12
-        roof = {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
-                "frac": ach / peak, "traffic": None}
-    roof.update({"kernel": dom, "launches": cnt, "avg_launch_ms": ms / cnt,
-                 "stage_ms": {s: round(v[0], 3) for s, v in stages.items() if v[3]}})
-    return roof
+def kept_bond_dimension(st):
+    """Mean number of kept singular values (Lambda > 0) over the inner bonds
+    that hold any."""
+    cnt = (st.Lambda > 0).sum(dim=1).cpu().numpy()
+    cnt = cnt[cnt > 0]
+    return float(cnt.mean()) if cnt.size else 0.0
 
 
 def main():
@@ -148,17 +180,20 @@ def main():
     M, N, d, chi = args.modes, args.photons, args.d, args.chi
     inputs = list(range(0, N * args.stride, args.stride))
     W, K = args.warmup, args.steps
-    if W + K > M:
-        raise SystemExit("warmup + steps exceeds the number of layers")
+    T0 = max(args.start, W)          # first timed layer
+    if T0 + K > M:
+        raise SystemExit("the timed layers run past the last layer of the mesh")
     paths = args.paths.split(",")
     B = args.instances
     meshes = [mesh.haar_clements_mesh(M, args.seed + b)[0] for b in range(B)]
     t0 = time.time()
     init = engine.initialize(M, d, chi, inputs, args.eta)
     torch.cuda.synchronize()
-    log(f"[bench] init {time.time() - t0:.2f}s")
+    log(f"[bench] init {time.time() - t0:.2f}s; untimed layers 0..{T0 - 1} "
+        f"(warmup {T0 - W}..{T0 - 1}), timed {T0}..{T0 + K - 1}")
     max_input = max(inputs)
     result = {}
+    shape = f"M={M} N={N} stride={args.stride} d={d} chi={chi} eta={args.eta} instances={B}"
 
     # ---------------------------------------------------------------- plain
     snap = None
@@ -176,8 +211,8 @@ def main():
 
         t1 = time.time()
         plain_tuned = fixed_threshold(args)
-        for layer in range(W):
-            if args.autotune and args.eig_threshold < 0 and layer == W - 1:
+        for layer in range(T0):
+            if args.autotune and args.eig_threshold < 0 and layer == T0 - 1:
                 best, times = engine.autotune_eig_threshold([st], [meshes[0]], inputs, layer,
                                                             precision=engine.C128, grouped=False)
                 plain_tuned = {"layer": layer, "eig_threshold": best,
@@ -186,20 +221,22 @@ def main():
                     f"threshold {best} {times}")
             run_layer(layer, engine.C128)
         torch.cuda.synchronize()
-        log(f"[bench] plain warmup {W} layers {time.time() - t1:.1f}s")
+        log(f"[bench] plain untimed {T0} layers {time.time() - t1:.1f}s")
         if args.cpu_seconds > 0:
             snap = st.to_host()
+        bond0 = kept_bond_dimension(st)
         _capi.timing_enable(True)
         _capi.timing_read(reset=True)
         torch.cuda.synchronize()
         ts = time.time()
-        n = sum(run_layer(layer, engine.C128) for layer in range(W, W + K))
+        n = sum(run_layer(layer, engine.C128) for layer in range(T0, T0 + K))
         torch.cuda.synchronize()
         el = time.time() - ts
         st_plain = _capi.timing_read(reset=True)
         _capi.timing_enable(False)
         result["plain"] = {"updates": n, "seconds": el, "rate": n / el,
-                           "observables": st.observables(), "tuned": plain_tuned}
+                           "observables": st.observables(), "tuned": plain_tuned,
+                           "bond": [bond0, kept_bond_dimension(st)]}
         log(f"[bench] plain c128: {n} updates in {el:.3f}s = {n / el:.2f} updates/s")
         stage_table(st_plain)
         del st
@@ -213,9 +250,9 @@ def main():
                                     streams=args.streams, first_frac=args.first_frac)
         t1 = time.time()
         tuned = fixed_threshold(args)
-        for layer in range(W):
-            if args.autotune and args.eig_threshold < 0 and layer == W - 1:
-                # tune the eigensolver route split on the last warmup layer (clones)
+        for layer in range(T0):
+            if args.autotune and args.eig_threshold < 0 and layer == T0 - 1:
+                # tune the eigensolver route split on the last untimed layer (clones)
                 best, times = engine.autotune_eig_threshold(states, meshes, inputs, layer,
                                                             precision=prec, runner=runner)
                 tuned = {"layer": layer, "eig_threshold": best,
@@ -226,15 +263,16 @@ def main():
                                   precision=prec)
             torch.cuda.synchronize()
             if args.verbose:
-                log(f"[bench] grouped warmup layer {layer}: {time.time() - t1:.1f}s "
+                log(f"[bench] grouped untimed layer {layer}: {time.time() - t1:.1f}s "
                     f"groups {runner.last_info.get('groups')}")
-        log(f"[bench] grouped warmup {W} layers x {B} instances {time.time() - t1:.1f}s")
+        log(f"[bench] grouped untimed {T0} layers x {B} instances {time.time() - t1:.1f}s")
+        bond0 = kept_bond_dimension(states[0])
         _capi.timing_enable(True)
         _capi.timing_read(reset=True)
         torch.cuda.synchronize()
         ts = time.time()
         n = 0
-        for layer in range(W, W + K):
+        for layer in range(T0, T0 + K):
             k, _ = engine.simulate_batch(states, meshes, inputs, layers=[layer], runner=runner,
                                          precision=prec)
             n += k
@@ -244,7 +282,8 @@ def main():
         st_grp = _capi.timing_read(reset=True)
         _capi.timing_enable(False)
         result["grouped"] = {"updates": n, "seconds": el, "rate": n / el,
-                             "observables0": states[0].observables(), "tuned": tuned}
+                             "observables0": states[0].observables(), "tuned": tuned,
+                             "bond": [bond0, kept_bond_dimension(states[0])]}
         log(f"[bench] grouped {args.precision}: {n} updates in {el:.3f}s = {n / el:.2f} updates/s")
         stage_table(st_grp)
         roof = roofline(st_grp, args.precision)
@@ -253,16 +292,17 @@ def main():
             roof["note"] = (f"groups go round {args.streams} launching streams: a launch's duration "
                             "includes sharing the GPU with the other streams' stages")
 
-    cpu = cpu_baseline(snap, d, chi, meshes[0], inputs, W, args.cpu_seconds) if snap else None
+    cpu = cpu_baseline(snap, d, chi, meshes[0], inputs, T0, args.cpu_seconds) if snap else None
     workload = (f"lossy boson-sampling MPO site updates, M={M} N={N} "
                 f"(modes 0,{args.stride},..) d={d} chi={chi} eta={args.eta}; "
-                f"layers {W}..{W + K - 1} timed")
+                f"layers {T0}..{T0 + K - 1} timed")
     g = result.get("grouped")
     p = result.get("plain")
-    if roof is not None:
+    if roof is not None and roof["bound"] == "hbm":
         thr = (g or {}).get("tuned") or {}
-        roof["traffic"], src = measured_traffic(roof["kernel"], workload, B, args.precision,
-                                                thr.get("eig_threshold", 96), roof["launches"])
+        alg = roof["stages"][roof["kernel"]]["work_per_launch"]
+        roof["traffic"], src = measured_traffic(roof["kernel"], shape, args.precision,
+                                                thr.get("eig_threshold", 96), alg)
         if src:
             roof["traffic_source"] = src
     main_path = g or p
@@ -282,13 +322,17 @@ def main():
                 % (args.seed, args.seed + (B - 1 if g else 0)),
         "config": {
             "workload": workload,
+            "timed_layers": [T0, T0 + K - 1],
+            "kept_bond_dimension": {"chi": chi,
+                                    "grouped_start_end": g["bond"] if g else None,
+                                    "plain_start_end": p["bond"] if p else None},
             "grouped": ({"instances": B, "precision": args.precision, "updates": g["updates"],
                          "updates_per_s": g["rate"], "autotune": g.get("tuned"),
                          "launch_streams": args.streams} if g else None),
             "plain_fp64": ({"instances": 1, "updates": p["updates"],
                             "updates_per_s": p["rate"], "autotune": p.get("tuned")} if p else None),
             "grouped_over_plain": (g["rate"] / p["rate"]) if (g and p) else None,
-            "precision_check": precision_check(g, p, args.precision, W + K),
+            "precision_check": precision_check(g, p, args.precision, T0 + K),
         },
         "roofline": roof,
         "cpu_baseline": cpu,

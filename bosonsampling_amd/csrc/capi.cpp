@@ -191,6 +191,12 @@ int bsmpo_timing_read(double *ms, double *flops, double *bytes, int64_t *count, 
     hipEventDestroy(r.a);
     hipEventDestroy(r.b);
   }
+  // the kept-pair stages: flops counted on the device by the kept counts
+  double kept[3] = {0.0, 0.0, 0.0};
+  if (int rc = kept_flops_read(kept, true)) return rc;
+  g_flops[ST_WMAT] += kept[0];
+  g_flops[ST_VECTORS] += kept[1];
+  g_flops[ST_BACKTRANS] += kept[2];
   for (int i = 0; i < ST_COUNT; ++i) {
     if (ms) ms[i] = g_ms[i];
     if (flops) flops[i] = g_flops[i];

@@ -115,7 +115,7 @@ struct UpdDesc {
   int32_t *charge_c;
   double *lambda_c;
   int32_t *num_lambda;
-  int32_t d, chi, edge, n_sectors, n_cblk, pad0;
+  int32_t d, chi, edge, n_sectors, n_cblk, eig_thr;  // eig_thr: n above it = tridiagonal route
   int64_t sv_total;
   double tie_rel;
   // tile formulation of the contraction

@@ -614,7 +614,24 @@ __global__ __launch_bounds__(256) void k_bisect(const UpdDesc *__restrict__ Ud,
 }
 
 // per-sector count of kept singular values (a prefix of the sorted list)
-__global__ void k_kcount(const UpdDesc *__restrict__ Ud, const int2 *__restrict__ list) {
+// Algorithmic flops of the stages that work on kept pairs only, accumulated
+// while stage timing is on (complex MAC = 8 flops): W = V^H C, 8 k L R;
+// V = T Y / sigma (sectors with L > R), 8 k L R; back-transformation of the
+// tridiagonal route, 8 k n^2 (each reflector of length m costs 2 m MACs per
+// vector).  k = kept pairs of the sector.
+__device__ double g_kept_flops[3];
+
+int kept_flops_read(double *out, bool reset) {
+  BSMPO_HIP(hipMemcpyFromSymbol(out, HIP_SYMBOL(g_kept_flops), sizeof(double) * 3));
+  if (reset) {
+    const double z[3] = {0.0, 0.0, 0.0};
+    BSMPO_HIP(hipMemcpyToSymbol(HIP_SYMBOL(g_kept_flops), z, sizeof(z)));
+  }
+  return 0;
+}
+
+__global__ void k_kcount(const UpdDesc *__restrict__ Ud, const int2 *__restrict__ list,
+                         int count_flops) {
   __shared__ int red[8];
   const int2 ref = list[blockIdx.x];
   const UpdDesc &u = Ud[ref.x];
@@ -628,6 +645,12 @@ __global__ void k_kcount(const UpdDesc *__restrict__ Ud, const int2 *__restrict_
     int tot = 0;
     for (int i = 0; i < (int)(blockDim.x >> 6); ++i) tot += red[i];
     u.kcount[ref.y] = tot;
+    if (count_flops && tot > 0) {
+      const double k = tot, L = s.L, R = s.R, n = s.n;
+      atomicAdd(&g_kept_flops[0], 8.0 * k * L * R);
+      if (!s.left_small) atomicAdd(&g_kept_flops[1], 8.0 * k * L * R);
+      if (s.n > u.eig_thr) atomicAdd(&g_kept_flops[2], 8.0 * k * n * n);
+    }
   }
 }
 
@@ -1091,7 +1114,7 @@ int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
 int launch_kcount(const GroupHeader &g, const char *w, hipStream_t stream) {
   if (g.n_sec == 0) return 0;
   hipLaunchKernelGGL(k_kcount, dim3(g.n_sec), dim3(256), 0, stream, gdesc(g, w),
-                     glist(w, g.o_sec));
+                     glist(w, g.o_sec), timing_on() ? 1 : 0);
   BSMPO_HIP(hipGetLastError());
   return 0;
 }

@@ -182,6 +182,7 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
     D.edge = io[u].edge;
     D.n_sectors = h->n_sectors;
     D.n_cblk = h->n_cblk;
+    D.eig_thr = h->eig_threshold;
     D.sv_total = h->sv_total;
     D.tie_rel = h->tie_rel;
     D.titems = reinterpret_cast<const TileItem *>(pb + h->o_titems);

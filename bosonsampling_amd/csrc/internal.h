@@ -51,7 +51,9 @@ struct PlanHeader {
   int64_t rows_total, cols_total;
   double flops_alg, flops_tau, bytes_alg16;
   // flops of the SVD stages (complex MAC = 8): Gram 8 m n^2, W 8 n L R,
-  // vectors 8 L R n, Jacobi one sweep of 8 n^3 (reported per sweep-equivalent)
+  // vectors 8 L R n, Jacobi one sweep of 8 n^3 (reported per sweep-equivalent);
+  // W and vectors are upper bounds over all n pairs (stage timing counts the
+  // kept pairs on the device instead)
   double flops_gram, flops_wmat, flops_vectors, flops_jacobi;
   // tridiagonal route: Householder flops (16/3) n^3 and the blocked reduction's
   // trailing-matrix traffic (16 m^2 bytes read per column, 32 m^2 per panel)
@@ -183,6 +185,8 @@ bool timing_on();
 // begin returns a token (or -1 when timing is off); end closes it.
 int timing_begin(int stage, hipStream_t stream);
 void timing_end(int token, hipStream_t stream, double flops, double bytes);
+// flops of the kept-pair stages (W, V, back-transformation) counted on the device
+int kept_flops_read(double *out, bool reset);
 
 // -------------------------------------------------------------- kernels --
 // launchers implemented in the .hip files; all enqueue on `stream`.

@@ -524,14 +524,14 @@ int launch_truncate(const GroupHeader &g, const char *w, hipStream_t stream) {
     hipLaunchKernelGGL(k_vectors, dim3(grid_for(g.n_v)), dim3(256), 0, stream, gdesc(g, w),
                        glist(w, g.o_v), g.n_v);
     BSMPO_HIP(hipGetLastError());
-    timing_end(tk, stream, g.flops_vectors, 0.0);
+    timing_end(tk, stream, 0.0, 0.0);  // flops: kept pairs, counted by k_kcount
   }
   if (g.n_w) {
     tk = timing_begin(ST_WMAT, stream);
     hipLaunchKernelGGL(k_wmat, dim3(grid_for(g.n_w)), dim3(256), 0, stream, gdesc(g, w),
                        glist(w, g.o_w), g.n_w);
     BSMPO_HIP(hipGetLastError());
-    timing_end(tk, stream, g.flops_wmat, 16.0 * (g.v_elems + g.c_elems + g.w_elems));
+    timing_end(tk, stream, 0.0, 16.0 * (g.v_elems + g.c_elems + g.w_elems));  // flops: k_kcount
   }
   tk = timing_begin(ST_SCATTER, stream);
   hipLaunchKernelGGL(k_zero, dim3(256, g.n_upd), dim3(256), 0, stream, gdesc(g, w));
