@@ -93,7 +93,7 @@ def stage_table(stages):
 # Roof of each stage: ("mfma" | "valu" | "hbm", peak key); None = latency-bound
 # chains with no roof (Jacobi sweeps, bisection, inverse iteration).
 STAGE_ROOF = {"contract": ("mfma", None), "gram": ("mfma", "f64"), "wmat": ("mfma", "f64"),
-              "vectors": ("mfma", "f64"), "backtransform": ("valu", "f64"),
+              "vectors": ("mfma", "f64"), "backtransform": ("mfma", "f64"),
               "tridiag": ("hbm", None), "topk": ("hbm", None), "scatter": ("hbm", None),
               "observables": ("hbm", None), "bs_table": ("hbm", None),
               "jacobi": None, "bisect": None, "trivec": None}

@@ -900,122 +900,6 @@ static int vec_lds_nsplit() {
   return n;
 }
 
-// x = H_0 H_1 ... H_{n-2} y for the kept eigenvectors of every large sector;
-// one wave per vector, reflectors staged in LDS up to kBtChunk at a time
-constexpr int kBtChunk = 8;
-constexpr int kBtLds = 4096;  // complex elements of the reflector cache (64 KiB)
-template <int kBtRows>
-__global__ __launch_bounds__(kBtRows <= 16 ? 1024 : 256) void k_tri_backtransform(
-    const UpdDesc *__restrict__ Ud, const int2 *__restrict__ list, int nmin) {
-  extern __shared__ double2 vsm[];  // reflector cache: per x n
-  const int2 ref = list[blockIdx.x];
-  const UpdDesc &u = Ud[ref.x];
-  const Sector s = u.sectors[ref.y];
-  const int n = s.n, k = u.kcount[ref.y];
-  if (k == 0 || n < nmin || n > 64 * kBtRows) return;  // nothing kept, or another size class
-  const int per = max(1, min(kBtChunk, kBtLds / max(n, 1)));
-  gptr<double2> X = gp(s.left_small ? reinterpret_cast<double2 *>(u.V) + s.v_off
-                                     : reinterpret_cast<double2 *>(u.W) + s.w_off);
-  const gptr<const double2> A = gp(reinterpret_cast<const double2 *>(u.G) + s.g_off);
-  const gptr<const double2> tau = gp(reinterpret_cast<const double2 *>(u.tau) + s.s_off);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  constexpr int kBtVec = 1;  // n <= 64 * kBtRows
-  // the chunk ilo..ihi of reflector columns (zero above the unit entry); for
-  // n <= 512 it is read into registers one chunk ahead of its use
-  constexpr bool kAhead = kBtRows <= 8;
-  constexpr int kPre = kAhead ? kBtLds / 1024 : 1;  // staged elements per thread
-  double2 pre[kPre];
-  auto column = [&](int e, int ilo) {
-    const int i = ilo + e / n, r = e % n;
-    return r <= i ? make_double2(0.0, 0.0)
-                  : (r == i + 1 ? make_double2(1.0, 0.0) : gload2(A, (int64_t)i * n + r));
-  };
-  auto fetch = [&](int ihi) {
-    if constexpr (kAhead) {
-      const int ilo = max(0, ihi - per + 1), cnt = (ihi - ilo + 1) * n;
-#pragma unroll
-      for (int q = 0; q < kPre; ++q) {
-        const int e = threadIdx.x + q * (int)blockDim.x;
-        pre[q] = (ihi < 0 || e >= cnt) ? make_double2(0.0, 0.0) : column(e, ilo);
-      }
-    }
-  };
-  for (int j0 = 0; j0 < k; j0 += nw * kBtVec) {
-    double2 x[kBtVec][kBtRows];
-    const int jb = j0 + wave * kBtVec;
-#pragma unroll
-    for (int u = 0; u < kBtVec; ++u)
-#pragma unroll
-      for (int q = 0; q < kBtRows; ++q) {
-        const int r = lane + 64 * q;
-        x[u][q] = (jb + u < k && r < n) ? X[(int64_t)(jb + u) * n + r] : make_double2(0.0, 0.0);
-      }
-    // reflectors n-2 .. 0, staged `per` at a time in LDS (one barrier pair per chunk)
-    fetch(n - 2);
-    for (int ihi = n - 2; ihi >= 0; ihi -= per) {
-      const int ilo = max(0, ihi - per + 1);
-      __syncthreads();
-      if constexpr (kAhead) {
-#pragma unroll
-        for (int q = 0; q < kPre; ++q) {
-          const int e = threadIdx.x + q * (int)blockDim.x;
-          if (e < (ihi - ilo + 1) * n) vsm[e] = pre[q];
-        }
-      } else {
-        for (int e = threadIdx.x; e < (ihi - ilo + 1) * n; e += blockDim.x) vsm[e] = column(e, ilo);
-      }
-      __syncthreads();
-      fetch(ihi - per);  // next chunk's loads in flight during this chunk
-      for (int i = ihi; i >= ilo; --i) {
-        const double2 ti = tau[i];
-        if (ti.x == 0.0 && ti.y == 0.0) continue;  // uniform
-        const double2 *v = vsm + (int64_t)(i - ilo) * n;
-        double sr[kBtVec], si[kBtVec];
-#pragma unroll
-        for (int u = 0; u < kBtVec; ++u) { sr[u] = 0.0; si[u] = 0.0; }
-#pragma unroll
-        for (int q = 0; q < kBtRows; ++q) {
-          const int r = lane + 64 * q;
-          if (r > i && r < n) {
-            const double2 a = v[r];
-#pragma unroll
-            for (int u = 0; u < kBtVec; ++u) {
-              sr[u] += a.x * x[u][q].x + a.y * x[u][q].y;
-              si[u] += a.x * x[u][q].y - a.y * x[u][q].x;
-            }
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < kBtVec; ++u)
-          for (int o = 32; o >= 1; o >>= 1) {
-            sr[u] += __shfl_xor(sr[u], o);
-            si[u] += __shfl_xor(si[u], o);
-          }
-#pragma unroll
-        for (int q = 0; q < kBtRows; ++q) {
-          const int r = lane + 64 * q;
-          if (r > i && r < n) {
-            const double2 a = v[r];
-#pragma unroll
-            for (int u = 0; u < kBtVec; ++u) {
-              const double2 f =
-                  make_double2(ti.x * sr[u] - ti.y * si[u], ti.x * si[u] + ti.y * sr[u]);
-              x[u][q].x -= a.x * f.x - a.y * f.y;
-              x[u][q].y -= a.x * f.y + a.y * f.x;
-            }
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kBtVec; ++u)
-#pragma unroll
-      for (int q = 0; q < kBtRows; ++q) {
-        const int r = lane + 64 * q;
-        if (jb + u < k && r < n) X[(int64_t)(jb + u) * n + r] = x[u][q];
-      }
-  }
-}
 
 static inline const UpdDesc *gdesc(const GroupHeader &g, const char *w) {
   return reinterpret_cast<const UpdDesc *>(w + g.o_desc);
@@ -1133,40 +1017,7 @@ int launch_tri_vectors(const GroupHeader &g, const char *w, hipStream_t stream) 
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, 0.0, 0.0);
   tk = timing_begin(ST_BACKTRANS, stream);
-  // one launch per sector size class (rows per lane), each over its segment of
-  // the list (sorted largest first): the few long-running sectors above 512
-  // go first on the launching stream, the bulk on a side stream forked before
-  const size_t lds = (size_t)kBtLds * 16;
-  const int2 *tl = glist(w, g.o_tri);
-  const int nbig = g.tri_nbt;  // n > 512
-  static_assert(kTriBtSplit == 512, "size class boundary of the back-transformation");
-  const bool both = nbig > 0 && nbig < g.n_tri;
-  SideStream *ss = nullptr;
-  if (both) {
-    int rc = side_stream(stream, &ss);
-    if (rc) return rc;
-    BSMPO_HIP(hipEventRecord(ss->fork, stream));
-    BSMPO_HIP(hipStreamWaitEvent(ss->stream, ss->fork, 0));
-  }
-  if (nbig > 0) {
-    hipLaunchKernelGGL(k_tri_backtransform<16>, dim3(nbig), dim3(1024), lds, stream, gdesc(g, w), tl,
-                       513);
-    BSMPO_HIP(hipGetLastError());
-    if (g.tri_max_n > 1024) {
-      hipLaunchKernelGGL(k_tri_backtransform<32>, dim3(nbig), dim3(256), lds, stream, gdesc(g, w),
-                         tl, 1025);
-      BSMPO_HIP(hipGetLastError());
-    }
-  }
-  if (nbig < g.n_tri) {
-    hipLaunchKernelGGL(k_tri_backtransform<8>, dim3(g.n_tri - nbig), dim3(1024), lds,
-                       both ? ss->stream : stream, gdesc(g, w), tl + nbig, 1);
-    BSMPO_HIP(hipGetLastError());
-  }
-  if (both) {
-    BSMPO_HIP(hipEventRecord(ss->join, ss->stream));
-    BSMPO_HIP(hipStreamWaitEvent(stream, ss->join, 0));
-  }
+  if (int rc = launch_wy_backtransform(g, w, stream)) return rc;
   timing_end(tk, stream, 0.0, 0.0);
   return 0;
 }

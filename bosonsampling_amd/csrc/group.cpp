@@ -88,6 +88,18 @@ int layout(int n, const void *const *plans, Layout &L) {
   g.o_sec = place(8 * n_sec);
   g.o_v = place(8 * n_v);
   g.o_w = place(8 * n_w);
+  // back-transformation items of the tridiagonal-route sectors: one per WY
+  // block of reflectors and one per panel of (at most) kWyBlock kept vectors
+  int64_t n_wyt = 0, n_wya = 0;
+  for (int u = 0; u < n; ++u) {
+    n_wyt += header(plans[u])->n_wyt;
+    n_wya += header(plans[u])->n_wya;
+  }
+  if (n_wyt > INT32_MAX || n_wya > INT32_MAX) return fail(-1, "group: too many work items");
+  g.n_wyt = (int32_t)n_wyt;
+  g.n_wya = (int32_t)n_wya;
+  g.o_wyt = place(8 * n_wyt);
+  g.o_wya = place(8 * n_wya);
   g.o_plans = o;
   for (int u = 0; u < n; ++u) o += region(header(plans[u])->dev_bytes);
   g.staging_bytes = o;
@@ -205,7 +217,7 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
     for (int i = h->jac_off[kTriClass]; i < h->jac_off[kTriClass + 1]; ++i) tri[nt++] = make_int2(u, jo[i]);
   }
   if (contract_only) {  // a group that only runs the contraction stage
-    g.n_gram = g.n_tri = g.n_sec = g.n_v = g.n_w = 0;
+    g.n_gram = g.n_tri = g.n_sec = g.n_v = g.n_w = g.n_wyt = g.n_wya = 0;
     for (int c = 0; c <= kTriClass; ++c) g.jac_off[c] = 0;
     std::memcpy(host_blob, &g, sizeof(GroupHeader));
     return 0;
@@ -225,7 +237,17 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
     };
     g.tri_split = tri_split();
     g.tri_nbig = count_above(g.tri_split);
-    g.tri_nbt = count_above(kTriBtSplit);
+    // back-transformation items in the same order (largest sectors first)
+    int2 *wyt = reinterpret_cast<int2 *>(img + g.o_wyt);
+    int2 *wya = reinterpret_cast<int2 *>(img + g.o_wya);
+    int64_t a = 0, b = 0;
+    for (int64_t t = 0; t < nt; ++t) {
+      const int nn = size_of(tri[t]);
+      for (int m = 0; m < (nn - 1 + kWyBlock - 1) / kWyBlock; ++m)
+        wyt[a++] = make_int2(tri[t].x, tri[t].y | m << 20);
+      for (int p = 0; p < (nn + kWyBlock - 1) / kWyBlock; ++p)
+        wya[b++] = make_int2(tri[t].x, tri[t].y | p << 20);
+    }
   }
   // LDS Jacobi list grouped by size class across updates
   int64_t nj = 0;

@@ -35,7 +35,7 @@ constexpr int kJacLimit[kJacClasses] = {16, 32, 64, 96, 1 << 30};
 constexpr int kTriClass = kJacClasses - 1;
 constexpr int kTriPanel = 8;  // column panel of the blocked Householder reduction
 constexpr int kTriVecThreads = 128;  // inverse-iteration threads per sector
-constexpr int kTriBtSplit = 512;  // back-transformation: sectors above take the 16-row class
+constexpr int kWyBlock = 32;  // back-transformation: reflectors per WY block, vectors per panel
 int eig_threshold();  // sectors with n > threshold take the tridiagonal route
 int tri_split();      // many-sector groups: Householder blocks of 512 threads for n > split
 double tie_rel();     // relative width of the tie band at the chi-th singular value
@@ -78,6 +78,9 @@ struct PlanHeader {
   double flops_weight;  // BS transforms: 8 sum_pairs nL nR B0 B1 (B0 + B1)
   double mfma_k4;       // sum over tile items and their centre blocks of K padded to 4
   double mfma_k32;      // the same with K padded to 32 (BF16x3 steps)
+  // back-transformation items of the tridiagonal-route sectors (WY blocks of
+  // reflectors, panels of kept vectors); layouts read only header fields
+  int32_t n_wyt, n_wya;
 };
 
 // Tile formulation of the sector contraction.  A TileItem is one pass over
@@ -154,7 +157,8 @@ struct GroupHeader {
   int64_t o_tiles;
   double flops_weight, mfma_k4, mfma_k32;
   int32_t tri_split, tri_nbig;  // the block-shape split in effect; sectors with n > tri_split
-  int32_t tri_nbt, pad2;        // sectors with n > kTriBtSplit
+  int32_t n_wyt, n_wya;         // back-transformation items: (sector, block), (sector, panel)
+  int64_t o_wyt, o_wya;
 };
 constexpr int64_t kGroupHostHead = 1024;  // host blob: header, then the staging image
 static_assert(sizeof(GroupHeader) <= kGroupHostHead, "group header too large");
@@ -194,6 +198,7 @@ int launch_bs_table(int d, int nb, const double *d_B, double *d_U3, hipStream_t 
 int launch_contract(const GroupHeader &g, const char *d_work, int precision, hipStream_t stream);
 int launch_svd(const GroupHeader &g, const char *d_work, hipStream_t stream);
 int launch_truncate(const GroupHeader &g, const char *d_work, hipStream_t stream);
+int launch_wy_backtransform(const GroupHeader &g, const char *d_work, hipStream_t stream);
 
 // group construction (group.cpp)
 struct UpdateIO {

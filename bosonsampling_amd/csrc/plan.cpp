@@ -237,7 +237,13 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
           // per-lane inverse-iteration arrays (6 n doubles for each of the
           // min(n, kTriVecThreads) lanes of the sector's block); the Householder
           // panels V, W (2 n kTriPanel complex) use the same area first
-          tw += (int64_t)n * std::max(6 * std::min(n, kTriVecThreads), 4 * kTriPanel);
+          // then the back-transformation's T blocks (kWyBlock^2 complex per
+          // block of reflectors), once the inverse iteration is done
+          const int nblk = (n - 1 + kWyBlock - 1) / kWyBlock;
+          b.h.n_wyt += nblk;
+          b.h.n_wya += (n + kWyBlock - 1) / kWyBlock;
+          const int64_t wy = 2 * (int64_t)kWyBlock * kWyBlock * nblk;
+          tw += std::max<int64_t>((int64_t)n * std::max(6 * std::min(n, kTriVecThreads), 4 * kTriPanel), wy);
         }
       }
     }

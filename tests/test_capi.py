@@ -130,3 +130,24 @@ def test_group_runner_stream_count():
         engine.GroupRunner("cpu", streams=5)
     assert engine.GroupRunner("cpu", streams=3).streams == 3
     assert engine.GroupRunner("cpu", streams=3, use_graph=True).streams == 1
+
+
+def test_plan_query_with_every_sector_on_the_tridiagonal_route():
+    """Planning reads only the plan header when it sizes the single-update
+    workspace; with every sector on the tridiagonal route (threshold 0) the
+    back-transformation's per-sector items enter that size too."""
+    from bosonsampling_amd import engine
+    from bosonsampling_amd._capi import check, lib
+    from synth import square_charges
+    d, chi = 6, 48
+    CL, CC, CR = square_charges(chi, 2, 5), square_charges(chi, 1, 4), square_charges(chi, 0, 3)
+    sizes = {}
+    try:
+        for thr in (96, 0):
+            check(lib().bsmpo_set_eig_threshold(thr), "set_eig_threshold")
+            p = engine.Plan(d, chi, CL, CC, CR)
+            sizes[thr] = p.work_bytes
+    finally:
+        check(lib().bsmpo_set_eig_threshold(96), "set_eig_threshold")
+    c_bytes = 16 * int(p.info().c_elems)
+    assert 2 * c_bytes < sizes[0] < 1000 * c_bytes and sizes[96] < 1000 * c_bytes
