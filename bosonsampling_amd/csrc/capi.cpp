@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -24,13 +25,31 @@ struct TimingRec {
   bool closed;
 };
 std::mutex g_tmu;
-bool g_timing = false;
+std::atomic<bool> g_timing{false};
 std::vector<TimingRec> g_recs;
 double g_ms[ST_COUNT], g_flops[ST_COUNT], g_bytes[ST_COUNT];
 int64_t g_cnt[ST_COUNT];
 }  // namespace
 
 bool timing_on() { return g_timing; }
+
+// Raises a kernel's dynamic-LDS limit once per (kernel, device), under a lock.
+int ensure_lds_limit(const void *kernel, int bytes) {
+  static std::mutex mu;
+  struct Entry {
+    const void *kernel;
+    int dev, bytes;
+  };
+  static std::vector<Entry> done;
+  int dev = 0;
+  BSMPO_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  for (const Entry &e : done)
+    if (e.kernel == kernel && e.dev == dev && e.bytes >= bytes) return 0;
+  BSMPO_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  done.push_back(Entry{kernel, dev, bytes});
+  return 0;
+}
 
 int timing_begin(int stage, hipStream_t stream) {
   if (!g_timing) return -1;

@@ -521,15 +521,10 @@ int launch_tile(const GroupHeader &g, const char *w, hipStream_t stream) {
   const int2 *list = reinterpret_cast<const int2 *>(w + g.o_tiles);
   const int wtab_elems = (wtab_off(g.d_max) + 1) / 2 * 2;  // keeps P 16-byte aligned
   const size_t wbytes = sizeof(cx<T>) * (size_t)wtab_elems;
-  static std::once_flag once;
-  static hipError_t attr_err = hipSuccess;
-  std::call_once(once, [&] {
-    attr_err = hipFuncSetAttribute((const void *)k_contract_tile<PREC, LCW>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(sizeof(cx<T>) * (kTileElems[kTileClasses - 1] +
-                                                          (size_t)wtab_off(kMaxD) + 1)));
-  });
-  BSMPO_HIP(attr_err);
+  if (int rc = ensure_lds_limit((const void *)k_contract_tile<PREC, LCW>,
+                                (int)(sizeof(cx<T>) * (kTileElems[kTileClasses - 1] +
+                                                       (size_t)wtab_off(kMaxD) + 1))))
+    return rc;
   // largest class first: the long items start early, the short ones fill in
   for (int c = kTileClasses - 1; c >= 0; --c) {
     const int n = g.tile_off[c + 1] - g.tile_off[c];

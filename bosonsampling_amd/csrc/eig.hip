@@ -943,14 +943,8 @@ static int side_stream(hipStream_t parent, SideStream **out) {
 int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
   if (g.n_tri == 0) return 0;
   if (g.tri_max_n > kTriMaxN) return fail(-1, "sector too large for the tridiagonal route (n > 2048)");
-  static bool attr = false;
-  if (!attr) {
-    BSMPO_HIP(hipFuncSetAttribute((const void *)k_tridiag<256, 2>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 512 * 32));
-    BSMPO_HIP(hipFuncSetAttribute((const void *)k_tridiag<512, 4>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, kTriMaxN * 32));
-    attr = true;
-  }
+  if (int rc = ensure_lds_limit((const void *)k_tridiag<256, 2>, 512 * 32)) return rc;
+  if (int rc = ensure_lds_limit((const void *)k_tridiag<512, 4>, kTriMaxN * 32)) return rc;
   int tk = timing_begin(ST_TRIDIAG, stream);
   // The list is sorted largest first.  Few sectors (a single update): the
   // larger block shape for every sector, so that each column chain gets more

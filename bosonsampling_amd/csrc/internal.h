@@ -186,6 +186,8 @@ enum Stage {
   ST_SCATTER, ST_OBSERVABLES, ST_TRIDIAG, ST_BISECT, ST_TRIVEC, ST_BACKTRANS, ST_COUNT
 };
 bool timing_on();
+// raise a kernel's dynamic-LDS limit (once per kernel and device, thread-safe)
+int ensure_lds_limit(const void *kernel, int bytes);
 // begin returns a token (or -1 when timing is off); end closes it.
 int timing_begin(int stage, hipStream_t stream);
 void timing_end(int token, hipStream_t stream, double flops, double bytes);

@@ -10,6 +10,7 @@
 // MFMA kernels mask ragged tiles instead.
 #include <algorithm>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -18,9 +19,10 @@
 namespace bsmpo {
 
 static thread_local std::string g_last_error;
-static int g_eig_threshold = 96;
-static int g_tri_split = 384;
-static double g_tie_rel = 1e-6;
+// process-wide knobs, set and read from any thread
+static std::atomic<int> g_eig_threshold{96};
+static std::atomic<int> g_tri_split{384};
+static std::atomic<double> g_tie_rel{1e-6};
 int eig_threshold() { return g_eig_threshold; }
 int tri_split() { return g_tri_split; }
 double tie_rel() { return g_tie_rel; }

@@ -492,12 +492,7 @@ int launch_svd(const GroupHeader &g, const char *w, hipStream_t stream) {
                      glist(w, g.o_gram), g.n_gram);
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, g.flops_gram, 16.0 * (g.c_elems + g.g_elems));
-  static bool attr_set = false;
-  if (!attr_set) {
-    BSMPO_HIP(hipFuncSetAttribute((const void *)k_jacobi,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 96 * 16));
-    attr_set = true;
-  }
+  if (int rc = ensure_lds_limit((const void *)k_jacobi, 96 * 96 * 16)) return rc;
   tk = timing_begin(ST_JACOBI, stream);
   for (int c = 0; c < kTriClass; ++c) {
     const int nsec = g.jac_off[c + 1] - g.jac_off[c];

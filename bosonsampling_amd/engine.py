@@ -125,7 +125,9 @@ class DeviceMPO:
         self.charge = torch.full((M + 1, chi, 2), d, dtype=torch.int32, device=self.device)
         self.edge = torch.ones(chi, dtype=torch.float64, device=self.device)
         self.charge_host = np.full((M + 1, chi, 2), d, np.int32)
-        self.num_lambda = torch.zeros(1, dtype=torch.int32, device=self.device)
+        # kept count of the last update of each pair (mode, mode + 1): one slot per
+        # mode, so the updates of one grouped run never write the same slot
+        self.num_lambda = torch.zeros(max(M - 1, 1), dtype=torch.int32, device=self.device)
         self.ws = Workspace(self.device)
 
     # ------------------------------------------------------------ host I/O
@@ -172,7 +174,7 @@ class DeviceMPO:
             C.c_void_p(self.Gamma[mode].data_ptr()), C.c_void_p(self.Gamma[mode + 1].data_ptr()),
             C.c_void_p(LR.data_ptr()), C.c_void_p(U3.data_ptr()),
             C.c_void_p(self.charge[mode + 1].data_ptr()), C.c_void_p(self.Lambda[mode].data_ptr()),
-            C.c_void_p(self.num_lambda.data_ptr()), 1 if mode == M - 2 else 0, int(precision),
+            C.c_void_p(self.num_lambda[mode].data_ptr()), 1 if mode == M - 2 else 0, int(precision),
             C.c_void_p(work.data_ptr()), work.numel(), _stream()), "site_update")
         if sync_charges:
             self.charge_host[mode + 1] = self.charge[mode + 1].cpu().numpy()
@@ -299,7 +301,8 @@ class GroupRunner:
                 io[j] = _capi.UpdateIO(st.Gamma[mode].data_ptr(), st.Gamma[mode + 1].data_ptr(),
                                        LR.data_ptr(), U3.data_ptr(),
                                        st.charge[mode + 1].data_ptr(), st.Lambda[mode].data_ptr(),
-                                       st.num_lambda.data_ptr(), 1 if mode == st.M - 2 else 0, 0)
+                                       st.num_lambda[mode].data_ptr(), 1 if mode == st.M - 2 else 0,
+                                       0)
             check(lib().bsmpo_group_build(n, arr, io, C.c_void_p(work.data_ptr()),
                                           C.c_void_p(host.data_ptr()), hb.value), "group_build")
             if self.use_graph:

@@ -247,7 +247,7 @@ def test_group_graph_replays_match_eager(gpu):
         LR = st.edge if mode == M - 2 else st.Lambda[mode + 1]
         io[j] = _capi.UpdateIO(st.Gamma[mode].data_ptr(), st.Gamma[mode + 1].data_ptr(),
                                LR.data_ptr(), U3.data_ptr(), st.charge[mode + 1].data_ptr(),
-                               st.Lambda[mode].data_ptr(), st.num_lambda.data_ptr(),
+                               st.Lambda[mode].data_ptr(), st.num_lambda[mode].data_ptr(),
                                1 if mode == M - 2 else 0, 0)
     check(lib().bsmpo_group_build(n, arr, io, C.c_void_p(work.data_ptr()),
                                   C.c_void_p(host.data_ptr()), hb.value), "group_build")

@@ -4,6 +4,7 @@ import json
 import os
 
 import numpy as np
+import pytest
 
 from bosonsampling_amd import run
 
@@ -61,3 +62,37 @@ def test_queue_doubles_until_accurate(tmp_path):
     assert ran == 2
     with open(path) as f:
         assert [e["status"] for e in json.load(f)] == ["complete", "complete"]
+
+
+class FailingWorker(FakeWorker):
+    def Simulate(self):
+        raise RuntimeError("sector too large for the tridiagonal route")
+
+
+def test_failed_experiment_is_released(tmp_path):
+    """A run that raises leaves its experiment 'failed', not 'in progress',
+    and the queue goes on to the next one."""
+    path = str(tmp_path / "experiments.json")
+    with open(path, "w") as f:
+        json.dump([EXP, dict(EXP, n_modes=8)], f)
+    ran = run.run_queue(path, FailingWorker, str(tmp_path / "Results"), log=lambda *a: None)
+    assert ran == 0
+    with open(path) as f:
+        assert [e["status"] for e in json.load(f)] == ["failed", "failed"]
+
+
+def test_chi_doubling_stops_at_the_library_limit(tmp_path):
+    root = str(tmp_path)
+    exp = dict(EXP, bond_dimension=run.LIB_MAX_CHI)
+    run.save_results(exp, run.LIB_MAX_CHI, np.array([1.0, 0.5, 0.4]), np.zeros((7, 3)), root)
+    assert run.choose_bond_dimension(exp, root)[0] == "skip"
+
+
+def test_pickled_queue_needs_opt_in(tmp_path):
+    import pickle
+    path = str(tmp_path / "experiments.pickle")
+    with open(path, "wb") as f:
+        pickle.dump([EXP], f)
+    with pytest.raises(ValueError):
+        run.load_experiments(path)
+    assert run.load_experiments(path, allow_pickle=True)[0]["n_modes"] == EXP["n_modes"]
