@@ -175,11 +175,14 @@ class _HaarReflectivity:
 def reflectivity_and_seeds(n_modes: int, seed=None):
     """Reference mesh parameters: per-BS reflectivity |r| from the
     Haar-induced law and an integer phase seed (src/interferometer.py:9-54).
-    Uses numpy's global RNG exactly as the reference does."""
+    Uses numpy's global RNG exactly as the reference does.  Slots no beam
+    splitter uses (the last one of odd layers) are 0 here, where the reference
+    leaves np.empty's contents; odd n_modes raise IndexError, as in the
+    reference."""
     if seed is not None:
         np.random.seed(seed)
-    reflectivity = np.empty([n_modes, n_modes // 2])
-    seeds = np.empty([n_modes, n_modes // 2], dtype=int)
+    reflectivity = np.zeros([n_modes, n_modes // 2])
+    seeds = np.zeros([n_modes, n_modes // 2], dtype=int)
     for k in range(n_modes - 1):
         if k < n_modes / 2:
             temp1, temp2 = 2 * k + 1, 2

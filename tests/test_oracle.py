@@ -6,6 +6,7 @@ here by mathematics chosen so that a plausible slip (dropped term, wrong
 sign or index, transposed operand) fails at least one check.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -264,3 +265,25 @@ def test_entropy_single_value_scale_and_zero_bond():
     lam = rng.random((3, 50))
     assert np.allclose(mpo.entanglement_entropy(7.5 * lam), mpo.entanglement_entropy(lam),
                        rtol=0, atol=1e-13)
+
+
+# ------------------------------------------------ reference mesh sampler --
+# src/interferometer.py:9-54 draws reflectivities through scipy's generic
+# rv_continuous sampler (numeric inverse CDF) and integer phase seeds from
+# numpy's global RNG; tests/golden/reflectivity.npz holds its own output
+# (tests/golden/make_reflectivity.py).  The mirror must give the same values
+# bit for bit on every slot that holds a beam splitter.
+
+def test_reflectivity_and_seeds_match_reference_vectors():
+    from golden.make_reflectivity import used_slots
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "reflectivity.npz"))
+    for M, seed in z["cases"]:
+        r, s = mesh.reflectivity_and_seeds(int(M), int(seed))
+        mask = used_slots(int(M))
+        assert np.array_equal(np.where(mask, r, 0.0), z[f"r_{M}_{seed}"])
+        assert np.array_equal(np.where(mask, s, 0), z[f"s_{M}_{seed}"])
+
+
+def test_reflectivity_and_seeds_odd_modes_raise_like_the_reference():
+    with pytest.raises(IndexError):
+        mesh.reflectivity_and_seeds(5, 1)
