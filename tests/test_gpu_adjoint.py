@@ -142,48 +142,95 @@ def test_mesh_gradient_matches_oracle(gpu, ckpt):
     assert err <= 1e-8
 
 
-@pytest.mark.parametrize("prec,tol", [(0, 1e-5), (1, 1e-3)], ids=["c128", "c64"])
-def test_diff_config_gradient_vs_finite_differences(gpu, prec, tol):
-    """M=16, photons in modes 0-7, d=9, chi=128, eta=0.8, Haar seed 2 mesh
-    (120 beam splitters); L = -log P(pattern) + 0.05 sum_{j >= 8} <n_j>;
-    checkpoints every 8 layers.  The adjoint gradient against central
-    differences (step 1e-6) of the complex128 forward on 16 of the 240 angles.
+DIFF = dict(M=16, d=9, chi=128, inputs=list(range(8)), eta=0.8, seed=2,
+            pattern=[1, 0] * 8, weights=np.array([0.0] * 8 + [0.05] * 8))
+_FD = {}
 
-    The pattern is one photon in each even mode.  One photon in each of modes
-    8-15 has P = 0 in this chi = 128 MPO: the truncation removes every slot of
-    charge (8, 8) on bond 8 (exact P = 1.5e-7 from the permanent; the MPO's
-    trace is 0.076), so its -log P is infinite."""
+
+def _diff_state(gpu):
     from bosonsampling_amd import engine
-    from bosonsampling_amd.adjoint import Run, mesh_loss
-    M, d, chi, inputs, eta = 16, 9, 128, list(range(8)), 0.8
-    params = _params(M, 2)
-    pattern = [1, 0] * 8
-    w = np.array([0.0] * 8 + [0.05] * 8)
-    st0 = engine.initialize(M, d, chi, inputs, eta)
-    run = Run(st0, inputs, pattern, w, checkpoint_every=8, precision=prec)
-    pt = torch.tensor(params, requires_grad=True)
-    loss = mesh_loss(pt, run)
-    loss.backward()
-    g = pt.grad.numpy()
-    print(f"diff config prec={prec}: loss {loss.item():.10f}", flush=True)
-    assert np.isfinite(loss.item()) and np.all(np.isfinite(g))
-    ref = Run(st0, inputs, pattern, w, checkpoint_every=8, precision=0)
+    c = DIFF
+    return engine.initialize(c["M"], c["d"], c["chi"], c["inputs"], c["eta"], device=gpu)
+
+
+def _central_differences(gpu):
+    """Central differences (step 1e-6) of the complex128 forward of the
+    differentiation workload at 64 angles: 32 random (layer, beam splitter)
+    slots, theta and phi each.  Computed once per session and shared by the
+    precisions under test."""
+    if "fd" in _FD:
+        return _FD["fd"]
+    from bosonsampling_amd import engine
+    from bosonsampling_amd.adjoint import Run
+    c = DIFF
+    M, params = c["M"], _params(c["M"], c["seed"])
+    ref = Run(_diff_state(gpu), c["inputs"], c["pattern"], c["weights"], checkpoint_every=8,
+              precision=0)
 
     def L(p):
         return ref.forward(oadj.mesh_from_params(M, p))[0]
 
     used = [(lay, i) for lay in range(M) for i, _ in engine.layer_modes(M, lay, 7)]
     rng = np.random.default_rng(0)
-    pick = [used[j] for j in rng.choice(len(used), 8, replace=False)]
-    h = 1e-6
-    errs, scale = [], np.abs(g).max()
+    pick = [used[j] for j in rng.choice(len(used), 32, replace=False)]
+    h, fd = 1e-6, {}
     for lay, i in pick:
-        for c in range(2):
+        for k in range(2):
             pp, pm = params.copy(), params.copy()
-            pp[lay, i, c] += h
-            pm[lay, i, c] -= h
-            fd = (L(pp) - L(pm)) / (2 * h)
-            errs.append(abs(fd - g[lay, i, c]) / scale)
-            print(f"  ({lay},{i},{c}) adjoint {g[lay, i, c]:+.8e} fd {fd:+.8e}", flush=True)
-    print(f"max |adjoint - fd| / max|grad| = {max(errs):.2e}")
-    assert max(errs) <= tol
+            pp[lay, i, k] += h
+            pm[lay, i, k] -= h
+            fd[(lay, i, k)] = (L(pp) - L(pm)) / (2 * h)
+    print(f"central differences at {len(fd)} angles", flush=True)
+    _FD["fd"] = fd
+    return fd
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("prec", [0, 1], ids=["c128", "c64"])
+def test_diff_config_gradient_vs_finite_differences(gpu, prec):
+    """M=16, photons in modes 0-7, d=9, chi=128, eta=0.8, Haar seed 2 mesh
+    (120 beam splitters, 240 angles); L = -log P(pattern) + 0.05 sum_{j >= 8}
+    <n_j>, the pattern one photon in each even mode; checkpoints every 8
+    layers.  The adjoint gradient of the complex128 and complex64 forwards
+    against central differences of the complex128 forward (step 1e-6) at 64
+    angles, within 1e-5 of the largest gradient entry (the target; measured
+    2.0e-8 and 2.7e-6)."""
+    from bosonsampling_amd.adjoint import Run, mesh_loss
+    c = DIFF
+    params = _params(c["M"], c["seed"])
+    run = Run(_diff_state(gpu), c["inputs"], c["pattern"], c["weights"], checkpoint_every=8,
+              precision=prec)
+    pt = torch.tensor(params, requires_grad=True)
+    loss = mesh_loss(pt, run)
+    loss.backward()
+    g = pt.grad.numpy()
+    print(f"diff config prec={prec}: loss {loss.item():.10f}", flush=True)
+    assert np.isfinite(loss.item()) and np.all(np.isfinite(g))
+    fd = _central_differences(gpu)
+    scale = np.abs(g).max()
+    errs = [abs(v - g[key]) / scale for key, v in fd.items()]
+    print(f"prec={prec}: max |adjoint - fd| / max|grad| over {len(errs)} angles = {max(errs):.2e}")
+    assert len(errs) >= 64 and max(errs) <= 1e-5
+
+
+def test_diff_config_pattern_of_modes_8_to_15_is_truncated_away(gpu):
+    """One photon in each of modes 8-15 (the pattern the differentiation
+    workload names) has P = 0 in the chi = 128 MPO: truncation leaves no slot
+    of charge (8, 8) on bond 8, though the exact value
+    eta^8 |Perm(W[8:16, 0:8])|^2 is positive.  This is why the loss uses one
+    photon in each even mode instead."""
+    from bosonsampling_amd import engine
+    c = DIFF
+    M, d, chi = c["M"], c["d"], c["chi"]
+    mm = oadj.mesh_from_params(M, _params(M, c["seed"]))
+    st = _diff_state(gpu)
+    engine.simulate(st, mm, c["inputs"], observe=False)
+    G, L, ch = st.to_host()
+    host = mpo.MPOState(d, chi, G, L, ch, np.ones(chi))
+    pattern = [0] * 8 + [1] * 8
+    assert mpo.pattern_probability(host, pattern) == 0.0
+    assert not np.any((ch[8][:, 0] == 8) & (ch[8][:, 1] == 8))
+    W = meshmod.mesh_unitary(mm)
+    exact = c["eta"] ** 8 * abs(fock.permanent(W[8:16, 0:8])) ** 2
+    print(f"exact P(modes 8-15) = {exact:.3e}", flush=True)
+    assert exact > 1e-9
