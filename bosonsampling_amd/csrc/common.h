@@ -120,7 +120,8 @@ struct UpdDesc {
   double tie_rel;
   // tile formulation of the contraction
   const TileItem *titems;
-  const int32_t *tau_sector, *cblk_of, *rowoff, *coloff, *row_blk, *col_blk;
+  const int32_t *tau_sector, *cblk_of, *rowoff, *coloff;
+  const int2 *row_info, *col_info;  // per bond slot, see SlotInfo
   int32_t n_lblk, n_rblk;
 };
 

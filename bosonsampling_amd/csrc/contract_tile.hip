@@ -324,6 +324,7 @@ __global__ __launch_bounds__(256) void k_contract_tile(const UpdDesc *__restrict
   __shared__ int eoff[256];    // row in left block | column in right block << 16
   __shared__ double ewt[256];  // LR[j] (LL[i]): the T weight
   __shared__ int wcount[4];
+  __shared__ unsigned char pempty[256];  // union-box position without a centre block
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const double *w_u3 = nullptr;  // BS table the W table in LDS was built from
   // XCD-aware order: work-list position p runs on XCD p % 8 (blocks are
@@ -352,21 +353,34 @@ __global__ __launch_bounds__(256) void k_contract_tile(const UpdDesc *__restrict
     int *coff = roff + nbox * na;
     __syncthreads();  // the previous item's readers of LDS are done
     if (u.U3 != w_u3) {  // W_A[t][c] = U3[A - t][t][A - c] for every A < d
+      // one flat gather: every load is issued before any of them is stored
       w_u3 = u.U3;
-      const double2 *U = reinterpret_cast<const double2 *>(u.U3);
-      for (int A = 0; A < d; ++A) {
-        const int n = A + 1;
-        for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-          const int t = e / n, c = e % n;
-          const double2 v = U[((A - t) * d + t) * d + (A - c)];
-          Wt[wtab_off(A) + e] = cx<T>{(T)v.x, (T)v.y};
+      const gptr<const double2> U = gp(reinterpret_cast<const double2 *>(u.U3));
+      constexpr int kWq = (wtab_off(kMaxD) + 255) / 256;
+      const int nw = wtab_off(d);
+      double2 wv[kWq];
+#pragma unroll
+      for (int q = 0; q < kWq; ++q) {
+        const int e = threadIdx.x + 256 * q;
+        wv[q] = make_double2(0.0, 0.0);
+        if (e < nw) {
+          int A = 0;
+          while (wtab_off(A + 1) <= e) ++A;
+          const int f = e - wtab_off(A), n = A + 1, t = f / n, c = f % n;
+          wv[q] = gload2(U, ((A - t) * d + t) * d + (A - c));
         }
+      }
+#pragma unroll
+      for (int q = 0; q < kWq; ++q) {
+        const int e = threadIdx.x + 256 * q;
+        if (e < nw) Wt[e] = cx<T>{(T)wv[q].x, (T)wv[q].y};
       }
     }
     // non-empty centre blocks of the union box, compacted
     int cb = -1;
     if ((int)threadIdx.x < nbox)
       cb = u.cblk_of[(ti.lo0 + (int)threadIdx.x / B1u) * D1 + ti.lo1 + (int)threadIdx.x % B1u];
+    if ((int)threadIdx.x < nbox) pempty[threadIdx.x] = cb < 0;
     const unsigned long long m = __ballot(cb >= 0);
     const int local = __popcll(m & ((1ull << lane) - 1ull));
     if (lane == 0) wcount[wave] = __popcll(m);
@@ -377,17 +391,17 @@ __global__ __launch_bounds__(256) void k_contract_tile(const UpdDesc *__restrict
       int info = -1;
       if (e < E && row < ti.nr && col < ti.nc) {
         const int i = ti.r0 + row, j = ti.c0 + col;
-        const int a = u.row_blk[i], b = u.col_blk[j];
-        if (a >= 0 && b >= 0) {
-          const Block la = u.lblk[a], rb = u.rblk[b];
-          const int A0 = la.c0 - rb.c0, A1 = la.c1 - rb.c1;
+        const int2 ri = u.row_info[i], ci = u.col_info[j];  // see SlotInfo
+        const double lr = u.LR[j], ll = u.LL ? u.LL[i] : 1.0;
+        if (ri.x >= 0 && ci.x >= 0) {
+          const int cl0 = (ri.x >> 16) & 255, cl1 = ri.x >> 24;
+          const int cr0 = (ci.x >> 16) & 255, cr1 = ci.x >> 24;
+          const int A0 = cl0 - cr0, A1 = cl1 - cr1;
           if (A0 >= 0 && A1 >= 0) {
-            info = A0 | A1 << 8 | (rb.c0 - ti.lo0) << 16 | (rb.c1 - ti.lo1) << 24;
-            eblk[e] = (a - ti.a_lo) | (b - ti.b_lo) << 16;
-            eoff[e] = (i - la.start) | (j - rb.start) << 16;
-            double wt = u.LR[j];
-            if (u.LL) wt *= u.LL[i];
-            ewt[e] = wt;
+            info = A0 | A1 << 8 | (cr0 - ti.lo0) << 16 | (cr1 - ti.lo1) << 24;
+            eblk[e] = ((ri.x & 0xffff) - ti.a_lo) | ((ci.x & 0xffff) - ti.b_lo) << 16;
+            eoff[e] = ri.y | ci.y << 16;
+            ewt[e] = lr * ll;
           }
         }
       }
@@ -415,8 +429,10 @@ __global__ __launch_bounds__(256) void k_contract_tile(const UpdDesc *__restrict
     if (cb >= 0) {
       scb[base + local] = cb;
       spos[base + local] = threadIdx.x;
-    } else if ((int)threadIdx.x < nbox) {  // empty position: P = 0
-      for (int e = 0; e < E; ++e) P[(int64_t)threadIdx.x * E + e] = cx<T>{(T)0, (T)0};
+    }
+    if (nslot < nbox) {  // empty positions of the union box hold P = 0 (all threads)
+      for (int t = threadIdx.x; t < nbox * E; t += blockDim.x)
+        if (pempty[t / E]) P[t] = cx<T>{(T)0, (T)0};
     }
     __syncthreads();
 

@@ -202,8 +202,8 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
     D.cblk_of = reinterpret_cast<const int32_t *>(pb + h->o_cblk_of);
     D.rowoff = reinterpret_cast<const int32_t *>(pb + h->o_rowoff);
     D.coloff = reinterpret_cast<const int32_t *>(pb + h->o_coloff);
-    D.row_blk = reinterpret_cast<const int32_t *>(pb + h->o_row_blk);
-    D.col_blk = reinterpret_cast<const int32_t *>(pb + h->o_col_blk);
+    D.row_info = reinterpret_cast<const int2 *>(pb + h->o_row_info);
+    D.col_info = reinterpret_cast<const int2 *>(pb + h->o_col_info);
     D.n_lblk = h->n_lblk;
     D.n_rblk = h->n_rblk;
     for (int c = 0; c < kTileClasses; ++c)

@@ -74,7 +74,7 @@ struct PlanHeader {
   double tie_rel;
   // MFMA tile formulation of the contraction (see TileItem)
   int32_t n_titems, tile_off[kTileClasses + 1];
-  int64_t o_titems, o_tau_sector, o_cblk_of, o_rowoff, o_coloff, o_row_blk, o_col_blk;
+  int64_t o_titems, o_tau_sector, o_cblk_of, o_rowoff, o_coloff, o_row_info, o_col_info;
   double flops_weight;  // BS transforms: 8 sum_pairs nL nR B0 B1 (B0 + B1)
   double mfma_k4;       // sum over tile items and their centre blocks of K padded to 4
   double mfma_k32;      // the same with K padded to 32 (BF16x3 steps)
@@ -99,6 +99,13 @@ struct PlanHeader {
 // kTileElems[c].
 constexpr int kMaxD = 16;  // the reference's limit too: 256-entry centre-charge tables
                            // (src/update_MPO_kernel.cu:236-238)
+// Per bond slot of the left (right) bond, for the contraction: x = block
+// index | charge component 0 << 16 | component 1 << 24 (-1 for padding
+// slots), y = position of the slot inside its block.  One load gives the
+// element's charges without a block-table lookup.
+struct SlotInfo {
+  int32_t x, y;
+};
 struct TileItem {
   int32_t r0, c0;
   int8_t nr, nc, s0, rp, lo0, lo1, hi0, hi1;

@@ -46,6 +46,7 @@ struct Built {
   std::vector<int32_t> jac_order;  // sector indices grouped by Jacobi class
   std::vector<TileItem> titems;    // contraction tiles, by LDS class
   std::vector<int32_t> tau_sector, cblk_of, rowoff, coloff, row_blk, col_blk;
+  std::vector<SlotInfo> row_info, col_info;  // device copies of the slot data
 };
 
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
@@ -304,6 +305,14 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
       for (int i = b.rblk[c].start; i < b.rblk[c].end; ++i) b.col_blk[i] = c;
       nRv = std::max(nRv, b.rblk[c].end);
     }
+    b.row_info.assign((size_t)chi, SlotInfo{-1, 0});
+    b.col_info.assign((size_t)chi, SlotInfo{-1, 0});
+    for (int a = 0; a < nl; ++a)
+      for (int i = b.lblk[a].start; i < b.lblk[a].end; ++i)
+        b.row_info[i] = SlotInfo{a | b.lblk[a].c0 << 16 | b.lblk[a].c1 << 24, i - b.lblk[a].start};
+    for (int c = 0; c < nr; ++c)
+      for (int i = b.rblk[c].start; i < b.rblk[c].end; ++i)
+        b.col_info[i] = SlotInfo{c | b.rblk[c].c0 << 16 | b.rblk[c].c1 << 24, i - b.rblk[c].start};
     std::vector<TileItem> cls[kTileClasses];
     const int kLast = kTileElems[kTileClasses - 1];
     struct Pass {
@@ -447,8 +456,8 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
   h.o_cblk_of = place(sizeof(int32_t) * b.cblk_of.size());
   h.o_rowoff = place(sizeof(int32_t) * b.rowoff.size());
   h.o_coloff = place(sizeof(int32_t) * b.coloff.size());
-  h.o_row_blk = place(sizeof(int32_t) * b.row_blk.size());
-  h.o_col_blk = place(sizeof(int32_t) * b.col_blk.size());
+  h.o_row_info = place(sizeof(SlotInfo) * b.row_info.size());
+  h.o_col_info = place(sizeof(SlotInfo) * b.col_info.size());
   h.dev_bytes = o;
   h.blob_bytes = h.dev_begin + h.dev_bytes;
 
@@ -598,8 +607,8 @@ int bsmpo_plan_build(int d, int chi, const int32_t *h_CL, const int32_t *h_CC,
   put(dev, b.h.o_cblk_of, b.cblk_of);
   put(dev, b.h.o_rowoff, b.rowoff);
   put(dev, b.h.o_coloff, b.coloff);
-  put(dev, b.h.o_row_blk, b.row_blk);
-  put(dev, b.h.o_col_blk, b.col_blk);
+  put(dev, b.h.o_row_info, b.row_info);
+  put(dev, b.h.o_col_info, b.col_info);
   return 0;
 }
 
