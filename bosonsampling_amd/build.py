@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbsmpo.so")
 SOURCES = ["plan.cpp", "group.cpp", "capi.cpp", "adjoint.cpp", "update.hip", "contract_tile.hip",
-           "eig.hip", "backtransform.hip", "observables.hip", "adjoint.hip"]
+           "eig.hip", "backtransform.hip", "twostage.hip", "observables.hip", "adjoint.hip"]
 ARCH = os.environ.get("BSMPO_ARCH", "gfx950")
 
 

@@ -3,10 +3,13 @@
 // needs the Gram matrix's eigenvectors; the route reduces G = Q Tri Q^H).
 //
 // The Householder reduction leaves reflectors H_i = I - tau_i v_i v_i^H,
-// i = 0 .. n-2, with v_i zero above row i + 1, one at row i + 1 and the rest
-// in column i of the reduced Gram matrix (below its subdiagonal).  The kept
-// eigenvectors of G are x = H_0 H_1 ... H_{n-2} y for the tridiagonal
-// eigenvectors y.  Reflectors are taken 32 at a time (block m holds
+// i = 0 .. n-1-off, with v_i zero above row i + off, one at row i + off and
+// the rest in column i of the reduced Gram matrix: off = 1 after the one-stage
+// reduction (the reflectors sit below the subdiagonal), off = kBand after the
+// band reduction of the two-stage one (below the band; the bulge-chasing
+// reflectors are applied to the vectors first).  The kept eigenvectors of G
+// are x = H_0 H_1 ... y for the eigenvectors y of the reduced matrix.
+// Reflectors are taken 32 at a time (block m holds
 // i = 32 m .. 32 m + 31) in the compact WY form
 //   H_i0 ... H_i0+31 = I - V T V^H,  T upper triangular,
 //   T[j][j] = tau_j,  T[0:j, j] = -tau_j T[0:j, 0:j] (V[:, 0:j]^H v_j)
@@ -34,18 +37,25 @@ struct WyItem {
 __device__ __forceinline__ WyItem wy_item(int2 r) { return WyItem{r.x, r.y & 0xfffff, r.y >> 20}; }
 
 // V[r][p] of block i0, rows rb .. rb + 31, into Vs[r - rb][p] (coalesced:
-// eight threads read 32 consecutive rows of one reflector column)
+// eight threads read 32 consecutive rows of one reflector column).  Reflector
+// i has its unit entry at row i + off (off = 1 after the one-stage reduction,
+// kBand after the two-stage one) and the rest below it in column i of A.
 __device__ __forceinline__ void wy_load_v(double2 *Vs, gptr<const double2> A, int n, int i0,
-                                          int rb) {
+                                          int rb, int off) {
   const int p = threadIdx.x >> 3, r0 = (threadIdx.x & 7) * 4;
   const int i = i0 + p;
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     const int r = rb + r0 + e;
     double2 v = make_double2(0.0, 0.0);
-    if (i <= n - 2 && r < n && r > i) v = r == i + 1 ? make_double2(1.0, 0.0) : gload2(A, (int64_t)i * n + r);
+    if (i <= n - 1 - off && r < n && r >= i + off)
+      v = r == i + off ? make_double2(1.0, 0.0) : gload2(A, (int64_t)i * n + r);
     Vs[(r0 + e) * kWyLd + p] = v;
   }
+}
+
+__device__ __forceinline__ int wy_offset(const UpdDesc &u, const Sector &s) {
+  return two_stage(s.n, u.ts_min) ? kBand : 1;
 }
 
 // Z rows rb .. rb + 31 of vectors j0 .. j0 + kp - 1 (column-major, length n)
@@ -125,15 +135,15 @@ __global__ __launch_bounds__(256) void k_wy_tmat(const UpdDesc *__restrict__ Ud,
     const UpdDesc &u = Ud[w.u];
     const Sector s = u.sectors[w.sector];
     if (u.kcount[w.sector] == 0) continue;  // uniform: nothing kept, nothing to transform
-    const int n = s.n, i0 = w.idx * kWy;
+    const int n = s.n, i0 = w.idx * kWy, off = wy_offset(u, s);
     const gptr<const double2> A = gp(reinterpret_cast<const double2 *>(u.G) + s.g_off);
     const gptr<const double2> tau = gp(reinterpret_cast<const double2 *>(u.tau) + s.s_off);
-    // S = V^H V over rows i0 + 1 .. n - 1, one 16 x 16 tile per wave
+    // S = V^H V over rows i0 + off .. n - 1, one 16 x 16 tile per wave
     const int tp = wave >> 1, tj = wave & 1;
     dbl4 ar = {0.0, 0.0, 0.0, 0.0}, ai = {0.0, 0.0, 0.0, 0.0};
-    for (int rb = i0 + 1; rb < n; rb += kWyRows) {
+    for (int rb = i0 + off; rb < n; rb += kWyRows) {
       __syncthreads();
-      wy_load_v(Vs, A, n, i0, rb);
+      wy_load_v(Vs, A, n, i0, rb, off);
       __syncthreads();
       wy_vhz(ar, ai, Vs, Vs, tp, tj, lane);
     }
@@ -142,7 +152,7 @@ __global__ __launch_bounds__(256) void k_wy_tmat(const UpdDesc *__restrict__ Ud,
     __syncthreads();
     // forward recurrence, column by column
     for (int j = 0; j < kWy; ++j) {
-      const double2 tj_ = i0 + j <= n - 2 ? gload2(tau, i0 + j) : make_double2(0.0, 0.0);
+      const double2 tj_ = i0 + j <= n - 1 - off ? gload2(tau, i0 + j) : make_double2(0.0, 0.0);
       const int t = threadIdx.x;
       if (t < j) {
         double sr = 0.0, si = 0.0;
@@ -186,17 +196,18 @@ __global__ __launch_bounds__(256) void k_wy_apply(const UpdDesc *__restrict__ Ud
                                       : reinterpret_cast<double2 *>(u.W) + s.w_off);
     const gptr<const double2> A = gp(reinterpret_cast<const double2 *>(u.G) + s.g_off);
     const gptr<const double2> Tw = wy_tstore(u, s);
-    const int nblk = (n - 1 + kWy - 1) / kWy;  // reflectors 0 .. n - 2
+    const int off = wy_offset(u, s);
+    const int nblk = (n - off + kWy - 1) / kWy;  // reflectors 0 .. n - 1 - off
     for (int m = nblk - 1; m >= 0; --m) {
       const int i0 = m * kWy;
       __syncthreads();  // the previous block's readers of Ts / Ws are done
       for (int e = threadIdx.x; e < kWy * kWy; e += blockDim.x)
         Ts[(e / kWy) * kWyLd + e % kWy] = gload2(Tw, (int64_t)m * kWy * kWy + e);
-      // (1) W = V^H Z over rows i0 + 1 .. n - 1
+      // (1) W = V^H Z over rows i0 + off .. n - 1
       dbl4 ar = {0.0, 0.0, 0.0, 0.0}, ai = {0.0, 0.0, 0.0, 0.0};
-      for (int rb = i0 + 1; rb < n; rb += kWyRows) {
+      for (int rb = i0 + off; rb < n; rb += kWyRows) {
         __syncthreads();
-        wy_load_v(Vs, A, n, i0, rb);
+        wy_load_v(Vs, A, n, i0, rb, off);
         wy_load_z(Zs, X, n, j0, kp, rb);
         __syncthreads();
         wy_vhz(ar, ai, Vs, Zs, ta, tb, lane);
@@ -210,9 +221,9 @@ __global__ __launch_bounds__(256) void k_wy_apply(const UpdDesc *__restrict__ Ud
       __syncthreads();
       wy_put(Ws, ar, ai, ta, tb, lane);
       // (3) Z <- Z - V W, chunk by chunk: staged, updated in LDS, written back
-      for (int rb = i0 + 1; rb < n; rb += kWyRows) {
+      for (int rb = i0 + off; rb < n; rb += kWyRows) {
         __syncthreads();
-        wy_load_v(Vs, A, n, i0, rb);
+        wy_load_v(Vs, A, n, i0, rb, off);
         wy_load_z(Zs, X, n, j0, kp, rb);
         __syncthreads();
         ar = dbl4{0.0, 0.0, 0.0, 0.0};

@@ -123,6 +123,7 @@ struct UpdDesc {
   const int32_t *tau_sector, *cblk_of, *rowoff, *coloff;
   const int2 *row_info, *col_info;  // per bond slot, see SlotInfo
   int32_t n_lblk, n_rblk;
+  int32_t ts_min, pad_ts;  // two-stage threshold of the plan (see two_stage)
 };
 
 // Plan arrays as seen by kernels (device copy at the front of the workspace).

@@ -649,7 +649,10 @@ __global__ void k_kcount(const UpdDesc *__restrict__ Ud, const int2 *__restrict_
       const double k = tot, L = s.L, R = s.R, n = s.n;
       atomicAdd(&g_kept_flops[0], 8.0 * k * L * R);
       if (!s.left_small) atomicAdd(&g_kept_flops[1], 8.0 * k * L * R);
-      if (s.n > u.eig_thr) atomicAdd(&g_kept_flops[2], 8.0 * k * n * n);
+      // back-transformation: 8 n^2 per vector (the bulge-chasing reflectors
+      // of a two-stage sector as much again)
+      if (s.n > u.eig_thr)
+        atomicAdd(&g_kept_flops[2], (two_stage(s.n, u.ts_min) ? 16.0 : 8.0) * k * n * n);
     }
   }
 }
@@ -946,17 +949,24 @@ int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
   if (int rc = ensure_lds_limit((const void *)k_tridiag<256, 2>, 512 * 32)) return rc;
   if (int rc = ensure_lds_limit((const void *)k_tridiag<512, 4>, kTriMaxN * 32)) return rc;
   int tk = timing_begin(ST_TRIDIAG, stream);
-  // The list is sorted largest first.  Few sectors (a single update): the
-  // larger block shape for every sector, so that each column chain gets more
-  // threads.  Many sectors (a group across instances): the sectors above the
-  // split (bsmpo_set_tri_split, default 384; the head of the list) take the
-  // 512-thread shape and go first, on the launching stream; the rest take
-  // 256-thread blocks, more per CU, on a side stream forked before the first
-  // launch (fork / join by events), so that they fill the slots the long
-  // column chains leave.
-  const int nbig = g.n_tri < kTriFewSectors ? g.n_tri : g.tri_nbig;
-  const int split = g.n_tri < kTriFewSectors ? kTriMaxN : g.tri_split;
-  const bool both = nbig > 0 && nbig < g.n_tri;
+  // The list holds the two-stage sectors first, then the one-stage ones, each
+  // part largest first.  The two-stage sectors (band reduction, then bulge
+  // chasing) run on the launching stream.  One-stage sectors: with few of them
+  // (a single update), the larger block shape for every sector, so that each
+  // column chain gets more threads; with many (a group across instances), the
+  // sectors above the split (bsmpo_set_tri_split, default 384) take the
+  // 512-thread shape and the rest 256-thread blocks, more per CU.  Work beside
+  // the launching stream's goes to a side stream forked before the first
+  // launch (fork / join by events), so that it fills the slots the long
+  // chains leave.
+  const int n1 = g.n_tri - g.n_ts;
+  const bool few = n1 < kTriFewSectors;
+  const int nbig = few ? n1 : g.tri_nbig;
+  const int split = few ? kTriMaxN : g.tri_split;
+  const bool big_on_side = few && g.n_ts > 0;
+  const bool main_work = g.n_ts > 0 || (nbig > 0 && !big_on_side);
+  const bool side_work = nbig < n1 || (big_on_side && nbig > 0);
+  const bool both = main_work && side_work;
   SideStream *ss = nullptr;
   if (both) {
     int rc = side_stream(stream, &ss);
@@ -964,16 +974,18 @@ int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
     BSMPO_HIP(hipEventRecord(ss->fork, stream));
     BSMPO_HIP(hipStreamWaitEvent(ss->stream, ss->fork, 0));
   }
-  const int2 *tl = glist(w, g.o_tri);
+  const hipStream_t sst = both ? ss->stream : stream;
+  if (int rc = launch_band(g, w, stream)) return rc;
+  const int2 *tl = glist(w, g.o_tri) + g.n_ts;
   if (nbig > 0) {
     hipLaunchKernelGGL((k_tridiag<512, 4>), dim3(nbig), dim3(512), (size_t)g.tri_max_n * 32,
-                       stream, gdesc(g, w), tl, 1, kTriMaxN);
+                       big_on_side ? sst : stream, gdesc(g, w), tl, 1, kTriMaxN);
     BSMPO_HIP(hipGetLastError());
   }
-  if (nbig < g.n_tri) {
-    hipLaunchKernelGGL((k_tridiag<256, 2>), dim3(g.n_tri - nbig), dim3(256),
-                       (size_t)std::min(split, g.tri_max_n) * 32,
-                       both ? ss->stream : stream, gdesc(g, w), tl + nbig, 1, split);
+  if (nbig < n1) {
+    hipLaunchKernelGGL((k_tridiag<256, 2>), dim3(n1 - nbig), dim3(256),
+                       (size_t)std::min(split, g.tri_max_n) * 32, sst, gdesc(g, w), tl + nbig, 1,
+                       split);
     BSMPO_HIP(hipGetLastError());
   }
   if (both) {
@@ -1011,6 +1023,7 @@ int launch_tri_vectors(const GroupHeader &g, const char *w, hipStream_t stream) 
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, 0.0, 0.0);
   tk = timing_begin(ST_BACKTRANS, stream);
+  if (int rc = launch_bulge_apply(g, w, stream)) return rc;  // Q2 first (two-stage sectors)
   if (int rc = launch_wy_backtransform(g, w, stream)) return rc;
   timing_end(tk, stream, 0.0, 0.0);
   return 0;

@@ -90,16 +90,20 @@ int layout(int n, const void *const *plans, Layout &L) {
   g.o_w = place(8 * n_w);
   // back-transformation items of the tridiagonal-route sectors: one per WY
   // block of reflectors and one per panel of (at most) kWyBlock kept vectors
-  int64_t n_wyt = 0, n_wya = 0;
+  int64_t n_wyt = 0, n_wya = 0, n_bc = 0;
   for (int u = 0; u < n; ++u) {
     n_wyt += header(plans[u])->n_wyt;
     n_wya += header(plans[u])->n_wya;
+    n_bc += header(plans[u])->n_bc;
   }
-  if (n_wyt > INT32_MAX || n_wya > INT32_MAX) return fail(-1, "group: too many work items");
+  if (n_wyt > INT32_MAX || n_wya > INT32_MAX || n_bc > INT32_MAX)
+    return fail(-1, "group: too many work items");
   g.n_wyt = (int32_t)n_wyt;
   g.n_wya = (int32_t)n_wya;
+  g.n_bc = (int32_t)n_bc;
   g.o_wyt = place(8 * n_wyt);
   g.o_wya = place(8 * n_wya);
+  g.o_bc = place(8 * n_bc);
   g.o_plans = o;
   for (int u = 0; u < n; ++u) o += region(header(plans[u])->dev_bytes);
   g.staging_bytes = o;
@@ -206,6 +210,7 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
     D.col_info = reinterpret_cast<const int2 *>(pb + h->o_col_info);
     D.n_lblk = h->n_lblk;
     D.n_rblk = h->n_rblk;
+    D.ts_min = h->ts_min;
     for (int c = 0; c < kTileClasses; ++c)
       for (int i = h->tile_off[c]; i < h->tile_off[c + 1]; ++i) tiles[tile_fill[c]++] = make_int2(u, i);
     if (contract_only) continue;  // the SVD-stage lists stay empty (zero)
@@ -217,37 +222,52 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
     for (int i = h->jac_off[kTriClass]; i < h->jac_off[kTriClass + 1]; ++i) tri[nt++] = make_int2(u, jo[i]);
   }
   if (contract_only) {  // a group that only runs the contraction stage
-    g.n_gram = g.n_tri = g.n_sec = g.n_v = g.n_w = g.n_wyt = g.n_wya = 0;
+    g.n_gram = g.n_tri = g.n_sec = g.n_v = g.n_w = g.n_wyt = g.n_wya = g.n_bc = 0;
     for (int c = 0; c <= kTriClass; ++c) g.jac_off[c] = 0;
     std::memcpy(host_blob, &g, sizeof(GroupHeader));
     return 0;
   }
-  // Tridiagonal-route sectors largest first across the group: blocks are
-  // dispatched in list order, so the longest column chains start at once and
-  // the short ones fill the slots they leave (longest-processing-time order).
+  // Tridiagonal-route sectors: the two-stage ones first, then the one-stage
+  // ones, each part largest first across the group: blocks are dispatched in
+  // list order, so the longest chains start at once and the short ones fill
+  // the slots they leave (longest-processing-time order).
   {
     auto size_of = [&](int2 r) {
       return host_array<Sector>(plans[r.x], header(plans[r.x])->o_sectors)[r.y].n;
     };
-    std::stable_sort(tri, tri + nt, [&](int2 x, int2 y) { return size_of(x) > size_of(y); });
-    auto count_above = [&](int m) {
-      int c = 0;
-      while (c < nt && size_of(tri[c]) > m) ++c;
-      return c;
-    };
+    auto ts_of = [&](int2 r) { return two_stage(size_of(r), header(plans[r.x])->ts_min); };
+    std::stable_sort(tri, tri + nt, [&](int2 x, int2 y) {
+      const bool tx = ts_of(x), ty = ts_of(y);
+      return tx != ty ? tx : size_of(x) > size_of(y);
+    });
+    int n_ts = 0;
+    while (n_ts < nt && ts_of(tri[n_ts])) ++n_ts;
+    g.n_ts = n_ts;
+    g.ts_max_n = n_ts > 0 ? size_of(tri[0]) : 0;
+    g.ts_nbig = 0;
+    while (g.ts_nbig < n_ts && size_of(tri[g.ts_nbig]) - kBand > kBandThreads) ++g.ts_nbig;
     g.tri_split = tri_split();
-    g.tri_nbig = count_above(g.tri_split);
+    g.tri_nbig = 0;  // one-stage sectors above the block-shape split
+    while (n_ts + g.tri_nbig < nt && size_of(tri[n_ts + g.tri_nbig]) > g.tri_split) ++g.tri_nbig;
     // back-transformation items in the same order (largest sectors first)
     int2 *wyt = reinterpret_cast<int2 *>(img + g.o_wyt);
     int2 *wya = reinterpret_cast<int2 *>(img + g.o_wya);
-    int64_t a = 0, b = 0;
+    int2 *bc = reinterpret_cast<int2 *>(img + g.o_bc);
+    int64_t a = 0, b = 0, cc = 0;
     for (int64_t t = 0; t < nt; ++t) {
       const int nn = size_of(tri[t]);
-      for (int m = 0; m < (nn - 1 + kWyBlock - 1) / kWyBlock; ++m)
+      const bool ts = t < n_ts;
+      const int off = ts ? kBand : 1;  // row offset of reflector i's unit entry
+      for (int m = 0; m < (nn - off + kWyBlock - 1) / kWyBlock; ++m)
         wyt[a++] = make_int2(tri[t].x, tri[t].y | m << 20);
       for (int p = 0; p < (nn + kWyBlock - 1) / kWyBlock; ++p)
         wya[b++] = make_int2(tri[t].x, tri[t].y | p << 20);
+      if (ts)
+        for (int q = 0; q < (nn + bc_vectors(nn) - 1) / bc_vectors(nn); ++q)
+          bc[cc++] = make_int2(tri[t].x, tri[t].y | q << 20);
     }
+    g.n_wyt = (int32_t)a;
+    g.n_bc = (int32_t)cc;
   }
   // LDS Jacobi list grouped by size class across updates
   int64_t nj = 0;

@@ -36,7 +36,23 @@ constexpr int kTriClass = kJacClasses - 1;
 constexpr int kTriPanel = 8;  // column panel of the blocked Householder reduction
 constexpr int kTriVecThreads = 128;  // inverse-iteration threads per sector
 constexpr int kWyBlock = 32;  // back-transformation: reflectors per WY block, vectors per panel
+// Two-stage reduction of the large tridiagonal-route sectors: dense -> band of
+// width kBand (panels on the FP64 MFMA) -> tridiagonal (bulge chasing).
+constexpr int kBand = 16;
+constexpr int kBandThreads = 512;                // stage-1 block; panel rows in registers
+constexpr int kBandMaxN = kBand + kBandThreads;  // one panel row per thread
+constexpr int kTwoStageMinN = 4 * kBand;           // smallest threshold accepted
+__host__ __device__ inline bool two_stage(int n, int ts_min) {
+  return ts_min > 0 && n >= ts_min && n <= kBandMaxN;
+}
+// kept vectors per work item of the bulge-chasing back-transformation
+__host__ __device__ inline int bc_vectors(int n) {
+  int nv = 16;
+  while (nv > 1 && nv * n > 2048) nv >>= 1;
+  return nv;
+}
 int eig_threshold();  // sectors with n > threshold take the tridiagonal route
+int two_stage_min();  // tridiagonal-route sectors with n >= this take the two-stage reduction
 int tri_split();      // many-sector groups: Householder blocks of 512 threads for n > split
 double tie_rel();     // relative width of the tie band at the chi-th singular value
 
@@ -81,6 +97,9 @@ struct PlanHeader {
   // back-transformation items of the tridiagonal-route sectors (WY blocks of
   // reflectors, panels of kept vectors); layouts read only header fields
   int32_t n_wyt, n_wya;
+  // two-stage threshold in effect, and the bulge-chasing back-transformation
+  // items (chunks of kept vectors) of the two-stage sectors
+  int32_t ts_min, n_bc;
 };
 
 // Tile formulation of the sector contraction.  A TileItem is one pass over
@@ -166,6 +185,11 @@ struct GroupHeader {
   int32_t tri_split, tri_nbig;  // the block-shape split in effect; sectors with n > tri_split
   int32_t n_wyt, n_wya;         // back-transformation items: (sector, block), (sector, panel)
   int64_t o_wyt, o_wya;
+  // two-stage sectors: the head of the tridiagonal list (n_ts of them, largest
+  // first; ts_nbig with two panel rows per thread), and their bulge-chasing
+  // back-transformation items (sector | chunk << 20)
+  int32_t n_ts, ts_nbig, n_bc, ts_max_n;
+  int64_t o_bc;
 };
 constexpr int64_t kGroupHostHead = 1024;  // host blob: header, then the staging image
 static_assert(sizeof(GroupHeader) <= kGroupHostHead, "group header too large");
@@ -208,6 +232,8 @@ int launch_contract(const GroupHeader &g, const char *d_work, int precision, hip
 int launch_svd(const GroupHeader &g, const char *d_work, hipStream_t stream);
 int launch_truncate(const GroupHeader &g, const char *d_work, hipStream_t stream);
 int launch_wy_backtransform(const GroupHeader &g, const char *d_work, hipStream_t stream);
+int launch_band(const GroupHeader &g, const char *d_work, hipStream_t stream);
+int launch_bulge_apply(const GroupHeader &g, const char *d_work, hipStream_t stream);
 
 // group construction (group.cpp)
 struct UpdateIO {

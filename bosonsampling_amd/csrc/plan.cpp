@@ -23,7 +23,9 @@ static thread_local std::string g_last_error;
 static std::atomic<int> g_eig_threshold{96};
 static std::atomic<int> g_tri_split{384};
 static std::atomic<double> g_tie_rel{1e-6};
+static std::atomic<int> g_two_stage{0};
 int eig_threshold() { return g_eig_threshold; }
+int two_stage_min() { return g_two_stage; }
 int tri_split() { return g_tri_split; }
 double tie_rel() { return g_tie_rel; }
 
@@ -213,6 +215,8 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
     for (int i = 0; i < b.sectors[sid].n; ++i) b.sv_sector.push_back(sid);
   b.h.eig_threshold = thr;
   b.h.tie_rel = tie_rel();
+  const int tsm = two_stage_min();
+  b.h.ts_min = tsm;
   int64_t tw = 0;
   for (int c = 0; c < kJacClasses; ++c) {
     b.h.jac_off[c] = (int32_t)b.jac_order.size();
@@ -226,7 +230,22 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
       if (in) {
         b.jac_order.push_back(sid);
         b.h.jac_max_n[c] = std::max(b.h.jac_max_n[c], n);
-        if (c == kTriClass) {
+        if (c == kTriClass && two_stage(n, tsm)) {
+          // band reduction: the lower triangle read and the full matrix
+          // written once (scaling, mirror); per panel of kBand columns the
+          // full trailing matrix read for Y = A22 V and read + written by the
+          // two-sided update (MFMA, 8 flops per complex MAC); the band copy.
+          // Bulge chasing: ~n^2 / (2 kBand) steps of ~4 kBand^2 complex MACs.
+          const double nn = n;
+          b.h.bytes_tridiag += 8.0 * nn * (nn + 1) + 16.0 * nn * nn + 16.0 * (kBand + 1 + 2 * kBand) * nn;
+          for (int j0 = 0; n - j0 - kBand >= 2; j0 += kBand) {
+            const double m = n - j0 - kBand;
+            b.h.bytes_tridiag += 48.0 * m * m + 2.0 * 16.0 * kBand * m;
+            b.h.flops_tridiag += 8.0 * (3.0 * kBand * m * m + 4.0 * kBand * kBand * m);
+          }
+          b.h.flops_tridiag += 8.0 * 4.0 * kBand * kBand * nn * nn / (2.0 * kBand);
+          b.h.n_bc += (n + bc_vectors(n) - 1) / bc_vectors(n);
+        } else if (c == kTriClass) {
           const double nn = n;
           b.h.flops_tridiag += 16.0 / 3.0 * nn * nn * nn;
           // lower triangle of the trailing matrix (m (m + 1) / 2 complex entries)
@@ -236,6 +255,8 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
             b.h.bytes_tridiag += 8.0 * m * (m + 1);
             if (j % kTriPanel == kTriPanel - 1 || j + 2 == n) b.h.bytes_tridiag += 16.0 * m * (m + 1);
           }
+        }
+        if (c == kTriClass) {
           sc.tw_off = tw;
           // per-lane inverse-iteration arrays (6 n doubles for each of the
           // min(n, kTriVecThreads) lanes of the sector's block); the Householder
@@ -246,7 +267,12 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
           b.h.n_wyt += nblk;
           b.h.n_wya += (n + kWyBlock - 1) / kWyBlock;
           const int64_t wy = 2 * (int64_t)kWyBlock * kWyBlock * nblk;
-          tw += std::max<int64_t>((int64_t)n * std::max(6 * std::min(n, kTriVecThreads), 4 * kTriPanel), wy);
+          // two-stage: the band (2 kBand n complex) and the panel's V, Y, W
+          // (kBand n complex each) before the inverse iteration
+          const int64_t band = two_stage(n, tsm) ? 2 * (int64_t)(2 * kBand + 3 * kBand) * n : 0;
+          tw += std::max<int64_t>(
+              std::max<int64_t>((int64_t)n * std::max(6 * std::min(n, kTriVecThreads), 4 * kTriPanel), wy),
+              band);
         }
       }
     }
@@ -488,7 +514,7 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
 // The last plan built on this thread, so that query + build cost one build.
 struct Cached {
   bool valid = false;
-  int d = 0, chi = 0, t0 = 0, t1 = 0, thr = 0;
+  int d = 0, chi = 0, t0 = 0, t1 = 0, thr = 0, tsm = 0;
   double tie = 0.0;
   std::vector<int32_t> key;
   Built b;
@@ -500,7 +526,8 @@ int build_cached(int d, int chi, const int32_t *CL, const int32_t *CC, const int
   Cached &c = t_cache;
   const size_t n = (size_t)2 * chi;
   bool hit = c.valid && c.d == d && c.chi == chi && c.t0 == t0 && c.t1 == t1 &&
-             c.thr == eig_threshold() && c.tie == tie_rel() && c.key.size() == 3 * n &&
+             c.thr == eig_threshold() && c.tsm == two_stage_min() && c.tie == tie_rel() &&
+             c.key.size() == 3 * n &&
              !std::memcmp(c.key.data(), CL, n * 4) && !std::memcmp(c.key.data() + n, CC, n * 4) &&
              !std::memcmp(c.key.data() + 2 * n, CR, n * 4);
   if (!hit) {
@@ -513,6 +540,7 @@ int build_cached(int d, int chi, const int32_t *CL, const int32_t *CC, const int
     std::memcpy(c.key.data() + n, CC, n * 4);
     std::memcpy(c.key.data() + 2 * n, CR, n * 4);
     c.d = d; c.chi = chi; c.t0 = t0; c.t1 = t1; c.thr = eig_threshold(); c.tie = tie_rel();
+    c.tsm = two_stage_min();
     c.valid = true;
   }
   *out = &c.b;
@@ -553,6 +581,14 @@ int bsmpo_set_tie_rel(double rel) {
 int bsmpo_set_eig_threshold(int n) {
   if (n < 0 || n > 96) return fail(-1, "eig threshold must be in [0, 96]");
   g_eig_threshold = n;
+  return 0;
+}
+
+int bsmpo_two_stage_threshold(void) { return g_two_stage; }
+
+int bsmpo_set_two_stage(int n) {
+  if (n < 0 || n > kBandMaxN) return fail(-1, "two-stage threshold must be in [0, 528]");
+  g_two_stage = n == 0 ? 0 : std::max(n, kTwoStageMinN);
   return 0;
 }
 

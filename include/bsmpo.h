@@ -58,6 +58,20 @@ int bsmpo_set_eig_threshold(int n);
 int bsmpo_set_tri_split(int n);
 
 /*
+ * Two-stage reduction threshold (process-wide, affects plans built
+ * afterwards): tridiagonal-route Gram matrices with threshold <= n <= 528 are
+ * reduced to a band of width 16 (panels of 16 columns, FP64 MFMA), then to
+ * tridiagonal form by bulge chasing, instead of the one-stage Householder
+ * reduction (which reads the trailing matrix once per column).  0 turns it
+ * off; other values are raised to at least 64.  Allowed [0, 528].  Same
+ * results to rounding.  No counterpart in the reference (cuSOLVER gesvd per
+ * sector, Rank.py:123).
+ */
+int bsmpo_set_two_stage(int n);
+/* The two-stage threshold in effect (0: off). */
+int bsmpo_two_stage_threshold(void);
+
+/*
  * Truncation tie band (process-wide, affects plans built afterwards): the
  * kept set is every singular value above t (1 + rel), t the chi-th largest,
  * then the values within [t (1 - rel), t (1 + rel)] in concatenated order

@@ -151,3 +151,47 @@ def test_plan_query_with_every_sector_on_the_tridiagonal_route():
         check(lib().bsmpo_set_eig_threshold(96), "set_eig_threshold")
     c_bytes = 16 * int(p.info().c_elems)
     assert 2 * c_bytes < sizes[0] < 1000 * c_bytes and sizes[96] < 1000 * c_bytes
+
+
+def test_two_stage_knob():
+    """bsmpo_set_two_stage: 0 turns the two-stage reduction off, other values
+    are raised to at least 64, values outside [0, 528] are rejected and leave
+    the setting alone."""
+    from bosonsampling_amd import _capi
+    L = _capi.lib()
+    prev = L.bsmpo_two_stage_threshold()
+    try:
+        assert L.bsmpo_set_two_stage(0) == 0 and L.bsmpo_two_stage_threshold() == 0
+        assert L.bsmpo_set_two_stage(10) == 0 and L.bsmpo_two_stage_threshold() == 64
+        assert L.bsmpo_set_two_stage(200) == 0 and L.bsmpo_two_stage_threshold() == 200
+        assert L.bsmpo_set_two_stage(-1) < 0 and L.bsmpo_set_two_stage(529) < 0
+        assert L.bsmpo_two_stage_threshold() == 200
+    finally:
+        L.bsmpo_set_two_stage(prev)
+
+
+def test_two_stage_plan_query_on_large_sectors():
+    """Planning a d=13, chi=512 update with the two-stage reduction on: the
+    same sectors as with it off (the route changes only the eigensolver), and
+    a workspace at least as large (the band and panel arrays share the
+    tridiagonal work area)."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from synth import synthetic_update
+    from bosonsampling_amd import _capi
+    from bosonsampling_amd.engine import Plan
+    L = _capi.lib()
+    prev = L.bsmpo_two_stage_threshold()
+    CL, CC, CR, *_ = synthetic_update(13, 512, (4, 12), (2, 10), (0, 8), 11)
+    try:
+        L.bsmpo_set_two_stage(0)
+        p0 = Plan(13, 512, CL, CC, CR)
+        L.bsmpo_set_two_stage(64)
+        p1 = Plan(13, 512, CL, CC, CR)
+    finally:
+        L.bsmpo_set_two_stage(prev)
+    s0, s1 = p0.sectors()[0], p1.sectors()[0]
+    assert np.array_equal(s0, s1)
+    n = np.minimum(s1[:, 2], s1[:, 3])
+    assert ((n >= 64) & (n <= 528)).sum() > 100
+    assert p1.work_bytes >= p0.work_bytes
