@@ -75,6 +75,9 @@ def parse():
                     help="time the eigensolver route split on the last warmup layer (grouped path)")
     ap.add_argument("--eig-threshold", type=int, default=-1,
                     help="fixed eigensolver route split for both paths instead of the autotuned one")
+    ap.add_argument("--two-stage", type=int, default=-1,
+                    help="two-stage reduction threshold (bsmpo_set_two_stage; 0 = off, "
+                         "-1 = the library default)")
     ap.add_argument("--verbose", action="store_true")
     return ap.parse_args()
 
@@ -168,6 +171,9 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a HIP device")
     prec = {"c128": engine.C128, "c64": engine.C64, "bf16x3": engine.BF16X3}[args.precision]
+    if args.two_stage >= 0:
+        _capi.check(_capi.lib().bsmpo_set_two_stage(args.two_stage), "set_two_stage")
+    two_stage = int(_capi.lib().bsmpo_two_stage_threshold())
     if args.copy_probe:
         # one elementwise kernel (torch's scalar multiply) that streams n bytes in and n out
         n = args.copy_probe << 20
@@ -328,7 +334,8 @@ def main():
                                     "plain_start_end": p["bond"] if p else None},
             "grouped": ({"instances": B, "precision": args.precision, "updates": g["updates"],
                          "updates_per_s": g["rate"], "autotune": g.get("tuned"),
-                         "launch_streams": args.streams} if g else None),
+                         "launch_streams": args.streams, "two_stage_min_n": two_stage}
+                        if g else None),
             "plain_fp64": ({"instances": 1, "updates": p["updates"],
                             "updates_per_s": p["rate"], "autotune": p.get("tuned")} if p else None),
             "grouped_over_plain": (g["rate"] / p["rate"]) if (g and p) else None,

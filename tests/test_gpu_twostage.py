@@ -103,13 +103,20 @@ def test_two_stage_matches_oracle_and_one_stage(gpu, two_stage):
 
 @pytest.mark.timeout(600)
 def test_two_stage_is_scale_invariant(gpu, two_stage):
+    """Every sector on the tridiagonal route (threshold 0; the Jacobi route
+    works on the unscaled Gram matrix in LDS), the large ones two-stage."""
     import torch
+    from bosonsampling_amd._capi import check, lib
     from golden.make_golden import synth_bs
     d, chi, c = 13, 512, 1e-50
     inp = _synthetic(chi, seed=12)
     U3 = torch.from_numpy(synth_bs(d)).to(gpu)
-    G1, L1, ch1 = _run(d, chi, inp, U3, gpu)
-    Gc, Lc, chc = _run(d, chi, inp, U3, gpu, scale=c)
+    check(lib().bsmpo_set_eig_threshold(0), "set_eig_threshold")
+    try:
+        G1, L1, ch1 = _run(d, chi, inp, U3, gpu)
+        Gc, Lc, chc = _run(d, chi, inp, U3, gpu, scale=c)
+    finally:
+        check(lib().bsmpo_set_eig_threshold(96), "set_eig_threshold")
     assert np.array_equal(ch1[1], chc[1])
     assert np.all(np.isfinite(Lc[0])) and np.all(np.isfinite(Gc[0]))
     assert np.abs(Lc[0] / c ** 2 - L1[0]).max() <= TOL * L1[0].max()
