@@ -110,6 +110,7 @@ struct UpdDesc {
   const int32_t *charges, *sv_sector;
   double *C, *T, *G, *V, *W, *S, *lam, *td, *te, *tau, *triwork;
   int32_t *flag, *slot, *kcount;
+  int32_t *hist;  // kHistBins counts, then the cutoff (a double)
   double *Glc, *Gcr;
   const double *LR, *U3, *LL, *LC;
   int32_t *charge_c;

@@ -518,12 +518,29 @@ __device__ double bisect_kth(const double *d, const double *e2, int n, int j, do
 // eigenvalue) stops there.  The largest eigenvalue itself is bisected once per
 // block, by the thread that owns index 0 in the first block and by thread 0 in
 // the others.
+//   pass 0 (grid (sectors, 1)): the sample counts only, added to the update's
+//          histogram of singular values: the count in [x_m, x_(m-1)) goes to
+//          the bin of sqrt(x_m unscale) when x_m is above the drop threshold's
+//          bound (those values are surely kept by the drop rule)
+//   pass 1: the eigenvalues; those whose bracket lies below half the update's
+//          cutoff (no value there can be among the chi kept, see k_cutoff) are
+//          not refined: flagged dropped, value the bracket midpoint
 constexpr int kBisSamples = 256;
 constexpr double kBisOctaves = 0.25;
+
+__device__ __forceinline__ int hist_bin(double sv) {  // -1: below the histogram
+  const int q = (int)floor(4.0 * log2(sv));
+  return q < -kHistZero ? -1 : min(q + kHistZero, kHistBins - 1);
+}
+__device__ __forceinline__ double hist_cut(const UpdDesc &u) {
+  return *reinterpret_cast<const double *>(u.hist + kHistBins);
+}
+
 __global__ __launch_bounds__(256) void k_bisect(const UpdDesc *__restrict__ Ud,
-                                                const int2 *__restrict__ list) {
+                                                const int2 *__restrict__ list, int pass) {
   extern __shared__ double bsm[];
-  __shared__ double sh_lo, sh_hi, sh_piv, sh_lmax;
+  __shared__ double sh_lmax;
+  __shared__ double red[3][4];
   __shared__ double xs[kBisSamples + 1];
   __shared__ int cs[kBisSamples + 1];
   const int2 ref = list[blockIdx.x];
@@ -535,26 +552,42 @@ __global__ __launch_bounds__(256) void k_bisect(const UpdDesc *__restrict__ Ud,
   double *lam = u.lam, *S = u.S;
   int32_t *flags = u.flag;
   double *ds = bsm, *e2s = bsm + n;
+  // Gershgorin interval and pivot guard, by a block reduction
+  double glo = INFINITY, ghi = -INFINITY, emax2 = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    ds[i] = d[i];
-    e2s[i] = i + 1 < n ? e[i] * e[i] : 0.0;
+    const double di = d[i], ei = i + 1 < n ? e[i] : 0.0;
+    const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + fabs(ei);
+    ds[i] = di;
+    e2s[i] = ei * ei;
+    glo = fmin(glo, di - r);
+    ghi = fmax(ghi, di + r);
+    emax2 = fmax(emax2, ei * ei);
   }
-  if (threadIdx.x == 0) {
-    // Gershgorin interval and pivot guard
-    double lo = d[0], hi = d[0], emax2 = 0.0;
-    for (int i = 0; i < n; ++i) {
-      const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0);
-      lo = fmin(lo, d[i] - r);
-      hi = fmax(hi, d[i] + r);
-      if (i + 1 < n) emax2 = fmax(emax2, e[i] * e[i]);
-    }
-    const double span = fmax(fabs(lo), fabs(hi));
-    sh_lo = lo - (4.0 * DBL_EPSILON * span + DBL_MIN);
-    sh_hi = hi + (4.0 * DBL_EPSILON * span + DBL_MIN);
-    sh_piv = DBL_MIN * fmax(1.0, emax2);
+  for (int o = 32; o >= 1; o >>= 1) {
+    glo = fmin(glo, __shfl_xor(glo, o));
+    ghi = fmax(ghi, __shfl_xor(ghi, o));
+    emax2 = fmax(emax2, __shfl_xor(emax2, o));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = glo;
+    red[1][threadIdx.x >> 6] = ghi;
+    red[2][threadIdx.x >> 6] = emax2;
   }
   __syncthreads();
-  const double lo = sh_lo, hi = sh_hi, pivmin = sh_piv;
+  glo = red[0][0];
+  ghi = red[1][0];
+  emax2 = red[2][0];
+#pragma unroll
+  for (int w = 1; w < 4; ++w) {
+    glo = fmin(glo, red[0][w]);
+    ghi = fmax(ghi, red[1][w]);
+    emax2 = fmax(emax2, red[2][w]);
+  }
+  const double span = fmax(fabs(glo), fabs(ghi));
+  const double lo = glo - (4.0 * DBL_EPSILON * span + DBL_MIN);
+  const double hi = ghi + (4.0 * DBL_EPSILON * span + DBL_MIN);
+  const double pivmin = DBL_MIN * fmax(1.0, emax2);
+  const double unscale = e[n - 1];  // 0: a zero matrix
   {  // sample counts: xs[m] = hi 2^(-m / 4), m < 256, decreasing; xs[256] = lo
     const int m = threadIdx.x;
     const double x = hi * exp2(-kBisOctaves * m);
@@ -566,6 +599,16 @@ __global__ __launch_bounds__(256) void k_bisect(const UpdDesc *__restrict__ Ud,
     }
   }
   __syncthreads();
+  if (pass == 0) {
+    const int m = threadIdx.x + 1;  // eigenvalues in [xs[m], xs[m - 1])
+    const int cnt = cs[m - 1] - cs[m];
+    const double x = xs[m];
+    if (cnt > 0 && unscale > 0.0 && x > 0.0 && x > kDropRel2Tri * hi) {
+      const int b = hist_bin(sqrt(x * unscale));
+      if (b >= 0) atomicAdd(&u.hist[b], cnt);
+    }
+    return;
+  }
   // a lower bound of the largest eigenvalue: the largest sample with an
   // eigenvalue at or above it
   double lmax_lb = 0.0;
@@ -588,15 +631,22 @@ __global__ __launch_bounds__(256) void k_bisect(const UpdDesc *__restrict__ Ud,
     blo = fmax(xs[m + 1], lo);
     bhi = xs[m];
   };
+  const double tcut = 0.5 * hist_cut(u);  // in singular-value units
   const int j = blockIdx.y * blockDim.x + threadIdx.x;
   double l = 0.0;
+  bool refined = false;
   if (j < n) {
     double blo, bhi;
     bracket(j, blo, bhi);
-    l = bisect_kth(ds, e2s, n, j, blo, bhi, pivmin, kDropRel2Tri * lmax_lb);
+    if (tcut > 0.0 && !(sqrt(fmax(bhi, 0.0) * unscale) >= tcut)) {
+      l = 0.5 * (blo + bhi);  // below every kept value: not refined
+    } else {
+      l = bisect_kth(ds, e2s, n, j, blo, bhi, pivmin, kDropRel2Tri * lmax_lb);
+      refined = true;
+    }
   }
   if (threadIdx.x == 0) {
-    if (blockIdx.y == 0) {
+    if (blockIdx.y == 0 && refined) {
       sh_lmax = l;
     } else {
       double blo, bhi;
@@ -606,11 +656,65 @@ __global__ __launch_bounds__(256) void k_bisect(const UpdDesc *__restrict__ Ud,
   }
   __syncthreads();
   if (j >= n) return;
-  const double lmax = sh_lmax, unscale = e[n - 1];  // 0: a zero matrix
-  const bool keep = unscale > 0.0 && lmax > 0.0 && l > kDropRel2Tri * lmax;
+  const double lmax = sh_lmax;
+  const bool keep = refined && unscale > 0.0 && lmax > 0.0 && l > kDropRel2Tri * lmax;
   lam[s.s_off + j] = l;  // of the scaled matrix, as the inverse iteration uses it
   S[s.s_off + j] = keep ? sqrt(l * unscale) : 0.0;
   flags[s.s_off + j] = keep ? 1 : 0;
+}
+
+// the histogram of every update of the group, and its cutoff, cleared
+__global__ void k_hist_zero(const UpdDesc *__restrict__ Ud) {
+  int32_t *h = Ud[blockIdx.x].hist;
+  for (int i = threadIdx.x; i < kHistBins + 2; i += blockDim.x) h[i] = 0;
+}
+
+// the kept (flagged) singular values of the Jacobi-route sectors, exact,
+// added to their update's histogram: one block per sector
+__global__ void k_hist_jacobi(const UpdDesc *__restrict__ Ud, const int2 *__restrict__ list) {
+  const int2 ref = list[blockIdx.x];
+  const UpdDesc &u = Ud[ref.x];
+  const Sector s = u.sectors[ref.y];
+  for (int r = threadIdx.x; r < s.n; r += blockDim.x) {
+    if (!u.flag[s.s_off + r]) continue;
+    const double sv = u.S[s.s_off + r];
+    if (!(sv > 0.0)) continue;
+    const int b = hist_bin(sv);
+    if (b >= 0) atomicAdd(&u.hist[b], 1);
+  }
+}
+
+// per update: the cutoff t = 2^((b - kHistZero) / 4) for the highest bin b
+// with at least chi surely-kept values at or above t (each value counted in a
+// bin is at least that bin's bound), 0 if there is none.  The chi-th largest
+// kept singular value is then >= t, so nothing below t (1 - tie band) is
+// kept; pass 1 of k_bisect refines every eigenvalue above t / 2.
+__global__ __launch_bounds__(256) void k_cutoff(const UpdDesc *__restrict__ Ud) {
+  __shared__ int part[256];
+  const UpdDesc &u = Ud[blockIdx.x];
+  constexpr int kPer = kHistBins / 256;
+  int a = 0;
+  for (int i = 0; i < kPer; ++i) a += u.hist[threadIdx.x * kPer + i];
+  part[threadIdx.x] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int chi = u.chi;
+    int cum = 0, b = -1;
+    for (int c = 255; c >= 0 && b < 0; --c) {
+      if (cum + part[c] >= chi) {
+        for (int i = kPer - 1; i >= 0; --i) {
+          cum += u.hist[c * kPer + i];
+          if (cum >= chi) {
+            b = c * kPer + i;
+            break;
+          }
+        }
+      } else {
+        cum += part[c];
+      }
+    }
+    *reinterpret_cast<double *>(u.hist + kHistBins) = b >= 0 ? exp2(0.25 * (b - kHistZero)) : 0.0;
+  }
 }
 
 // per-sector count of kept singular values (a prefix of the sorted list)
@@ -994,8 +1098,22 @@ int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
   }
   timing_end(tk, stream, g.flops_tridiag, g.bytes_tridiag);
   tk = timing_begin(ST_BISECT, stream);
+  // the update's cutoff from sample counts (tridiagonal route) and exact
+  // values (Jacobi route), then the eigenvalues above half of it
+  hipLaunchKernelGGL(k_hist_zero, dim3(g.n_upd), dim3(256), 0, stream, gdesc(g, w));
+  BSMPO_HIP(hipGetLastError());
+  const int njac = g.jac_off[kTriClass];
+  if (njac > 0) {
+    hipLaunchKernelGGL(k_hist_jacobi, dim3(njac), dim3(64), 0, stream, gdesc(g, w), glist(w, g.o_jac));
+    BSMPO_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_bisect, dim3(g.n_tri, 1), dim3(256), (size_t)g.tri_max_n * 16, stream,
+                     gdesc(g, w), glist(w, g.o_tri), 0);
+  BSMPO_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_cutoff, dim3(g.n_upd), dim3(256), 0, stream, gdesc(g, w));
+  BSMPO_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_bisect, dim3(g.n_tri, (g.tri_max_n + 255) / 256), dim3(256),
-                     (size_t)g.tri_max_n * 16, stream, gdesc(g, w), glist(w, g.o_tri));
+                     (size_t)g.tri_max_n * 16, stream, gdesc(g, w), glist(w, g.o_tri), 1);
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, 0.0, 0.0);
   return 0;

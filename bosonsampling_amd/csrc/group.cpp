@@ -183,6 +183,7 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
     D.flag = reinterpret_cast<int32_t *>(wb + h->w_flag);
     D.slot = reinterpret_cast<int32_t *>(wb + h->w_slot);
     D.kcount = reinterpret_cast<int32_t *>(wb + h->w_kcount);
+    D.hist = reinterpret_cast<int32_t *>(wb + h->w_hist);
     D.Glc = io[u].Glc;
     D.Gcr = io[u].Gcr;
     D.LR = io[u].LR;

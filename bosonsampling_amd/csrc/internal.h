@@ -56,6 +56,12 @@ int two_stage_min();  // tridiagonal-route sectors with n >= this take the two-s
 int tri_split();      // many-sector groups: Householder blocks of 512 threads for n > split
 double tie_rel();     // relative width of the tie band at the chi-th singular value
 
+// Singular-value histogram of an update (counts of surely-kept values in bins
+// of a quarter octave, bin b holding values >= 2^((b - kHistZero) / 4)) and
+// the cutoff below which no value can be among the chi kept (stored after it)
+constexpr int kHistBins = 8192;
+constexpr int kHistZero = 4096;
+
 constexpr int kTileClasses = 4;
 constexpr int kTileElems[kTileClasses] = {1024, 2048, 4096, 8192};
 
@@ -85,7 +91,7 @@ struct PlanHeader {
   int32_t jac_max_n[kJacClasses];
   // workspace layout (byte offsets from the start of d_work)
   int64_t w_C, w_T, w_G, w_V, w_W, w_S, w_lam, w_flag, w_slot;
-  int64_t w_td, w_te, w_tau, w_kcount, w_triwork, w_total;
+  int64_t w_td, w_te, w_tau, w_kcount, w_triwork, w_hist, w_total;
   int32_t eig_threshold, pad_t;
   double tie_rel;
   // MFMA tile formulation of the contraction (see TileItem)

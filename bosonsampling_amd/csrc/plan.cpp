@@ -507,6 +507,7 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
   h.w_tau = wplace(16 * h.sv_total);
   h.w_kcount = wplace(4 * (int64_t)b.sectors.size());
   h.w_triwork = wplace(8 * tw);
+  h.w_hist = wplace(4 * kHistBins + 8);
   h.w_total = w;
   return 0;
 }
