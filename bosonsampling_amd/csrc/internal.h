@@ -36,6 +36,7 @@ constexpr int kTriClass = kJacClasses - 1;
 constexpr int kTriPanel = 8;  // column panel of the blocked Householder reduction
 constexpr int kTriVecThreads = 128;  // inverse-iteration threads per sector
 constexpr int kWyBlock = 32;  // back-transformation: reflectors per WY block, vectors per panel
+constexpr int kGramBlock = 32;  // Gram stage: output block of one workgroup (2 x 2 MFMA tiles)
 // Two-stage reduction of the large tridiagonal-route sectors: dense -> band of
 // width kBand (panels on the FP64 MFMA) -> tridiagonal (bulge chasing).
 constexpr int kBand = 16;

@@ -161,14 +161,12 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
         if (!lower_only) b.h.flops_jacobi += 8.0 * n * n * n;
       }
       int sid = (int)b.sectors.size();
-      if (lower_only) {  // 16 x 16 tiles on or below the diagonal
-        for (int r = 0; r < s.n; r += 16)
-          for (int c = 0; c <= r; c += 16)
-            b.gram_items.push_back(
-                Item{sid, 0, r, std::min(16, s.n - r), c, std::min(16, s.n - c), 0, 0});
-      } else {
-        add_tiles(b.gram_items, sid, 0, 0, s.n, 0, s.n);
-      }
+      // Gram blocks of kGramBlock x kGramBlock; on the tridiagonal route only
+      // those on or below the diagonal
+      for (int r = 0; r < s.n; r += kGramBlock)
+        for (int c = 0; c < s.n && (!lower_only || c <= r); c += kGramBlock)
+          b.gram_items.push_back(Item{sid, 0, r, std::min(kGramBlock, s.n - r), c,
+                                      std::min(kGramBlock, s.n - c), 0, 0});
       add_tiles(b.w_items, sid, 0, 0, s.n, 0, R);
       if (!s.left_small) add_tiles(b.v_items, sid, 0, 0, L, 0, s.n);
       b.sectors.push_back(s);

@@ -59,47 +59,96 @@ int launch_bs_table(int d, int nb, const double *d_B, double *d_U3, hipStream_t 
 
 // ================================================================ Gram ==
 // G (n x n, column-major) = T T^H if L <= R else T^H T, T row-major L x R.
+// One workgroup per 32 x 32 block of G (its four waves one 16 x 16 tile
+// each): the two 32-row operand panels of T are staged through LDS in chunks
+// of kGramK along the contracted index, read with coalesced loads one chunk
+// ahead, so each panel is read once per block instead of once per tile.
+// Tridiagonal-route sectors (n above the eigensolver threshold) need only the
+// lower triangle: the strictly upper tile of a diagonal block is skipped.
+constexpr int kGramK = 16;
+constexpr int kGramLd = kGramK + 1;  // LDS row stride (complex)
+
 __global__ __launch_bounds__(256) void k_gram(const UpdDesc *__restrict__ Ud,
                                               const int2 *__restrict__ list, int nlist) {
-  const int lane = threadIdx.x & 63;
+  __shared__ double2 Ps[2][kGramBlock * kGramLd];  // [buffer][row * kGramLd + k]
+  __shared__ double2 Qs[2][kGramBlock * kGramLd];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i16 = lane & 15, kq = lane >> 4;
-  const int nwaves = gridDim.x * 4;
-  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < nlist; it += nwaves) {
+  const int tr = wave >> 1, tc = wave & 1;  // this wave's tile of the block
+  // staging: thread -> (row x of the panel, k pair); 32 x 16 complex per panel
+  for (int it = blockIdx.x; it < nlist; it += gridDim.x) {
     const int2 ref = list[it];
     const UpdDesc &u = Ud[ref.x];
     const Item item = u.gram_items[ref.y];
     const Sector s = u.sectors[item.sector];
-    const double2 *Ts = reinterpret_cast<const double2 *>(u.T) + s.c_off;
-    const int p = item.r0 + i16, q = item.c0 + i16;
-    const bool pv = i16 < item.nr, qv = i16 < item.nc;
-    dbl4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
-    if (s.left_small) {
-      for (int k0 = 0; k0 < s.R; k0 += 4) {
-        const int k = k0 + kq;
-        cplx a{0.0, 0.0}, b{0.0, 0.0};
-        if (k < s.R) {
-          if (pv) { const double2 v = Ts[(int64_t)p * s.R + k]; a = cplx{v.x, v.y}; }
-          if (qv) { const double2 v = Ts[(int64_t)q * s.R + k]; b = cplx{v.x, -v.y}; }
+    const gptr<const double2> Tg = gp(reinterpret_cast<const double2 *>(u.T) + s.c_off);
+    const bool ls = s.left_small;
+    const int K = ls ? s.R : s.L;
+    const bool skip = s.n > u.eig_thr && item.r0 == item.c0 && tr < tc;  // strictly upper
+    // element e = threadIdx.x + 256 h (h = 0, 1) of a 32 x 16 chunk: for T T^H
+    // (rows of T contiguous in k) row e / 16, k e % 16; for T^H T (columns of
+    // T contiguous in the panel index) k e / 32, row e % 32
+    auto fetch = [&](int k0, double2 (&pv)[2], double2 (&qv)[2]) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int e = threadIdx.x + 256 * h;
+        const int x = ls ? e / kGramK : e % kGramBlock, k = ls ? e % kGramK : e / kGramBlock;
+        const int kk = k0 + k;
+        pv[h] = make_double2(0.0, 0.0);
+        qv[h] = make_double2(0.0, 0.0);
+        if (kk < K) {
+          if (x < item.nr)
+            pv[h] = gload2(Tg, ls ? (int64_t)(item.r0 + x) * s.R + kk : (int64_t)kk * s.R + item.r0 + x);
+          if (x < item.nc)
+            qv[h] = gload2(Tg, ls ? (int64_t)(item.c0 + x) * s.R + kk : (int64_t)kk * s.R + item.c0 + x);
         }
-        cmfma_f64(accr, acci, a, b);
       }
-    } else {
-      for (int k0 = 0; k0 < s.L; k0 += 4) {
-        const int k = k0 + kq;
-        cplx a{0.0, 0.0}, b{0.0, 0.0};
-        if (k < s.L) {
-          if (pv) { const double2 v = Ts[(int64_t)k * s.R + p]; a = cplx{v.x, -v.y}; }
-          if (qv) { const double2 v = Ts[(int64_t)k * s.R + q]; b = cplx{v.x, v.y}; }
+    };
+    auto stash = [&](int buf, const double2 (&pv)[2], const double2 (&qv)[2]) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int e = threadIdx.x + 256 * h;
+        const int x = ls ? e / kGramK : e % kGramBlock, k = ls ? e % kGramK : e / kGramBlock;
+        Ps[buf][x * kGramLd + k] = pv[h];
+        Qs[buf][x * kGramLd + k] = qv[h];
+      }
+    };
+    double2 pv[2], qv[2];
+    fetch(0, pv, qv);
+    __syncthreads();  // the previous item's readers of LDS are done
+    stash(0, pv, qv);
+    __syncthreads();
+    dbl4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
+    int buf = 0;
+    for (int k0 = 0; k0 < K; k0 += kGramK) {
+      const bool more = k0 + kGramK < K;  // uniform
+      if (more) fetch(k0 + kGramK, pv, qv);
+      if (!skip) {
+        const double2 *P = Ps[buf] + (16 * tr + i16) * kGramLd;
+        const double2 *Q = Qs[buf] + (16 * tc + i16) * kGramLd;
+#pragma unroll
+        for (int st = 0; st < kGramK / 4; ++st) {
+          const double2 a = P[4 * st + kq], b = Q[4 * st + kq];
+          // T T^H: A = T(p, k), B = conj(T(q, k)); T^H T: A = conj(T(k, p)), B = T(k, q)
+          cmfma_f64(accr, acci, cplx{a.x, ls ? a.y : -a.y}, cplx{b.x, ls ? -b.y : b.y});
         }
-        cmfma_f64(accr, acci, a, b);
+      }
+      if (more) {
+        stash(buf ^ 1, pv, qv);
+        __syncthreads();
+        buf ^= 1;
       }
     }
-    if (qv) {
+    if (!skip) {
+      const int qc = 16 * tc + i16;  // column of G in the block
+      if (qc < item.nc) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = mfma64_row(kq, r);
-        if (row < item.nr)
-          cstore(u.G, s.g_off + (int64_t)q * s.n + item.r0 + row, cplx{accr[r], acci[r]});
+        for (int r = 0; r < 4; ++r) {
+          const int row = 16 * tr + mfma64_row(kq, r);
+          if (row < item.nr)
+            cstore(u.G, s.g_off + (int64_t)(item.c0 + qc) * s.n + item.r0 + row,
+                   cplx{accr[r], acci[r]});
+        }
       }
     }
   }
@@ -488,7 +537,7 @@ static inline const int2 *glist(const char *w, int64_t off) {
 int launch_svd(const GroupHeader &g, const char *w, hipStream_t stream) {
   if (g.n_sec == 0) return 0;
   int tk = timing_begin(ST_GRAM, stream);
-  hipLaunchKernelGGL(k_gram, dim3(grid_for(g.n_gram)), dim3(256), 0, stream, gdesc(g, w),
+  hipLaunchKernelGGL(k_gram, dim3(std::max(1, std::min(g.n_gram, 16384))), dim3(256), 0, stream, gdesc(g, w),
                      glist(w, g.o_gram), g.n_gram);
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, g.flops_gram, 16.0 * (g.c_elems + g.g_elems));
