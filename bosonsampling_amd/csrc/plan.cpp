@@ -73,10 +73,10 @@ int make_blocks(int d, int chi, const int32_t *ch, std::vector<Block> &out, cons
 }
 
 void add_tiles(std::vector<Item> &items, int sector, int blk, int r_begin, int r_end, int c_begin,
-               int c_end) {
-  for (int r = r_begin; r < r_end; r += 16)
-    for (int c = c_begin; c < c_end; c += 16)
-      items.push_back(Item{sector, blk, r, std::min(16, r_end - r), c, std::min(16, c_end - c), 0, 0});
+               int c_end, int t = 16) {
+  for (int r = r_begin; r < r_end; r += t)
+    for (int c = c_begin; c < c_end; c += t)
+      items.push_back(Item{sector, blk, r, std::min(t, r_end - r), c, std::min(t, c_end - c), 0, 0});
 }
 
 int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *CR, int only_t0,
@@ -167,8 +167,8 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
         for (int c = 0; c < s.n && (!lower_only || c <= r); c += kGramBlock)
           b.gram_items.push_back(Item{sid, 0, r, std::min(kGramBlock, s.n - r), c,
                                       std::min(kGramBlock, s.n - c), 0, 0});
-      add_tiles(b.w_items, sid, 0, 0, s.n, 0, R);
-      if (!s.left_small) add_tiles(b.v_items, sid, 0, 0, L, 0, s.n);
+      add_tiles(b.w_items, sid, 0, 0, s.n, 0, R, kGramBlock);
+      if (!s.left_small) add_tiles(b.v_items, sid, 0, 0, L, 0, s.n, kGramBlock);
       b.sectors.push_back(s);
     }
   }

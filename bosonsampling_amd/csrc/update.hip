@@ -57,87 +57,112 @@ int launch_bs_table(int d, int nb, const double *d_B, double *d_U3, hipStream_t 
   return 0;
 }
 
-// ================================================================ Gram ==
-// G (n x n, column-major) = T T^H if L <= R else T^H T, T row-major L x R.
-// One workgroup per 32 x 32 block of G (its four waves one 16 x 16 tile
-// each): the two 32-row operand panels of T are staged through LDS in chunks
-// of kGramK along the contracted index, read with coalesced loads one chunk
-// ahead, so each panel is read once per block instead of once per tile.
-// Tridiagonal-route sectors (n above the eigensolver threshold) need only the
-// lower triangle: the strictly upper tile of a diagonal block is skipped.
+// ============================================ 32 x 32 block products ==
+// D (32 x 32 block) = sum_k A(i, k) B(k, j) on the FP64 MFMA, one workgroup
+// per block (its four waves one 16 x 16 tile each), the two 32-row operand
+// panels P(i, k) = A(i, k) and Q(j, k) = B(k, j) staged through LDS in chunks
+// of kGramK along k with coalesced loads one chunk ahead: each panel is read
+// once per block instead of once per 16 x 16 tile.  A panel is "k-major" when
+// its memory runs along i (resp. j) for fixed k, else along k; the staging
+// order follows memory.  fetch(x, k) returns the panel element (masked).
 constexpr int kGramK = 16;
 constexpr int kGramLd = kGramK + 1;  // LDS row stride (complex)
 
-__global__ __launch_bounds__(256) void k_gram(const UpdDesc *__restrict__ Ud,
-                                              const int2 *__restrict__ list, int nlist) {
-  __shared__ double2 Ps[2][kGramBlock * kGramLd];  // [buffer][row * kGramLd + k]
-  __shared__ double2 Qs[2][kGramBlock * kGramLd];
+struct BlockLds {
+  double2 P[2][kGramBlock * kGramLd], Q[2][kGramBlock * kGramLd];
+};
+
+template <bool PK, bool QK, bool PC, bool QC, class FP, class FQ>
+__device__ __forceinline__ void block_product(BlockLds &sm, int K, bool skip, FP fp, FQ fq,
+                                              dbl4 &accr, dbl4 &acci) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i16 = lane & 15, kq = lane >> 4;
-  const int tr = wave >> 1, tc = wave & 1;  // this wave's tile of the block
-  // staging: thread -> (row x of the panel, k pair); 32 x 16 complex per panel
+  const int tr = wave >> 1, tc = wave & 1;
+  auto xk = [](bool kmajor, int e, int &x, int &k) {
+    x = kmajor ? e % kGramBlock : e / kGramK;
+    k = kmajor ? e / kGramBlock : e % kGramK;
+  };
+  double2 pv[2], qv[2];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int e = threadIdx.x + 256 * h;
+      int x, k;
+      xk(PK, e, x, k);
+      pv[h] = k0 + k < K ? fp(x, k0 + k) : make_double2(0.0, 0.0);
+      xk(QK, e, x, k);
+      qv[h] = k0 + k < K ? fq(x, k0 + k) : make_double2(0.0, 0.0);
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int e = threadIdx.x + 256 * h;
+      int x, k;
+      xk(PK, e, x, k);
+      sm.P[buf][x * kGramLd + k] = pv[h];
+      xk(QK, e, x, k);
+      sm.Q[buf][x * kGramLd + k] = qv[h];
+    }
+  };
+  fetch(0);
+  __syncthreads();  // the previous block's readers of LDS are done
+  stash(0);
+  __syncthreads();
+  accr = dbl4{0.0, 0.0, 0.0, 0.0};
+  acci = dbl4{0.0, 0.0, 0.0, 0.0};
+  int buf = 0;
+  for (int k0 = 0; k0 < K; k0 += kGramK) {
+    const bool more = k0 + kGramK < K;  // uniform
+    if (more) fetch(k0 + kGramK);
+    if (!skip) {
+      const double2 *Pr = sm.P[buf] + (16 * tr + i16) * kGramLd;
+      const double2 *Qr = sm.Q[buf] + (16 * tc + i16) * kGramLd;
+#pragma unroll
+      for (int st = 0; st < kGramK / 4; ++st) {
+        const double2 a = Pr[4 * st + kq], b = Qr[4 * st + kq];
+        cmfma_f64(accr, acci, cplx{a.x, PC ? -a.y : a.y}, cplx{b.x, QC ? -b.y : b.y});
+      }
+    }
+    if (more) {
+      stash(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+  }
+}
+
+// ================================================================ Gram ==
+// G (n x n, column-major) = T T^H if L <= R else T^H T, T row-major L x R.
+// Tridiagonal-route sectors (n above the eigensolver threshold) need only the
+// lower triangle: the strictly upper tile of a diagonal block is skipped.
+__global__ __launch_bounds__(256) void k_gram(const UpdDesc *__restrict__ Ud,
+                                              const int2 *__restrict__ list, int nlist) {
+  __shared__ BlockLds sm;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i16 = lane & 15, kq = lane >> 4;
+  const int tr = wave >> 1, tc = wave & 1;
   for (int it = blockIdx.x; it < nlist; it += gridDim.x) {
     const int2 ref = list[it];
     const UpdDesc &u = Ud[ref.x];
     const Item item = u.gram_items[ref.y];
     const Sector s = u.sectors[item.sector];
     const gptr<const double2> Tg = gp(reinterpret_cast<const double2 *>(u.T) + s.c_off);
-    const bool ls = s.left_small;
-    const int K = ls ? s.R : s.L;
     const bool skip = s.n > u.eig_thr && item.r0 == item.c0 && tr < tc;  // strictly upper
-    // element e = threadIdx.x + 256 h (h = 0, 1) of a 32 x 16 chunk: for T T^H
-    // (rows of T contiguous in k) row e / 16, k e % 16; for T^H T (columns of
-    // T contiguous in the panel index) k e / 32, row e % 32
-    auto fetch = [&](int k0, double2 (&pv)[2], double2 (&qv)[2]) {
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int e = threadIdx.x + 256 * h;
-        const int x = ls ? e / kGramK : e % kGramBlock, k = ls ? e % kGramK : e / kGramBlock;
-        const int kk = k0 + k;
-        pv[h] = make_double2(0.0, 0.0);
-        qv[h] = make_double2(0.0, 0.0);
-        if (kk < K) {
-          if (x < item.nr)
-            pv[h] = gload2(Tg, ls ? (int64_t)(item.r0 + x) * s.R + kk : (int64_t)kk * s.R + item.r0 + x);
-          if (x < item.nc)
-            qv[h] = gload2(Tg, ls ? (int64_t)(item.c0 + x) * s.R + kk : (int64_t)kk * s.R + item.c0 + x);
-        }
-      }
-    };
-    auto stash = [&](int buf, const double2 (&pv)[2], const double2 (&qv)[2]) {
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int e = threadIdx.x + 256 * h;
-        const int x = ls ? e / kGramK : e % kGramBlock, k = ls ? e % kGramK : e / kGramBlock;
-        Ps[buf][x * kGramLd + k] = pv[h];
-        Qs[buf][x * kGramLd + k] = qv[h];
-      }
-    };
-    double2 pv[2], qv[2];
-    fetch(0, pv, qv);
-    __syncthreads();  // the previous item's readers of LDS are done
-    stash(0, pv, qv);
-    __syncthreads();
-    dbl4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
-    int buf = 0;
-    for (int k0 = 0; k0 < K; k0 += kGramK) {
-      const bool more = k0 + kGramK < K;  // uniform
-      if (more) fetch(k0 + kGramK, pv, qv);
-      if (!skip) {
-        const double2 *P = Ps[buf] + (16 * tr + i16) * kGramLd;
-        const double2 *Q = Qs[buf] + (16 * tc + i16) * kGramLd;
-#pragma unroll
-        for (int st = 0; st < kGramK / 4; ++st) {
-          const double2 a = P[4 * st + kq], b = Q[4 * st + kq];
-          // T T^H: A = T(p, k), B = conj(T(q, k)); T^H T: A = conj(T(k, p)), B = T(k, q)
-          cmfma_f64(accr, acci, cplx{a.x, ls ? a.y : -a.y}, cplx{b.x, ls ? -b.y : b.y});
-        }
-      }
-      if (more) {
-        stash(buf ^ 1, pv, qv);
-        __syncthreads();
-        buf ^= 1;
-      }
+    const int R = s.R;
+    dbl4 accr, acci;
+    if (s.left_small) {  // P = T(r0 + i, k), Q = T(c0 + j, k) conjugated; rows of T contiguous
+      block_product<false, false, false, true>(
+          sm, R, skip,
+          [&](int x, int k) { return x < item.nr ? gload2(Tg, (int64_t)(item.r0 + x) * R + k) : make_double2(0.0, 0.0); },
+          [&](int x, int k) { return x < item.nc ? gload2(Tg, (int64_t)(item.c0 + x) * R + k) : make_double2(0.0, 0.0); },
+          accr, acci);
+    } else {  // P = T(k, r0 + i) conjugated, Q = T(k, c0 + j); columns of T along a row
+      block_product<true, true, true, false>(
+          sm, s.L, skip,
+          [&](int x, int k) { return x < item.nr ? gload2(Tg, (int64_t)k * R + item.r0 + x) : make_double2(0.0, 0.0); },
+          [&](int x, int k) { return x < item.nc ? gload2(Tg, (int64_t)k * R + item.c0 + x) : make_double2(0.0, 0.0); },
+          accr, acci);
     }
     if (!skip) {
       const int qc = 16 * tc + i16;  // column of G in the block
@@ -296,41 +321,40 @@ __global__ __launch_bounds__(256) void k_jacobi(const UpdDesc *__restrict__ Ud,
   }
 }
 
-// V = T Y diag(1/sigma) (L > R sectors); Y lives in the W area, n x n col-major.
+// V (L x n column-major) = T Y diag(1 / sigma) for sectors with L > R (Y
+// the Gram eigenvectors, n x n column-major in W's storage), kept columns only.
 __global__ __launch_bounds__(256) void k_vectors(const UpdDesc *__restrict__ Ud,
                                                  const int2 *__restrict__ list, int nlist) {
-  const int lane = threadIdx.x & 63;
+  __shared__ BlockLds sm;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i16 = lane & 15, kq = lane >> 4;
-  const int nwaves = gridDim.x * 4;
-  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < nlist; it += nwaves) {
+  const int tr = wave >> 1, tc = wave & 1;
+  for (int it = blockIdx.x; it < nlist; it += gridDim.x) {
     const int2 ref = list[it];
     const UpdDesc &u = Ud[ref.x];
     const Item item = u.v_items[ref.y];
-    if (item.c0 >= u.kcount[item.sector]) continue;  // only kept vectors are needed
+    if (item.c0 >= u.kcount[item.sector]) continue;  // uniform: only kept vectors are needed
     const Sector s = u.sectors[item.sector];
-    const double2 *Tc = reinterpret_cast<const double2 *>(u.T);
-    const double2 *Yc = reinterpret_cast<const double2 *>(u.W);
-    const int i = item.r0 + i16, p = item.c0 + i16;
-    const bool iv = i16 < item.nr, pv = i16 < item.nc;
-    dbl4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < s.R; k0 += 4) {
-      const int k = k0 + kq;
-      cplx a{0.0, 0.0}, b{0.0, 0.0};
-      if (k < s.R) {
-        if (iv) { const double2 v = Tc[s.c_off + (int64_t)i * s.R + k]; a = cplx{v.x, v.y}; }
-        if (pv) { const double2 v = Yc[s.w_off + (int64_t)p * s.n + k]; b = cplx{v.x, v.y}; }
-      }
-      cmfma_f64(accr, acci, a, b);
-    }
-    if (pv) {
+    const gptr<const double2> Tg = gp(reinterpret_cast<const double2 *>(u.T) + s.c_off);
+    const gptr<const double2> Yg = gp(reinterpret_cast<const double2 *>(u.W) + s.w_off);
+    const int R = s.R, n = s.n;
+    dbl4 accr, acci;
+    // P = T(r0 + i, k) (rows of T contiguous), Q = Y(k, c0 + p) (columns of Y contiguous)
+    block_product<false, false, false, false>(
+        sm, R, false,
+        [&](int x, int k) { return x < item.nr ? gload2(Tg, (int64_t)(item.r0 + x) * R + k) : make_double2(0.0, 0.0); },
+        [&](int x, int k) { return x < item.nc ? gload2(Yg, (int64_t)(item.c0 + x) * n + k) : make_double2(0.0, 0.0); },
+        accr, acci);
+    const int pc = 16 * tc + i16;
+    if (pc < item.nc) {
+      const int p = item.c0 + pc;
       const double sg = u.S[s.s_off + p];
       const double inv = sg > 0.0 ? 1.0 / sg : 0.0;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = mfma64_row(kq, r);
+        const int row = 16 * tr + mfma64_row(kq, r);
         if (row < item.nr)
-          cstore(u.V, s.v_off + (int64_t)p * s.L + item.r0 + row,
-                 cplx{accr[r] * inv, acci[r] * inv});
+          cstore(u.V, s.v_off + (int64_t)p * s.L + item.r0 + row, cplx{accr[r] * inv, acci[r] * inv});
       }
     }
   }
@@ -339,35 +363,33 @@ __global__ __launch_bounds__(256) void k_vectors(const UpdDesc *__restrict__ Ud,
 // W (n x R row-major) = V^H C, V L x n column-major, C L x R row-major.
 __global__ __launch_bounds__(256) void k_wmat(const UpdDesc *__restrict__ Ud,
                                               const int2 *__restrict__ list, int nlist) {
-  const int lane = threadIdx.x & 63;
+  __shared__ BlockLds sm;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i16 = lane & 15, kq = lane >> 4;
-  const int nwaves = gridDim.x * 4;
-  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < nlist; it += nwaves) {
+  const int tr = wave >> 1, tc = wave & 1;
+  for (int it = blockIdx.x; it < nlist; it += gridDim.x) {
     const int2 ref = list[it];
     const UpdDesc &u = Ud[ref.x];
     const Item item = u.w_items[ref.y];
-    if (item.r0 >= u.kcount[item.sector]) continue;  // only kept rows of W are needed
+    if (item.r0 >= u.kcount[item.sector]) continue;  // uniform: only kept rows of W are needed
     const Sector s = u.sectors[item.sector];
-    const double2 *Vc = reinterpret_cast<const double2 *>(u.V);
-    const double2 *Cc = reinterpret_cast<const double2 *>(u.C);
-    const int p = item.r0 + i16, j = item.c0 + i16;
-    const bool pv = i16 < item.nr, jv = i16 < item.nc;
-    dbl4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < s.L; k0 += 4) {
-      const int k = k0 + kq;
-      cplx a{0.0, 0.0}, b{0.0, 0.0};
-      if (k < s.L) {
-        if (pv) { const double2 v = Vc[s.v_off + (int64_t)p * s.L + k]; a = cplx{v.x, -v.y}; }
-        if (jv) { const double2 v = Cc[s.c_off + (int64_t)k * s.R + j]; b = cplx{v.x, v.y}; }
-      }
-      cmfma_f64(accr, acci, a, b);
-    }
-    if (jv) {
+    const gptr<const double2> Vg = gp(reinterpret_cast<const double2 *>(u.V) + s.v_off);
+    const gptr<const double2> Cg = gp(reinterpret_cast<const double2 *>(u.C) + s.c_off);
+    const int L = s.L, R = s.R;
+    dbl4 accr, acci;
+    // P = V(k, r0 + p) conjugated (columns of V contiguous), Q = C(k, c0 + j) (rows of C)
+    block_product<false, true, true, false>(
+        sm, L, false,
+        [&](int x, int k) { return x < item.nr ? gload2(Vg, (int64_t)(item.r0 + x) * L + k) : make_double2(0.0, 0.0); },
+        [&](int x, int k) { return x < item.nc ? gload2(Cg, (int64_t)k * R + item.c0 + x) : make_double2(0.0, 0.0); },
+        accr, acci);
+    const int jc = 16 * tc + i16;
+    if (jc < item.nc) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = mfma64_row(kq, r);
+        const int row = 16 * tr + mfma64_row(kq, r);
         if (row < item.nr)
-          cstore(u.W, s.w_off + (int64_t)(item.r0 + row) * s.R + j, cplx{accr[r], acci[r]});
+          cstore(u.W, s.w_off + (int64_t)(item.r0 + row) * R + item.c0 + jc, cplx{accr[r], acci[r]});
       }
     }
   }
@@ -565,14 +587,14 @@ int launch_truncate(const GroupHeader &g, const char *w, hipStream_t stream) {
   if ((rc = launch_tri_vectors(g, w, stream))) return rc;
   if (g.n_v) {
     tk = timing_begin(ST_VECTORS, stream);
-    hipLaunchKernelGGL(k_vectors, dim3(grid_for(g.n_v)), dim3(256), 0, stream, gdesc(g, w),
+    hipLaunchKernelGGL(k_vectors, dim3(std::min(g.n_v, 16384)), dim3(256), 0, stream, gdesc(g, w),
                        glist(w, g.o_v), g.n_v);
     BSMPO_HIP(hipGetLastError());
     timing_end(tk, stream, 0.0, 0.0);  // flops: kept pairs, counted by k_kcount
   }
   if (g.n_w) {
     tk = timing_begin(ST_WMAT, stream);
-    hipLaunchKernelGGL(k_wmat, dim3(grid_for(g.n_w)), dim3(256), 0, stream, gdesc(g, w),
+    hipLaunchKernelGGL(k_wmat, dim3(std::min(g.n_w, 16384)), dim3(256), 0, stream, gdesc(g, w),
                        glist(w, g.o_w), g.n_w);
     BSMPO_HIP(hipGetLastError());
     timing_end(tk, stream, 0.0, 16.0 * (g.v_elems + g.c_elems + g.w_elems));  // flops: k_kcount
