@@ -40,18 +40,28 @@ __device__ __forceinline__ WyItem wy_item(int2 r) { return WyItem{r.x, r.y & 0xf
 // eight threads read 32 consecutive rows of one reflector column).  Reflector
 // i has its unit entry at row i + off (off = 1 after the one-stage reduction,
 // kBand after the two-stage one) and the rest below it in column i of A.
-__device__ __forceinline__ void wy_load_v(double2 *Vs, gptr<const double2> A, int n, int i0,
-                                          int rb, int off) {
+// A chunk lives in four registers per thread between its fetch and its
+// stash, so that the next chunk's loads are in flight while the current one is
+// multiplied.
+struct WyChunk {
+  double2 x[4];
+};
+__device__ __forceinline__ void wy_fetch_v(WyChunk &c, gptr<const double2> A, int n, int i0, int rb,
+                                           int off) {
   const int p = threadIdx.x >> 3, r0 = (threadIdx.x & 7) * 4;
   const int i = i0 + p;
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     const int r = rb + r0 + e;
-    double2 v = make_double2(0.0, 0.0);
+    c.x[e] = make_double2(0.0, 0.0);
     if (i <= n - 1 - off && r < n && r >= i + off)
-      v = r == i + off ? make_double2(1.0, 0.0) : gload2(A, (int64_t)i * n + r);
-    Vs[(r0 + e) * kWyLd + p] = v;
+      c.x[e] = r == i + off ? make_double2(1.0, 0.0) : gload2(A, (int64_t)i * n + r);
   }
+}
+__device__ __forceinline__ void wy_stash(double2 *S, const WyChunk &c) {
+  const int p = threadIdx.x >> 3, r0 = (threadIdx.x & 7) * 4;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) S[(r0 + e) * kWyLd + p] = c.x[e];
 }
 
 __device__ __forceinline__ int wy_offset(const UpdDesc &u, const Sector &s) {
@@ -59,14 +69,13 @@ __device__ __forceinline__ int wy_offset(const UpdDesc &u, const Sector &s) {
 }
 
 // Z rows rb .. rb + 31 of vectors j0 .. j0 + kp - 1 (column-major, length n)
-__device__ __forceinline__ void wy_load_z(double2 *Zs, gptr<const double2> X, int n, int j0, int kp,
-                                          int rb) {
+__device__ __forceinline__ void wy_fetch_z(WyChunk &c, gptr<const double2> X, int n, int j0, int kp,
+                                           int rb) {
   const int j = threadIdx.x >> 3, r0 = (threadIdx.x & 7) * 4;
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     const int r = rb + r0 + e;
-    Zs[(r0 + e) * kWyLd + j] =
-        (j < kp && r < n) ? gload2(X, (int64_t)(j0 + j) * n + r) : make_double2(0.0, 0.0);
+    c.x[e] = (j < kp && r < n) ? gload2(X, (int64_t)(j0 + j) * n + r) : make_double2(0.0, 0.0);
   }
 }
 
@@ -141,10 +150,13 @@ __global__ __launch_bounds__(256) void k_wy_tmat(const UpdDesc *__restrict__ Ud,
     // S = V^H V over rows i0 + off .. n - 1, one 16 x 16 tile per wave
     const int tp = wave >> 1, tj = wave & 1;
     dbl4 ar = {0.0, 0.0, 0.0, 0.0}, ai = {0.0, 0.0, 0.0, 0.0};
+    WyChunk cv;
+    wy_fetch_v(cv, A, n, i0, i0 + off, off);
     for (int rb = i0 + off; rb < n; rb += kWyRows) {
       __syncthreads();
-      wy_load_v(Vs, A, n, i0, rb, off);
+      wy_stash(Vs, cv);
       __syncthreads();
+      if (rb + kWyRows < n) wy_fetch_v(cv, A, n, i0, rb + kWyRows, off);
       wy_vhz(ar, ai, Vs, Vs, tp, tj, lane);
     }
     wy_put(Ss, ar, ai, tp, tj, lane);
@@ -205,11 +217,18 @@ __global__ __launch_bounds__(256) void k_wy_apply(const UpdDesc *__restrict__ Ud
         Ts[(e / kWy) * kWyLd + e % kWy] = gload2(Tw, (int64_t)m * kWy * kWy + e);
       // (1) W = V^H Z over rows i0 + off .. n - 1
       dbl4 ar = {0.0, 0.0, 0.0, 0.0}, ai = {0.0, 0.0, 0.0, 0.0};
+      WyChunk cv, cz;
+      wy_fetch_v(cv, A, n, i0, i0 + off, off);
+      wy_fetch_z(cz, X, n, j0, kp, i0 + off);
       for (int rb = i0 + off; rb < n; rb += kWyRows) {
         __syncthreads();
-        wy_load_v(Vs, A, n, i0, rb, off);
-        wy_load_z(Zs, X, n, j0, kp, rb);
+        wy_stash(Vs, cv);
+        wy_stash(Zs, cz);
         __syncthreads();
+        if (rb + kWyRows < n) {
+          wy_fetch_v(cv, A, n, i0, rb + kWyRows, off);
+          wy_fetch_z(cz, X, n, j0, kp, rb + kWyRows);
+        }
         wy_vhz(ar, ai, Vs, Zs, ta, tb, lane);
       }
       wy_put(Ws, ar, ai, ta, tb, lane);
@@ -221,11 +240,17 @@ __global__ __launch_bounds__(256) void k_wy_apply(const UpdDesc *__restrict__ Ud
       __syncthreads();
       wy_put(Ws, ar, ai, ta, tb, lane);
       // (3) Z <- Z - V W, chunk by chunk: staged, updated in LDS, written back
+      wy_fetch_v(cv, A, n, i0, i0 + off, off);
+      wy_fetch_z(cz, X, n, j0, kp, i0 + off);
       for (int rb = i0 + off; rb < n; rb += kWyRows) {
         __syncthreads();
-        wy_load_v(Vs, A, n, i0, rb, off);
-        wy_load_z(Zs, X, n, j0, kp, rb);
+        wy_stash(Vs, cv);
+        wy_stash(Zs, cz);
         __syncthreads();
+        if (rb + kWyRows < n) {
+          wy_fetch_v(cv, A, n, i0, rb + kWyRows, off);
+          wy_fetch_z(cz, X, n, j0, kp, rb + kWyRows);
+        }
         ar = dbl4{0.0, 0.0, 0.0, 0.0};
         ai = dbl4{0.0, 0.0, 0.0, 0.0};
         wy_mm(ar, ai, Vs, Ws, ta, tb, lane);
