@@ -251,6 +251,7 @@ def main():
     # -------------------------------------------------------------- grouped
     roof = None
     if "grouped" in paths:
+        rules = engine.set_precision_rules(prec)  # tie band and drop rule of the precision
         states = [init.clone() for _ in range(B)]
         runner = engine.GroupRunner(init.device, max_bytes=int(args.group_gib * (1 << 30)),
                                     streams=args.streams, first_frac=args.first_frac)
@@ -289,7 +290,8 @@ def main():
         _capi.timing_enable(False)
         result["grouped"] = {"updates": n, "seconds": el, "rate": n / el,
                              "observables0": states[0].observables(), "tuned": tuned,
-                             "bond": [bond0, kept_bond_dimension(states[0])]}
+                             "bond": [bond0, kept_bond_dimension(states[0])], "rules": rules}
+        engine.set_precision_rules(engine.C128)
         log(f"[bench] grouped {args.precision}: {n} updates in {el:.3f}s = {n / el:.2f} updates/s")
         stage_table(st_grp)
         roof = roofline(st_grp, args.precision)
@@ -334,7 +336,8 @@ def main():
                                     "plain_start_end": p["bond"] if p else None},
             "grouped": ({"instances": B, "precision": args.precision, "updates": g["updates"],
                          "updates_per_s": g["rate"], "autotune": g.get("tuned"),
-                         "launch_streams": args.streams, "two_stage_min_n": two_stage}
+                         "launch_streams": args.streams, "two_stage_min_n": two_stage,
+                         "tie_rel": g["rules"][0], "drop_rel": g["rules"][1]}
                         if g else None),
             "plain_fp64": ({"instances": 1, "updates": p["updates"],
                             "updates_per_s": p["rate"], "autotune": p.get("tuned")} if p else None),

@@ -26,6 +26,8 @@ SIGNATURES = {
     "bsmpo_set_two_stage": (C.c_int, [_i]),
     "bsmpo_two_stage_threshold": (C.c_int, []),
     "bsmpo_set_tie_rel": (C.c_int, [C.c_double]),
+    "bsmpo_set_drop_rel": (C.c_int, [C.c_double]),
+    "bsmpo_drop_rel": (C.c_double, []),
     "bsmpo_bs_table": (C.c_int, [_i, _i, _dp, _dp, _vp]),
     "bsmpo_plan_query": (C.c_int, [_i, _i, _i32p, _i32p, _i32p, _i, _i, C.POINTER(C.c_size_t),
                                    C.POINTER(C.c_size_t)]),

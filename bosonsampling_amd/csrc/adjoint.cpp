@@ -236,6 +236,7 @@ int bsmpo_site_update_vjp(int d, int chi, const void *h_plan, const int32_t *h_c
   a.gU3 = d_gU3;
   a.scr = reinterpret_cast<double *>(w + Lo.o_scr);
   a.dscr = reinterpret_cast<double *>(w + Lo.o_dscr);
+  a.drop_rel2 = drop_rel() * drop_rel();
   return launch_site_vjp(a, Lo.n_pairs, st);
 }
 

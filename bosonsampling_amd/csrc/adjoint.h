@@ -40,6 +40,7 @@ struct VjpArgs {
   const double *gG0, *gG1, *gLam;   // gradients of the outputs
   double *gGlc, *gGcr, *gLR, *gU3;  // gradients of the inputs (written)
   double *scr, *dscr;               // scratch
+  double drop_rel2;                 // the forward's drop rule, on lambda
 };
 
 int launch_site_vjp(const VjpArgs &a, int n_pairs, hipStream_t st);

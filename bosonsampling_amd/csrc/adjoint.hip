@@ -24,7 +24,6 @@ namespace {
 
 constexpr int kVjpThreads = 256;
 constexpr int kVjpSweeps = 60;
-constexpr double kVjpDrop2 = 1e-12;   // lambda <= 1e-12 lambda_max: dropped (as the forward)
 constexpr double kVjpDegen = 1e-13;   // |lambda_i - lambda_j| <= 1e-13 lambda_max: no coupling
 
 __device__ __forceinline__ cplx cadd(cplx a, cplx b) { return cplx{a.x + b.x, a.y + b.y}; }
@@ -196,7 +195,7 @@ __global__ __launch_bounds__(kVjpThreads) void k_sector_vjp(VjpArgs a) {
   for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
     const int p = e / n, i = e % n;
     const double lp = tmp[p];
-    const bool keep = lam_max > 0.0 && lp > kVjpDrop2 * lam_max;
+    const bool keep = lam_max > 0.0 && lp > a.drop_rel2 * lam_max;
     const cplx v = cload(Gm, e);
     cstore(Y, (int64_t)rank[p] * n + i, keep ? cscale(v, 1.0 / lp) : zero);
     if (i == 0) lam[rank[p]] = keep ? lp : 0.0;

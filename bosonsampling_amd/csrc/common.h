@@ -125,6 +125,7 @@ struct UpdDesc {
   const int2 *row_info, *col_info;  // per bond slot, see SlotInfo
   int32_t n_lblk, n_rblk;
   int32_t ts_min, pad_ts;  // two-stage threshold of the plan (see two_stage)
+  double drop_rel2;        // lambda <= drop_rel2 * lambda_max of the sector: dropped
 };
 
 // Plan arrays as seen by kernels (device copy at the front of the workspace).

@@ -20,7 +20,6 @@
 
 namespace bsmpo {
 
-constexpr double kDropRel2Tri = 1e-12;
 constexpr double kClusterRel = 1e-6;
 
 __device__ double block_sum_d(double v, double *red) {
@@ -79,6 +78,14 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
 constexpr int kTriNB = kTriPanel;
 constexpr int kTriMaxN = 2048;
 constexpr int kTriFewSectors = 256;  // below this many sectors per launch: one block shape
+// Streaming shape of the reduction.  Measured on the M=64, chi=1024 grouped
+// workload (one launching stream): strips of 4 columns at 3 waves per SIMD
+// (168 VGPRs, one trailing tile per pass) 1.7% faster, 4 waves per SIMD
+// (spills) 25% slower, strips of 4 at 2 waves 4% slower; the stage streams
+// ~76% of the copy bandwidth either way, so the plain shape stays.
+constexpr int kTriSW = 8;    // columns per strip of the streamed product A22 v (8 or 4)
+constexpr int kTriOcc = 2;   // waves per SIMD the launch bound asks for
+constexpr int kTriTT = 2;    // trailing-update tiles per wave pass
 
 __device__ __forceinline__ double2 csub_mul(double2 a, double2 x, double2 y) {  // a - x conj(y)
   a.x -= x.x * y.x + x.y * y.y;
@@ -134,18 +141,37 @@ __device__ __forceinline__ double2 wave_colsum8(const double2 (&x)[8], int lane)
   return y;
 }
 
-template <int NT, int RPT>
-__global__ __launch_bounds__(NT, 2) void k_tridiag(const UpdDesc *__restrict__ Ud,
-                                                const int2 *__restrict__ list, int nmin, int nmax) {
+// x[q] (q = 0..3) summed over the 64 lanes of a wave by recursive halving:
+// lane l returns the total of x[(l >> 4) & 3].
+__device__ __forceinline__ double2 wave_colsum4(const double2 (&x)[4], int lane) {
+  const bool h5 = lane & 32, h4 = lane & 16;
+  double2 y2[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const double2 snd = h5 ? x[q] : x[q + 2], kp = h5 ? x[q + 2] : x[q];
+    y2[q] = make_double2(kp.x + __shfl_xor(snd.x, 32), kp.y + __shfl_xor(snd.y, 32));
+  }
+  const double2 snd = h4 ? y2[0] : y2[1], kp = h4 ? y2[1] : y2[0];
+  double2 y = make_double2(kp.x + __shfl_xor(snd.x, 16), kp.y + __shfl_xor(snd.y, 16));
+#pragma unroll
+  for (int o = 8; o >= 1; o >>= 1) {
+    y.x += __shfl_xor(y.x, o);
+    y.y += __shfl_xor(y.y, o);
+  }
+  return y;
+}
+template <int SW>
+__device__ __forceinline__ double2 wave_colsum(const double2 (&x)[SW], int lane) {
+  if constexpr (SW == 8) return wave_colsum8(x, lane);
+  else return wave_colsum4(x, lane);
+}
+
+// One sector's reduction by the whole block (tsm: 2 n complex of LDS).
+template <int NT, int RPT, int SW>
+__device__ __forceinline__ void tridiag_sector(const UpdDesc &u, const Sector &s, double2 *tsm,
+                                               double2 (*red)[16], double (*red4)[4 * kTriNB]) {
   constexpr int NW = NT / 64;
-  extern __shared__ double2 tsm[];
-  __shared__ double2 red[2][16];
-  __shared__ double red4[NW][4 * kTriNB];
-  const int2 ref = list[blockIdx.x];
-  const UpdDesc &u = Ud[ref.x];
-  const Sector s = u.sectors[ref.y];
   const int n = s.n;
-  if (n < nmin || n > nmax) return;  // the other block shape's sector
   gptr<double2> A = gp(reinterpret_cast<double2 *>(u.G) + s.g_off);  // A(r, c) = A[c n + r]
   double2 *vs = tsm;                                                  // v_j, n entries
   double2 *ys = tsm + n;                                              // A22 v_j, n entries
@@ -229,7 +255,7 @@ __global__ __launch_bounds__(NT, 2) void k_tridiag(const UpdDesc *__restrict__ U
       //     next strip) form one sequence whose loads run two items ahead,
       //     across strip boundaries.
       {
-        constexpr int stride = 8 * NW;
+        constexpr int stride = SW * NW;
         auto advance = [&](int &s, int &r0) {  // next item of this wave
           r0 += 64;
           if (r0 >= n) {
@@ -237,16 +263,16 @@ __global__ __launch_bounds__(NT, 2) void k_tridiag(const UpdDesc *__restrict__ U
             r0 = s;
           }
         };
-        auto fetch = [&](double2(&x)[8], int s, int r0) {
+        auto fetch = [&](double2(&x)[SW], int s, int r0) {
           const int r = r0 + lane;
 #pragma unroll
-          for (int q = 0; q < 8; ++q)
+          for (int q = 0; q < SW; ++q)
             x[q] = (r < n && s + q <= r) ? A[(int64_t)(s + q) * n + r] : make_double2(0.0, 0.0);
         };
-        double2 cp[8], a[8], b[8], c[8];
+        double2 cp[SW], a[SW], b[SW], c[SW];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) cp[q] = make_double2(0.0, 0.0);
-        int sa = j + 1 + 8 * wave, ra = sa;  // item held in a
+        for (int q = 0; q < SW; ++q) cp[q] = make_double2(0.0, 0.0);
+        int sa = j + 1 + SW * wave, ra = sa;  // item held in a
         int sb = sa, rb = ra;
         advance(sb, rb);                     // item held in b
         int sc = sb, rc = rb;
@@ -260,7 +286,7 @@ __global__ __launch_bounds__(NT, 2) void k_tridiag(const UpdDesc *__restrict__ U
             const double2 vr = vs[r];
             double2 rp = make_double2(0.0, 0.0);
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
+            for (int q = 0; q < SW; ++q) {
               rp = cfma2(rp, a[q], vs[min(sa + q, n - 1)]);
               if (sa + q < r) cp[q] = cfma2(cp[q], make_double2(a[q].x, -a[q].y), vr);
             }
@@ -268,17 +294,17 @@ __global__ __launch_bounds__(NT, 2) void k_tridiag(const UpdDesc *__restrict__ U
             atomicAdd(&ys[r].y, rp.y * scl);
           }
           if (sb != sa) {  // strip sa done: its column sums
-            const double2 cs = wave_colsum8(cp, lane);
-            const int cq = sa + (lane >> 3);
-            if ((lane & 7) == 0 && cq < n) {
+            const double2 cs = wave_colsum<SW>(cp, lane);
+            const int cq = sa + (lane >> 3) / (8 / SW);
+            if ((lane & (64 / SW - 1)) == 0 && cq < n) {
               atomicAdd(&ys[cq].x, cs.x * scl);
               atomicAdd(&ys[cq].y, cs.y * scl);
             }
 #pragma unroll
-            for (int q = 0; q < 8; ++q) cp[q] = make_double2(0.0, 0.0);
+            for (int q = 0; q < SW; ++q) cp[q] = make_double2(0.0, 0.0);
           }
 #pragma unroll
-          for (int q = 0; q < 8; ++q) {
+          for (int q = 0; q < SW; ++q) {
             a[q] = b[q];
             b[q] = c[q];
           }
@@ -385,7 +411,7 @@ __global__ __launch_bounds__(NT, 2) void k_tridiag(const UpdDesc *__restrict__ U
       const int i16 = lane & 15, kq = lane >> 4;
       // two tiles per pass (tix and tix + NW), every load of both issued
       // before the first product, so that a pass waits on memory once
-      constexpr int kTT = 2;
+      constexpr int kTT = kTriTT;
       for (int tix0 = wave; tix0 < ntiles; tix0 += kTT * NW) {
         int R0[kTT], C0[kTT];
         bool live[kTT];
@@ -453,6 +479,25 @@ __global__ __launch_bounds__(NT, 2) void k_tridiag(const UpdDesc *__restrict__ U
   }
 }
 
+// Blocks walk the list (grid-stride, largest sectors first); sectors with n
+// outside [nmin, nmax] belong to the other block shape's launch.
+template <int NT, int RPT, int SW, int OCC>
+__global__ __launch_bounds__(NT, OCC) void k_tridiag(const UpdDesc *__restrict__ Ud,
+                                                const int2 *__restrict__ list, int nlist,
+                                                int nmin, int nmax) {
+  extern __shared__ double2 tsm[];
+  __shared__ double2 red[2][16];
+  __shared__ double red4[NT / 64][4 * kTriNB];
+  for (int it = blockIdx.x; it < nlist; it += gridDim.x) {
+    const int2 ref = list[it];
+    const UpdDesc &u = Ud[ref.x];
+    const Sector s = u.sectors[ref.y];
+    if (s.n < nmin || s.n > nmax) continue;  // uniform
+    __syncthreads();  // the previous sector's readers of LDS are done
+    tridiag_sector<NT, RPT, SW>(u, s, tsm, red, red4);
+  }
+}
+
 // 1 / q from the hardware reciprocal and one Newton step (relative error a few
 // ulp): the Sturm recurrence below is a chain of these, and IEEE division is
 // about twice as long a dependency chain.  |q| >= pivmin keeps 1 / q finite.
@@ -514,7 +559,7 @@ __device__ double bisect_kth(const double *d, const double *e2, int n, int j, do
 // at 256 points spaced a quarter octave apart below the Gershgorin bound (one
 // per thread), so that each eigenvalue starts from a bracket of relative width
 // 2^(1/4) - 1 instead of the whole spectrum; an eigenvalue whose bracket falls
-// below the drop threshold (kDropRel2Tri times a lower bound of the largest
+// below the drop threshold (u.drop_rel2 times a lower bound of the largest
 // eigenvalue) stops there.  The largest eigenvalue itself is bisected once per
 // block, by the thread that owns index 0 in the first block and by thread 0 in
 // the others.
@@ -603,7 +648,7 @@ __global__ __launch_bounds__(256) void k_bisect(const UpdDesc *__restrict__ Ud,
     const int m = threadIdx.x + 1;  // eigenvalues in [xs[m], xs[m - 1])
     const int cnt = cs[m - 1] - cs[m];
     const double x = xs[m];
-    if (cnt > 0 && unscale > 0.0 && x > 0.0 && x > kDropRel2Tri * hi) {
+    if (cnt > 0 && unscale > 0.0 && x > 0.0 && x > u.drop_rel2 * hi) {
       const int b = hist_bin(sqrt(x * unscale));
       if (b >= 0) atomicAdd(&u.hist[b], cnt);
     }
@@ -641,7 +686,7 @@ __global__ __launch_bounds__(256) void k_bisect(const UpdDesc *__restrict__ Ud,
     if (tcut > 0.0 && !(sqrt(fmax(bhi, 0.0) * unscale) >= tcut)) {
       l = 0.5 * (blo + bhi);  // below every kept value: not refined
     } else {
-      l = bisect_kth(ds, e2s, n, j, blo, bhi, pivmin, kDropRel2Tri * lmax_lb);
+      l = bisect_kth(ds, e2s, n, j, blo, bhi, pivmin, u.drop_rel2 * lmax_lb);
       refined = true;
     }
   }
@@ -657,7 +702,7 @@ __global__ __launch_bounds__(256) void k_bisect(const UpdDesc *__restrict__ Ud,
   __syncthreads();
   if (j >= n) return;
   const double lmax = sh_lmax;
-  const bool keep = refined && unscale > 0.0 && lmax > 0.0 && l > kDropRel2Tri * lmax;
+  const bool keep = refined && unscale > 0.0 && lmax > 0.0 && l > u.drop_rel2 * lmax;
   lam[s.s_off + j] = l;  // of the scaled matrix, as the inverse iteration uses it
   S[s.s_off + j] = keep ? sqrt(l * unscale) : 0.0;
   flags[s.s_off + j] = keep ? 1 : 0;
@@ -1050,8 +1095,8 @@ static int side_stream(hipStream_t parent, SideStream **out) {
 int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
   if (g.n_tri == 0) return 0;
   if (g.tri_max_n > kTriMaxN) return fail(-1, "sector too large for the tridiagonal route (n > 2048)");
-  if (int rc = ensure_lds_limit((const void *)k_tridiag<256, 2>, 512 * 32)) return rc;
-  if (int rc = ensure_lds_limit((const void *)k_tridiag<512, 4>, kTriMaxN * 32)) return rc;
+  if (int rc = ensure_lds_limit((const void *)k_tridiag<256, 2, kTriSW, kTriOcc>, 512 * 32)) return rc;
+  if (int rc = ensure_lds_limit((const void *)k_tridiag<512, 4, kTriSW, kTriOcc>, kTriMaxN * 32)) return rc;
   int tk = timing_begin(ST_TRIDIAG, stream);
   // The list holds the two-stage sectors first, then the one-stage ones, each
   // part largest first.  The two-stage sectors (band reduction, then bulge
@@ -1082,14 +1127,15 @@ int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
   if (int rc = launch_band(g, w, stream)) return rc;
   const int2 *tl = glist(w, g.o_tri) + g.n_ts;
   if (nbig > 0) {
-    hipLaunchKernelGGL((k_tridiag<512, 4>), dim3(nbig), dim3(512), (size_t)g.tri_max_n * 32,
-                       big_on_side ? sst : stream, gdesc(g, w), tl, 1, kTriMaxN);
+    hipLaunchKernelGGL((k_tridiag<512, 4, kTriSW, kTriOcc>), dim3(nbig), dim3(512),
+                       (size_t)g.tri_max_n * 32, big_on_side ? sst : stream, gdesc(g, w), tl, nbig,
+                       1, kTriMaxN);
     BSMPO_HIP(hipGetLastError());
   }
   if (nbig < n1) {
-    hipLaunchKernelGGL((k_tridiag<256, 2>), dim3(n1 - nbig), dim3(256),
-                       (size_t)std::min(split, g.tri_max_n) * 32, sst, gdesc(g, w), tl + nbig, 1,
-                       split);
+    hipLaunchKernelGGL((k_tridiag<256, 2, kTriSW, kTriOcc>), dim3(n1 - nbig), dim3(256),
+                       (size_t)std::min(split, g.tri_max_n) * 32, sst, gdesc(g, w), tl + nbig,
+                       n1 - nbig, 1, split);
     BSMPO_HIP(hipGetLastError());
   }
   if (both) {

@@ -212,6 +212,7 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
     D.n_lblk = h->n_lblk;
     D.n_rblk = h->n_rblk;
     D.ts_min = h->ts_min;
+    D.drop_rel2 = h->drop_rel * h->drop_rel;
     for (int c = 0; c < kTileClasses; ++c)
       for (int i = h->tile_off[c]; i < h->tile_off[c + 1]; ++i) tiles[tile_fill[c]++] = make_int2(u, i);
     if (contract_only) continue;  // the SVD-stage lists stay empty (zero)

@@ -56,6 +56,7 @@ int eig_threshold();  // sectors with n > threshold take the tridiagonal route
 int two_stage_min();  // tridiagonal-route sectors with n >= this take the two-stage reduction
 int tri_split();      // many-sector groups: Householder blocks of 512 threads for n > split
 double tie_rel();     // relative width of the tie band at the chi-th singular value
+double drop_rel();    // singular values <= drop_rel * the sector's largest are dropped
 
 // Singular-value histogram of an update (counts of surely-kept values in bins
 // of a quarter octave, bin b holding values >= 2^((b - kHistZero) / 4)) and
@@ -107,6 +108,7 @@ struct PlanHeader {
   // two-stage threshold in effect, and the bulge-chasing back-transformation
   // items (chunks of kept vectors) of the two-stage sectors
   int32_t ts_min, n_bc;
+  double drop_rel;  // drop rule in effect (on sigma, relative to the sector's largest)
 };
 
 // Tile formulation of the sector contraction.  A TileItem is one pass over

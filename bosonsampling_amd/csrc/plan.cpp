@@ -23,11 +23,13 @@ static thread_local std::string g_last_error;
 static std::atomic<int> g_eig_threshold{96};
 static std::atomic<int> g_tri_split{384};
 static std::atomic<double> g_tie_rel{1e-6};
+static std::atomic<double> g_drop_rel{1e-6};
 static std::atomic<int> g_two_stage{0};
 int eig_threshold() { return g_eig_threshold; }
 int two_stage_min() { return g_two_stage; }
 int tri_split() { return g_tri_split; }
 double tie_rel() { return g_tie_rel; }
+double drop_rel() { return g_drop_rel; }
 
 void set_error(const std::string &msg) { g_last_error = msg; }
 int fail(int code, const std::string &msg) {
@@ -213,6 +215,7 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
     for (int i = 0; i < b.sectors[sid].n; ++i) b.sv_sector.push_back(sid);
   b.h.eig_threshold = thr;
   b.h.tie_rel = tie_rel();
+  b.h.drop_rel = drop_rel();
   const int tsm = two_stage_min();
   b.h.ts_min = tsm;
   int64_t tw = 0;
@@ -514,7 +517,7 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
 struct Cached {
   bool valid = false;
   int d = 0, chi = 0, t0 = 0, t1 = 0, thr = 0, tsm = 0;
-  double tie = 0.0;
+  double tie = 0.0, drop = 0.0;
   std::vector<int32_t> key;
   Built b;
 };
@@ -526,6 +529,7 @@ int build_cached(int d, int chi, const int32_t *CL, const int32_t *CC, const int
   const size_t n = (size_t)2 * chi;
   bool hit = c.valid && c.d == d && c.chi == chi && c.t0 == t0 && c.t1 == t1 &&
              c.thr == eig_threshold() && c.tsm == two_stage_min() && c.tie == tie_rel() &&
+             c.drop == drop_rel() &&
              c.key.size() == 3 * n &&
              !std::memcmp(c.key.data(), CL, n * 4) && !std::memcmp(c.key.data() + n, CC, n * 4) &&
              !std::memcmp(c.key.data() + 2 * n, CR, n * 4);
@@ -539,6 +543,7 @@ int build_cached(int d, int chi, const int32_t *CL, const int32_t *CC, const int
     std::memcpy(c.key.data() + n, CC, n * 4);
     std::memcpy(c.key.data() + 2 * n, CR, n * 4);
     c.d = d; c.chi = chi; c.t0 = t0; c.t1 = t1; c.thr = eig_threshold(); c.tie = tie_rel();
+    c.drop = drop_rel();
     c.tsm = two_stage_min();
     c.valid = true;
   }
@@ -570,6 +575,14 @@ extern "C" {
 const char *bsmpo_last_error(void) { return bsmpo::g_last_error.c_str(); }
 
 int bsmpo_abi_version(void) { return 1; }
+
+int bsmpo_set_drop_rel(double rel) {
+  if (!(rel >= 1e-8 && rel <= 1e-3)) return fail(-1, "drop threshold must be in [1e-8, 1e-3]");
+  g_drop_rel = rel;
+  return 0;
+}
+
+double bsmpo_drop_rel(void) { return g_drop_rel; }
 
 int bsmpo_set_tie_rel(double rel) {
   if (!(rel >= 0.0 && rel < 1e-2)) return fail(-1, "tie band must be in [0, 1e-2)");

@@ -186,9 +186,9 @@ __global__ __launch_bounds__(256) void k_gram(const UpdDesc *__restrict__ Ud,
 // Column pairs follow the round-robin tournament; 8 lanes per pair.
 constexpr int kJacSweeps = 40;
 constexpr double kJacTol = 1e-15;
-// singular values with sigma <= 1e-6 sigma_max(sector) are the Gram route's
-// noise floor and are dropped (sigma := 0, vectors 0).
-constexpr double kDropRel2 = 1e-12;  // on lambda = sigma^2
+// singular values with sigma <= drop_rel sigma_max(sector) (default 1e-6: the
+// Gram route's noise floor; bsmpo_set_drop_rel) are dropped (sigma := 0,
+// vectors 0).  UpdDesc::drop_rel2 holds drop_rel^2, on lambda = sigma^2.
 
 __device__ __forceinline__ void rr_pair(int step, int i, int n2, int &p, int &q) {
   if (i == 0) {
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(256) void k_jacobi(const UpdDesc *__restrict__ Ud,
   __syncthreads();
   double2 *Yout = s.left_small ? reinterpret_cast<double2 *>(u.V) + s.v_off
                                : reinterpret_cast<double2 *>(u.W) + s.w_off;
-  const double thr = kDropRel2 * lam_max;
+  const double thr = u.drop_rel2 * lam_max;
   for (int p = threadIdx.x; p < n; p += blockDim.x) {
     const double lp = lam[s.s_off + p];
     const bool keep = lam_max > 0.0 && lp > thr;

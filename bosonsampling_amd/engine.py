@@ -353,6 +353,23 @@ class GroupRunner:
             st.charge_host[mode + 1] = ch
 
 
+# Truncation rules per contraction precision: (tie band, drop rule), both
+# relative.  Rounding of the contraction moves degenerate twin singular values
+# apart by about its relative error, and leaves a floor of noise singular
+# values below which nothing is signal: complex64 about 1e-7, split BF16x3
+# about 1e-6 (hence its wider tie band and its drop rule at 1e-5 sigma_max).
+PRECISION_RULES = {C128: (1e-6, 1e-6), C64: (1e-6, 1e-6), BF16X3: (1e-4, 1e-5)}
+
+
+def set_precision_rules(precision: int):
+    """Set the library's tie band and drop rule (process-wide, plans built
+    afterwards) for a contraction precision; returns (tie_rel, drop_rel)."""
+    tie, drop = PRECISION_RULES[int(precision)]
+    check(lib().bsmpo_set_tie_rel(tie), "set_tie_rel")
+    check(lib().bsmpo_set_drop_rel(drop), "set_drop_rel")
+    return tie, drop
+
+
 def simulate_batch(states, meshes, input_modes, layers=None, precision=C128, runner=None,
                    observe=False):
     """Layer-synchronous simulation of several independent instances: every

@@ -186,6 +186,7 @@ def main():
     from . import engine
     from .worker import Worker
     prec = {"c128": engine.C128, "c64": engine.C64, "bf16x3": engine.BF16X3}[args.precision]
+    engine.set_precision_rules(prec)
     run_queue(args.experiments, lambda: Worker(prec), args.results, args.seed,
               allow_pickle=args.allow_pickle)
 

@@ -82,6 +82,21 @@ int bsmpo_two_stage_threshold(void);
  */
 int bsmpo_set_tie_rel(double rel);
 
+/*
+ * Drop rule (process-wide, affects plans built afterwards): a singular value
+ * sigma <= rel * sigma_max of its sector is dropped with its vectors before
+ * the truncation; the site update's adjoint uses the same rule.  Default
+ * 1e-6, the noise floor of the Gram route in FP64 (sigma_min / sigma_max ~
+ * sqrt(eps)); a contraction of lower precision has a higher floor (split
+ * BF16x3: about 1e-5), below which the values are rounding noise that the
+ * inverse iteration would otherwise treat as one long cluster.  Accepted
+ * range [1e-8, 1e-3].  The reference keeps every value
+ * (src/beam_splitter_parallel/Rank.py:139-145).
+ */
+int bsmpo_set_drop_rel(double rel);
+/* The drop rule in effect. */
+double bsmpo_drop_rel(void);
+
 /* Contraction arithmetic of the sector GEMMs (the 16 x 16 block products
  * P_q = Glc Gcr of the sector contraction; the BS-weight transforms that
  * follow run on the vector ALUs in the same precision). */

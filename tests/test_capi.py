@@ -123,6 +123,29 @@ def test_launch_knobs_validate_their_range():
     assert L.bsmpo_set_eig_threshold(96) == 0
 
 
+def test_drop_rule_knob_and_precision_rules():
+    """The drop rule is a process-wide knob in [1e-8, 1e-3] (default 1e-6,
+    the oracle's DROP_REL); a value outside it is refused and leaves the
+    setting as it was.  The per-precision rules set both truncation knobs and
+    are widest for split BF16x3."""
+    from bosonsampling_amd import engine
+    from bosonsampling_amd._capi import lib
+    from oracle import mpo
+    L = lib()
+    assert L.bsmpo_drop_rel() == mpo.DROP_REL == 1e-6
+    assert L.bsmpo_set_drop_rel(1e-2) < 0 and b"drop" in L.bsmpo_last_error()
+    assert L.bsmpo_set_drop_rel(0.0) < 0
+    assert L.bsmpo_drop_rel() == 1e-6
+    try:
+        assert engine.set_precision_rules(engine.BF16X3) == (1e-4, 1e-5)
+        assert L.bsmpo_drop_rel() == 1e-5
+        for p in (engine.C128, engine.C64):
+            assert engine.PRECISION_RULES[p] == (mpo.TIE_REL, mpo.DROP_REL)
+    finally:
+        engine.set_precision_rules(engine.C128)
+    assert L.bsmpo_drop_rel() == 1e-6
+
+
 def test_group_runner_stream_count():
     """GroupRunner takes 1..4 launching streams; a graph-replay runner uses one."""
     from bosonsampling_amd import engine

@@ -29,11 +29,6 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 TOL = 1e-10
 
 
-def _set_tie(rel):
-    from bosonsampling_amd._capi import check, lib
-    check(lib().bsmpo_set_tie_rel(rel), "set_tie_rel")
-
-
 @pytest.mark.timeout(600)
 def test_m32_plain_matches_oracle_fixture(gpu):
     """M=32, photons in modes 0,4,..,28, eta=0.7, d=9, chi=256, Haar seed 1
@@ -172,17 +167,18 @@ def test_low_precision_against_c128_m64(gpu, chi, layers):
     """M=64, d=13, seed 100: the grouped complex64 and split-BF16x3
     contractions against the plain complex128 run after the same layers.
     Output photon-number marginals and trace: complex64 within 1e-5 (the
-    target); BF16x3 within 1e-3 (measured ~1e-5).  The tie band of the
-    truncation is set above each contraction's rounding noise (1e-6 for
-    complex64, 1e-4 for BF16x3) on both sides of the comparison."""
+    target); BF16x3 within 1e-3 (measured ~1e-5).  The truncation rules of
+    each precision (tie band above its rounding noise, drop rule above its
+    noise floor: 1e-6 / 1e-6 for complex64, 1e-4 / 1e-5 for BF16x3) hold on
+    both sides of the comparison."""
     from bosonsampling_amd import engine, mesh
     M, d, inputs = 64, 13, list(range(0, 60, 5))
     mm, _ = mesh.haar_clements_mesh(M, 100)
     init = engine.initialize(M, d, chi, inputs, 0.6, device=gpu)
     errs = {}
     try:
-        for prec, band, bound in ((engine.C64, 1e-6, 1e-5), (engine.BF16X3, 1e-4, 1e-3)):
-            _set_tie(band)
+        for prec, bound in ((engine.C64, 1e-5), (engine.BF16X3, 1e-3)):
+            band, _ = engine.set_precision_rules(prec)
             ref = init.clone()
             print(f"chi={chi} plain c128 (band {band})", flush=True)
             engine.simulate(ref, mm, inputs, layers=range(layers), observe=False)
@@ -196,4 +192,4 @@ def test_low_precision_against_c128_m64(gpu, chi, layers):
                   f"trace error {errs[prec][1]:.2e}, trace {tr:.6f}", flush=True)
             assert errs[prec][0] <= bound and errs[prec][1] <= bound
     finally:
-        _set_tie(1e-6)
+        engine.set_precision_rules(engine.C128)
