@@ -133,12 +133,14 @@ __device__ __forceinline__ gptr<double2> wy_tstore(const UpdDesc &u, const Secto
 // T of block `idx` of a sector, written to its work area (the inverse
 // iteration's arrays there are dead by now): blocks after one another,
 // 32 x 32 row-major each.
-__global__ __launch_bounds__(256) void k_wy_tmat(const UpdDesc *__restrict__ Ud,
-                                                 const int2 *__restrict__ list, int nlist) {
+__global__ __launch_bounds__(256) void k_wy_tmat(const UpdDesc *__restrict__ Ud, WorkSeg ws) {
   __shared__ double2 Vs[kWyRows * kWyLd];
   __shared__ double2 Ss[kWy * kWyLd];
   __shared__ double2 Ts[kWy * kWyLd];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int2 sg = *ws.seg;
+  const int2 *__restrict__ list = ws.list + sg.x;
+  const int nlist = sg.y;
   for (int it = blockIdx.x; it < nlist; it += gridDim.x) {
     const WyItem w = wy_item(list[it]);
     const UpdDesc &u = Ud[w.u];
@@ -189,14 +191,16 @@ __global__ __launch_bounds__(256) void k_wy_tmat(const UpdDesc *__restrict__ Ud,
 }
 
 // Z <- B_0 B_1 ... B_last Z for the panel of vectors j0 .. j0 + kp - 1.
-__global__ __launch_bounds__(256) void k_wy_apply(const UpdDesc *__restrict__ Ud,
-                                                  const int2 *__restrict__ list, int nlist) {
+__global__ __launch_bounds__(256) void k_wy_apply(const UpdDesc *__restrict__ Ud, WorkSeg ws) {
   __shared__ double2 Vs[kWyRows * kWyLd];
   __shared__ double2 Zs[kWyRows * kWyLd];
   __shared__ double2 Ts[kWy * kWyLd];
   __shared__ double2 Ws[kWy * kWyLd];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int ta = wave >> 1, tb = wave & 1;  // this wave's 16 x 16 tile of a 32 x 32 product
+  const int2 sg = *ws.seg;
+  const int2 *__restrict__ list = ws.list + sg.x;
+  const int nlist = sg.y;
   for (int it = blockIdx.x; it < nlist; it += gridDim.x) {
     const WyItem w = wy_item(list[it]);
     const UpdDesc &u = Ud[w.u];
@@ -270,11 +274,12 @@ __global__ __launch_bounds__(256) void k_wy_apply(const UpdDesc *__restrict__ Ud
 int launch_wy_backtransform(const GroupHeader &g, const char *w, hipStream_t stream) {
   if (g.n_wyt == 0) return 0;
   const UpdDesc *desc = reinterpret_cast<const UpdDesc *>(w + g.o_desc);
+  const int2 *segs = reinterpret_cast<const GroupCounts *>(w + g.o_counts)->seg;
   hipLaunchKernelGGL(k_wy_tmat, dim3(std::min(g.n_wyt, 16384)), dim3(256), 0, stream, desc,
-                     reinterpret_cast<const int2 *>(w + g.o_wyt), g.n_wyt);
+                     WorkSeg{reinterpret_cast<const int2 *>(w + g.o_wyt), segs + SG_WYT});
   BSMPO_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_wy_apply, dim3(std::min(g.n_wya, 16384)), dim3(256), 0, stream, desc,
-                     reinterpret_cast<const int2 *>(w + g.o_wya), g.n_wya);
+                     WorkSeg{reinterpret_cast<const int2 *>(w + g.o_wya), segs + SG_WYA});
   BSMPO_HIP(hipGetLastError());
   return 0;
 }

@@ -310,8 +310,7 @@ __device__ __forceinline__ void wave_products(const UpdDesc &u, const int *scb, 
 // pass's left blocks and column offsets of the tile's right blocks per
 // position).
 template <int PREC, bool LCW>
-__global__ __launch_bounds__(256) void k_contract_tile(const UpdDesc *__restrict__ Ud,
-                                                       const int2 *__restrict__ list, int nlist,
+__global__ __launch_bounds__(256) void k_contract_tile(const UpdDesc *__restrict__ Ud, WorkSeg ws,
                                                        int wtab_elems) {
   typedef typename Prod<PREC, LCW>::T T;
   extern __shared__ __align__(16) char lds_raw[];
@@ -331,6 +330,9 @@ __global__ __launch_bounds__(256) void k_contract_tile(const UpdDesc *__restrict
   // dispatched round robin over the 8 XCDs and the grid is a multiple of 8),
   // so each XCD walks one contiguous eighth of the list and the tiles that
   // share operand panels share its L2.
+  const int2 sg = *ws.seg;
+  const int2 *__restrict__ list = ws.list + sg.x;
+  const int nlist = sg.y;
   const int chunk = (nlist + 7) >> 3;
   const int total = chunk << 3;
   for (int p = blockIdx.x; p < total; p += gridDim.x) {
@@ -535,6 +537,7 @@ int launch_tile(const GroupHeader &g, const char *w, hipStream_t stream) {
   typedef typename Prod<PREC, LCW>::T T;
   const UpdDesc *desc = reinterpret_cast<const UpdDesc *>(w + g.o_desc);
   const int2 *list = reinterpret_cast<const int2 *>(w + g.o_tiles);
+  const int2 *segs = reinterpret_cast<const GroupCounts *>(w + g.o_counts)->seg;
   const int wtab_elems = (wtab_off(g.d_max) + 1) / 2 * 2;  // keeps P 16-byte aligned
   const size_t wbytes = sizeof(cx<T>) * (size_t)wtab_elems;
   if (int rc = ensure_lds_limit((const void *)k_contract_tile<PREC, LCW>,
@@ -548,7 +551,7 @@ int launch_tile(const GroupHeader &g, const char *w, hipStream_t stream) {
     const int grid = std::min((n + 7) / 8 * 8, 8192);
     const size_t lds = sizeof(cx<T>) * (size_t)kTileElems[c] + wbytes;
     hipLaunchKernelGGL((k_contract_tile<PREC, LCW>), dim3(grid), dim3(256), lds, stream, desc,
-                       list + g.tile_off[c], n, wtab_elems);
+                       WorkSeg{list, segs + SG_TILE + c}, wtab_elems);
     BSMPO_HIP(hipGetLastError());
   }
   return 0;

@@ -77,7 +77,6 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
 // skipped, a guard of the LDS sizing).
 constexpr int kTriNB = kTriPanel;
 constexpr int kTriMaxN = 2048;
-constexpr int kTriFewSectors = 256;  // below this many sectors per launch: one block shape
 // Streaming shape of the reduction.  Measured on the M=64, chi=1024 grouped
 // workload (one launching stream): strips of 4 columns at 3 waves per SIMD
 // (168 VGPRs, one trailing tile per pass) 1.7% faster, 4 waves per SIMD
@@ -482,12 +481,14 @@ __device__ __forceinline__ void tridiag_sector(const UpdDesc &u, const Sector &s
 // Blocks walk the list (grid-stride, largest sectors first); sectors with n
 // outside [nmin, nmax] belong to the other block shape's launch.
 template <int NT, int RPT, int SW, int OCC>
-__global__ __launch_bounds__(NT, OCC) void k_tridiag(const UpdDesc *__restrict__ Ud,
-                                                const int2 *__restrict__ list, int nlist,
+__global__ __launch_bounds__(NT, OCC) void k_tridiag(const UpdDesc *__restrict__ Ud, WorkSeg ws,
                                                 int nmin, int nmax) {
   extern __shared__ double2 tsm[];
   __shared__ double2 red[2][16];
   __shared__ double red4[NT / 64][4 * kTriNB];
+  const int2 sg = *ws.seg;
+  const int2 *__restrict__ list = ws.list + sg.x;
+  const int nlist = sg.y;
   for (int it = blockIdx.x; it < nlist; it += gridDim.x) {
     const int2 ref = list[it];
     const UpdDesc &u = Ud[ref.x];
@@ -581,14 +582,16 @@ __device__ __forceinline__ double hist_cut(const UpdDesc &u) {
   return *reinterpret_cast<const double *>(u.hist + kHistBins);
 }
 
-__global__ __launch_bounds__(256) void k_bisect(const UpdDesc *__restrict__ Ud,
-                                                const int2 *__restrict__ list, int pass) {
+__global__ __launch_bounds__(256) void k_bisect(const UpdDesc *__restrict__ Ud, WorkSeg ws,
+                                                int pass) {
   extern __shared__ double bsm[];
   __shared__ double sh_lmax;
   __shared__ double red[3][4];
   __shared__ double xs[kBisSamples + 1];
   __shared__ int cs[kBisSamples + 1];
-  const int2 ref = list[blockIdx.x];
+  const int2 sg = *ws.seg;
+  if ((int)blockIdx.x >= sg.y) return;  // a launch sized for the group's capacity
+  const int2 ref = ws.list[sg.x + blockIdx.x];
   const UpdDesc &u = Ud[ref.x];
   const Sector s = u.sectors[ref.y];
   const int n = s.n;
@@ -709,15 +712,18 @@ __global__ __launch_bounds__(256) void k_bisect(const UpdDesc *__restrict__ Ud,
 }
 
 // the histogram of every update of the group, and its cutoff, cleared
-__global__ void k_hist_zero(const UpdDesc *__restrict__ Ud) {
+__global__ void k_hist_zero(const UpdDesc *__restrict__ Ud, const int2 *__restrict__ useg) {
+  if ((int)blockIdx.x >= useg->y) return;  // a skipped group
   int32_t *h = Ud[blockIdx.x].hist;
   for (int i = threadIdx.x; i < kHistBins + 2; i += blockDim.x) h[i] = 0;
 }
 
 // the kept (flagged) singular values of the Jacobi-route sectors, exact,
 // added to their update's histogram: one block per sector
-__global__ void k_hist_jacobi(const UpdDesc *__restrict__ Ud, const int2 *__restrict__ list) {
-  const int2 ref = list[blockIdx.x];
+__global__ void k_hist_jacobi(const UpdDesc *__restrict__ Ud, WorkSeg ws) {
+  const int2 sg = *ws.seg;
+  if ((int)blockIdx.x >= sg.y) return;
+  const int2 ref = ws.list[sg.x + blockIdx.x];
   const UpdDesc &u = Ud[ref.x];
   const Sector s = u.sectors[ref.y];
   for (int r = threadIdx.x; r < s.n; r += blockDim.x) {
@@ -734,8 +740,10 @@ __global__ void k_hist_jacobi(const UpdDesc *__restrict__ Ud, const int2 *__rest
 // bin is at least that bin's bound), 0 if there is none.  The chi-th largest
 // kept singular value is then >= t, so nothing below t (1 - tie band) is
 // kept; pass 1 of k_bisect refines every eigenvalue above t / 2.
-__global__ __launch_bounds__(256) void k_cutoff(const UpdDesc *__restrict__ Ud) {
+__global__ __launch_bounds__(256) void k_cutoff(const UpdDesc *__restrict__ Ud,
+                                                const int2 *__restrict__ useg) {
   __shared__ int part[256];
+  if ((int)blockIdx.x >= useg->y) return;  // a skipped group
   const UpdDesc &u = Ud[blockIdx.x];
   constexpr int kPer = kHistBins / 256;
   int a = 0;
@@ -779,10 +787,11 @@ int kept_flops_read(double *out, bool reset) {
   return 0;
 }
 
-__global__ void k_kcount(const UpdDesc *__restrict__ Ud, const int2 *__restrict__ list,
-                         int count_flops) {
+__global__ void k_kcount(const UpdDesc *__restrict__ Ud, WorkSeg ws, int count_flops) {
   __shared__ int red[8];
-  const int2 ref = list[blockIdx.x];
+  const int2 sg = *ws.seg;
+  if ((int)blockIdx.x >= sg.y) return;
+  const int2 ref = ws.list[sg.x + blockIdx.x];
   const UpdDesc &u = Ud[ref.x];
   const Sector s = u.sectors[ref.y];
   int c = 0;
@@ -960,11 +969,13 @@ __device__ __forceinline__ void vec_lanes(PT base, int64_t sa, int si, int P, in
 
 template <bool kInLds>
 __global__ __launch_bounds__(kInLds ? kVecLdsThreads : kTriVecThreads) void k_tri_vectors(
-    const UpdDesc *__restrict__ Ud, const int2 *__restrict__ list, int nsplit) {
+    const UpdDesc *__restrict__ Ud, WorkSeg ws, int nsplit) {
   extern __shared__ double tvs[];  // d, e (2 n), lambda (k), cluster starts (k + 1), lane arrays
   __shared__ int ncl;
   __shared__ double sh_tn[2][kTriVecThreads / 64];
-  const int2 ref = list[blockIdx.x];
+  const int2 sg = *ws.seg;
+  if ((int)blockIdx.x >= sg.y) return;  // a launch sized for the group's capacity
+  const int2 ref = ws.list[sg.x + blockIdx.x];
   const UpdDesc &u = Ud[ref.x];
   const Sector s = u.sectors[ref.y];
   const int n = s.n, k = u.kcount[ref.y];
@@ -1056,8 +1067,11 @@ static int vec_lds_nsplit() {
 static inline const UpdDesc *gdesc(const GroupHeader &g, const char *w) {
   return reinterpret_cast<const UpdDesc *>(w + g.o_desc);
 }
-static inline const int2 *glist(const char *w, int64_t off) {
-  return reinterpret_cast<const int2 *>(w + off);
+static inline const int2 *gseg(const GroupHeader &g, const char *w, int seg) {
+  return reinterpret_cast<const GroupCounts *>(w + g.o_counts)->seg + seg;
+}
+static inline WorkSeg wseg(const GroupHeader &g, const char *w, int64_t o_list, int seg) {
+  return WorkSeg{reinterpret_cast<const int2 *>(w + o_list), gseg(g, w, seg)};
 }
 
 // Side stream (default priority, non-blocking) with a fork and a join event,
@@ -1125,17 +1139,16 @@ int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
   }
   const hipStream_t sst = both ? ss->stream : stream;
   if (int rc = launch_band(g, w, stream)) return rc;
-  const int2 *tl = glist(w, g.o_tri) + g.n_ts;
   if (nbig > 0) {
     hipLaunchKernelGGL((k_tridiag<512, 4, kTriSW, kTriOcc>), dim3(nbig), dim3(512),
-                       (size_t)g.tri_max_n * 32, big_on_side ? sst : stream, gdesc(g, w), tl, nbig,
-                       1, kTriMaxN);
+                       (size_t)g.tri_max_n * 32, big_on_side ? sst : stream, gdesc(g, w),
+                       wseg(g, w, g.o_tri, SG_TRIBIG), 1, kTriMaxN);
     BSMPO_HIP(hipGetLastError());
   }
   if (nbig < n1) {
     hipLaunchKernelGGL((k_tridiag<256, 2, kTriSW, kTriOcc>), dim3(n1 - nbig), dim3(256),
-                       (size_t)std::min(split, g.tri_max_n) * 32, sst, gdesc(g, w), tl + nbig,
-                       n1 - nbig, 1, split);
+                       (size_t)std::min(split, g.tri_max_n) * 32, sst, gdesc(g, w),
+                       wseg(g, w, g.o_tri, SG_TRISMALL), 1, split);
     BSMPO_HIP(hipGetLastError());
   }
   if (both) {
@@ -1146,20 +1159,23 @@ int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
   tk = timing_begin(ST_BISECT, stream);
   // the update's cutoff from sample counts (tridiagonal route) and exact
   // values (Jacobi route), then the eigenvalues above half of it
-  hipLaunchKernelGGL(k_hist_zero, dim3(g.n_upd), dim3(256), 0, stream, gdesc(g, w));
+  hipLaunchKernelGGL(k_hist_zero, dim3(g.n_upd), dim3(256), 0, stream, gdesc(g, w),
+                     gseg(g, w, SG_UPD));
   BSMPO_HIP(hipGetLastError());
   const int njac = g.jac_off[kTriClass];
   if (njac > 0) {
-    hipLaunchKernelGGL(k_hist_jacobi, dim3(njac), dim3(64), 0, stream, gdesc(g, w), glist(w, g.o_jac));
+    hipLaunchKernelGGL(k_hist_jacobi, dim3(njac), dim3(64), 0, stream, gdesc(g, w),
+                       wseg(g, w, g.o_jac, SG_JACALL));
     BSMPO_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_bisect, dim3(g.n_tri, 1), dim3(256), (size_t)g.tri_max_n * 16, stream,
-                     gdesc(g, w), glist(w, g.o_tri), 0);
+                     gdesc(g, w), wseg(g, w, g.o_tri, SG_TRI), 0);
   BSMPO_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_cutoff, dim3(g.n_upd), dim3(256), 0, stream, gdesc(g, w));
+  hipLaunchKernelGGL(k_cutoff, dim3(g.n_upd), dim3(256), 0, stream, gdesc(g, w),
+                     gseg(g, w, SG_UPD));
   BSMPO_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_bisect, dim3(g.n_tri, (g.tri_max_n + 255) / 256), dim3(256),
-                     (size_t)g.tri_max_n * 16, stream, gdesc(g, w), glist(w, g.o_tri), 1);
+                     (size_t)g.tri_max_n * 16, stream, gdesc(g, w), wseg(g, w, g.o_tri, SG_TRI), 1);
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, 0.0, 0.0);
   return 0;
@@ -1168,7 +1184,7 @@ int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
 int launch_kcount(const GroupHeader &g, const char *w, hipStream_t stream) {
   if (g.n_sec == 0) return 0;
   hipLaunchKernelGGL(k_kcount, dim3(g.n_sec), dim3(256), 0, stream, gdesc(g, w),
-                     glist(w, g.o_sec), timing_on() ? 1 : 0);
+                     wseg(g, w, g.o_sec, SG_SEC), timing_on() ? 1 : 0);
   BSMPO_HIP(hipGetLastError());
   return 0;
 }
@@ -1178,12 +1194,12 @@ int launch_tri_vectors(const GroupHeader &g, const char *w, hipStream_t stream) 
   int tk = timing_begin(ST_TRIVEC, stream);
   static const int nsplit = vec_lds_nsplit();
   hipLaunchKernelGGL((k_tri_vectors<true>), dim3(g.n_tri), dim3(kVecLdsThreads), (size_t)kVecLds,
-                     stream, gdesc(g, w), glist(w, g.o_tri), nsplit);
+                     stream, gdesc(g, w), wseg(g, w, g.o_tri, SG_TRI), nsplit);
   BSMPO_HIP(hipGetLastError());
   // sectors past nsplit, or with more kept vectors than the LDS lanes take
   hipLaunchKernelGGL((k_tri_vectors<false>), dim3(g.n_tri), dim3(kTriVecThreads),
                      (size_t)vec_lds_head(g.tri_max_n, g.tri_max_n), stream, gdesc(g, w),
-                     glist(w, g.o_tri), nsplit);
+                     wseg(g, w, g.o_tri, SG_TRI), nsplit);
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, 0.0, 0.0);
   tk = timing_begin(ST_BACKTRANS, stream);

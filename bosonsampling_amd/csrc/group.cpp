@@ -23,6 +23,34 @@ struct Layout {
 
 inline int64_t region(int64_t bytes) { return align_up(std::max<int64_t>(bytes, 8), 256); }
 
+// The device counts of a built group (its image's GroupCounts).  The split of
+// the one-stage tridiagonal sectors between the two block shapes follows
+// launch_tri_values: with few of them (a single update) all take the larger
+// shape.
+void write_counts(const GroupHeader &g, char *img) {
+  GroupCounts &k = *reinterpret_cast<GroupCounts *>(img + g.o_counts);
+  std::memset(&k, 0, sizeof(k));
+  k.seg[SG_UPD] = make_int2(0, g.n_upd);
+  for (int c = 0; c < kTileClasses; ++c)
+    k.seg[SG_TILE + c] = make_int2(g.tile_off[c], g.tile_off[c + 1] - g.tile_off[c]);
+  k.seg[SG_GRAM] = make_int2(0, g.n_gram);
+  for (int c = 0; c < kTriClass; ++c)
+    k.seg[SG_JAC + c] = make_int2(g.jac_off[c], g.jac_off[c + 1] - g.jac_off[c]);
+  k.seg[SG_JACALL] = make_int2(0, g.jac_off[kTriClass]);
+  k.seg[SG_TRI] = make_int2(0, g.n_tri);
+  k.seg[SG_TS] = make_int2(0, g.n_ts);
+  const int n1 = g.n_tri - g.n_ts;
+  const int nbig = n1 < kTriFewSectors ? n1 : g.tri_nbig;
+  k.seg[SG_TRIBIG] = make_int2(g.n_ts, nbig);
+  k.seg[SG_TRISMALL] = make_int2(g.n_ts + nbig, n1 - nbig);
+  k.seg[SG_SEC] = make_int2(0, g.n_sec);
+  k.seg[SG_V] = make_int2(0, g.n_v);
+  k.seg[SG_W] = make_int2(0, g.n_w);
+  k.seg[SG_WYT] = make_int2(0, g.n_wyt);
+  k.seg[SG_WYA] = make_int2(0, g.n_wya);
+  k.seg[SG_BC] = make_int2(0, g.n_bc);
+}
+
 int layout(int n, const void *const *plans, Layout &L) {
   if (n <= 0) return fail(-1, "group: no updates");
   GroupHeader &g = L.g;
@@ -80,6 +108,7 @@ int layout(int n, const void *const *plans, Layout &L) {
     o = align_up(o + std::max<int64_t>(bytes, 8), 256);
     return at;
   };
+  g.o_counts = place((int64_t)sizeof(GroupCounts));
   g.o_desc = place((int64_t)sizeof(UpdDesc) * n);
   g.o_tiles = place(8 * n_tiles);
   g.o_gram = place(8 * n_gram);
@@ -226,6 +255,7 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
   if (contract_only) {  // a group that only runs the contraction stage
     g.n_gram = g.n_tri = g.n_sec = g.n_v = g.n_w = g.n_wyt = g.n_wya = g.n_bc = 0;
     for (int c = 0; c <= kTriClass; ++c) g.jac_off[c] = 0;
+    write_counts(g, img);
     std::memcpy(host_blob, &g, sizeof(GroupHeader));
     return 0;
   }
@@ -282,6 +312,7 @@ int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_wor
     }
   }
   g.jac_off[kTriClass] = (int32_t)nj;
+  write_counts(g, img);
   std::memcpy(host_blob, &g, sizeof(GroupHeader));
   return 0;
 }

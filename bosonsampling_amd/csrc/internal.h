@@ -176,6 +176,44 @@ struct Item {
 //   [UpdDesc x n_upd][work lists (int2 pairs)][plan device parts] = staging,
 //   uploaded from the host image in one copy, then each update's scratch.
 constexpr int32_t kGroupMagic = 0x4253475a;  // "BSGZ"
+
+// Work-list segments of a group: (offset into its list, count), kept in
+// device memory at the front of the staging region (GroupCounts), so that
+// every stage kernel reads how much work there is from the device.  A group
+// planned on the device launches with grids sized for its capacity, and the
+// counts it planned never pass through the host.
+enum GroupSeg {
+  SG_UPD = 0,                          // the updates (per-update kernels); 0 for a skipped group
+  SG_TILE = 1,                         // + LDS class c: contraction tile items
+  SG_GRAM = SG_TILE + kTileClasses,    // Gram blocks
+  SG_JAC = SG_GRAM + 1,                // + size class c < kTriClass: one-sided Jacobi sectors
+  SG_JACALL = SG_JAC + kTriClass,      // every Jacobi-route sector
+  SG_TRI = SG_JACALL + 1,              // every tridiagonal-route sector (two-stage ones first)
+  SG_TS = SG_TRI + 1,                  // the two-stage ones
+  SG_TRIBIG = SG_TS + 1,               // one-stage sectors on 512-thread blocks
+  SG_TRISMALL = SG_TRIBIG + 1,         // one-stage sectors on 256-thread blocks
+  SG_SEC = SG_TRISMALL + 1,            // every sector
+  SG_V = SG_SEC + 1,                   // V = T Y / sigma blocks
+  SG_W = SG_V + 1,                     // W = V^H C blocks
+  SG_WYT = SG_W + 1,                   // WY blocks of reflectors (T factors)
+  SG_WYA = SG_WYT + 1,                 // panels of kept vectors (WY application)
+  SG_BC = SG_WYA + 1,                  // bulge-chasing back-transformation chunks
+  SG_COUNT = SG_BC + 1
+};
+struct GroupCounts {
+  int2 seg[SG_COUNT];
+  int32_t status;  // device planning: 0, or why the group was skipped (all counts zero)
+  int32_t pad[3];
+};
+// A stage's work: the group's list array and the segment it launches over.
+struct WorkSeg {
+  const int2 *list;
+  const int2 *seg;
+};
+// below this many one-stage tridiagonal sectors in a launch (a single update):
+// one block shape for all of them
+constexpr int kTriFewSectors = 256;
+
 struct GroupHeader {
   int32_t magic, n_upd, chi, pad0;
   int32_t n_gram, n_tri, n_sec, n_v, n_w, pad4;
@@ -199,6 +237,8 @@ struct GroupHeader {
   // back-transformation items (sector | chunk << 20)
   int32_t n_ts, ts_nbig, n_bc, ts_max_n;
   int64_t o_bc;
+  int64_t o_counts;  // GroupCounts at the front of the staging region
+  int32_t dev_planned, pad_dp;  // counts are capacities: the group was planned on the device
 };
 constexpr int64_t kGroupHostHead = 1024;  // host blob: header, then the staging image
 static_assert(sizeof(GroupHeader) <= kGroupHostHead, "group header too large");
@@ -259,6 +299,162 @@ int group_query(int n, const void *const *plans, int64_t *host_bytes, int64_t *w
 // work lists stay empty); no heap allocation in either case
 int group_build(int n, const void *const *plans, const UpdateIO *io, char *d_work,
                 char *host_blob, int64_t host_bytes, bool contract_only = false);
+
+// ------------------------------------------------------ device planning --
+// A layer program (layer.cpp) plans its groups on the device (devplan.hip):
+// the same Sector / Block / Item / TileItem arrays and work lists the host
+// planner builds, written into arenas of fixed capacity.  Per update, a fixed
+// part (tables whose size depends on d and chi only) and a variable part
+// (pools and item arrays) placed by a scan over the group's updates.
+struct DpUpdate {
+  const int32_t *CL, *CC, *CR;  // charges of bonds mode, mode + 1, mode + 2 (chi x 2)
+  double *Glc, *Gcr;
+  const double *LR;
+  double *U3;                   // the program's BS table of this update
+  int32_t *charge_c;
+  double *lambda_c;
+  int32_t *num_lambda;
+  int32_t edge, group;          // group: index of the update's group
+  int32_t local, pad;           // position inside the group
+};
+// sizes found by the first planning kernel, offsets by the group scan
+struct DpSizes {
+  int32_t status, n_sec, n_lblk, n_cblk, n_rblk, rows_total, cols_total, n_gram;
+  int32_t n_w, n_v, sv_total, n_tri, n_titems, pad0;
+  int32_t jac_cnt[kTriClass], tile_cnt[kTileClasses];
+  int64_t c_elems, g_elems, v_elems, w_elems, tw;
+  int64_t var_bytes, work_bytes, var_off, work_off;
+  // positions of the update's entries in the group's work lists
+  int32_t l_tile[kTileClasses], l_jac[kTriClass];
+  int32_t l_gram, l_tri, l_sec, l_v, l_w, pad1;
+};
+// device planning status codes (GroupCounts::status, DpSizes::status, and
+// the program's sticky status)
+enum DpStatus {
+  DP_OK = 0,
+  DP_OVER_CAPACITY = 1,  // a group needed more than its arenas or lists hold
+  DP_BAD_CHARGES = 2,    // charges out of range or not sorted
+  DP_TILE_TOO_BIG = 3,   // a contraction tile does not fit the LDS classes
+  DP_SECTOR_TOO_BIG = 4, // a tridiagonal-route sector above the eigensolver's limit
+  DP_NO_TABLE = 5,       // replayed more often than BS matrices were provided
+  DP_SKIPPED = 6         // an earlier replay failed: this one did nothing
+};
+// capacities of one group's arenas and work lists
+struct DpCaps {
+  int64_t var_bytes, work_bytes;
+  int32_t tiles, gram, jac, tri, sec, v, w, wyt, wya, pad;
+};
+// what a group needed in its last planning (written even when over capacity)
+struct DpNeeds {
+  int64_t var_bytes, work_bytes;
+  int32_t tiles, gram, jac, tri, sec, v, w, wyt, wya, status;
+};
+struct DpGroup {
+  char *staging;  // GroupCounts, UpdDesc[n_upd], the work lists (at the capacity header's offsets)
+  char *fixed;    // fixed plan parts, fixed_bytes per update
+  char *var;      // variable plan parts
+  char *work;     // scratch arena (shared by the groups launched on one stream)
+  int2 *tri_tmp;  // the tridiagonal list before its sort (caps.tri entries)
+  DpCaps caps;
+  DpNeeds needs;
+  int64_t o_desc, o_tiles, o_gram, o_jac, o_tri, o_sec, o_v, o_w, o_wyt, o_wya;
+  int32_t first, n_upd;  // updates [first, first + n_upd) of the program
+};
+struct DpParams {
+  int32_t d, chi, thr, tri_split, n_upd, n_groups, n_layers, pad;
+  double tie_rel, drop_rel;
+  int64_t fixed_bytes;
+  int32_t *state;  // [0] sticky status, [1] replay counter, [2] first failed replay, [3] spare
+  const double *B; // BS matrices, [n_layers][n_upd][2][2] complex128
+};
+// fixed per-update plan part: byte offsets (multiples of 256)
+struct FixedLayout {
+  int64_t o_sectors, o_lblk, o_cblk, o_rblk, o_tau_sector, o_cblk_of, o_rowoff, o_coloff;
+  int64_t o_row_info, o_col_info, o_charges, o_secx, bytes;
+};
+__host__ __device__ inline int64_t dp_align(int64_t x) { return (x + 255) / 256 * 256; }
+__host__ __device__ inline FixedLayout fixed_layout(int d, int chi) {
+  const int64_t nb = (int64_t)d * d, D1 = d + 1;  // blocks per bond, sectors: at most d^2
+  FixedLayout f;
+  int64_t o = 0;
+  f.o_sectors = o; o = dp_align(o + (int64_t)sizeof(Sector) * nb);
+  f.o_lblk = o; o = dp_align(o + (int64_t)sizeof(Block) * nb);
+  f.o_cblk = o; o = dp_align(o + (int64_t)sizeof(Block) * nb);
+  f.o_rblk = o; o = dp_align(o + (int64_t)sizeof(Block) * nb);
+  f.o_tau_sector = o; o = dp_align(o + 4 * D1 * D1);
+  f.o_cblk_of = o; o = dp_align(o + 4 * D1 * D1);
+  f.o_rowoff = o; o = dp_align(o + 4 * nb * nb);
+  f.o_coloff = o; o = dp_align(o + 4 * nb * nb);
+  f.o_row_info = o; o = dp_align(o + (int64_t)sizeof(SlotInfo) * chi);
+  f.o_col_info = o; o = dp_align(o + (int64_t)sizeof(SlotInfo) * chi);
+  f.o_charges = o; o = dp_align(o + 4 * 6 * (int64_t)chi);
+  f.o_secx = o; o = dp_align(o + 16 * nb);  // per sector: gram, W, V item offsets, class rank
+  f.bytes = o;
+  return f;
+}
+// variable per-update plan part: byte offsets of its arrays, and its size
+struct VarLayout {
+  int64_t o_rows, o_cols, o_gram, o_w, o_v, o_sv, o_titems, bytes;
+};
+__host__ __device__ inline VarLayout var_layout(int64_t rows, int64_t cols, int64_t gram,
+                                                int64_t wi, int64_t vi, int64_t sv,
+                                                int64_t titems) {
+  VarLayout v;
+  int64_t o = 0;
+  v.o_rows = o; o = dp_align(o + 4 * rows);
+  v.o_cols = o; o = dp_align(o + 4 * cols);
+  v.o_gram = o; o = dp_align(o + (int64_t)sizeof(Item) * gram);
+  v.o_w = o; o = dp_align(o + (int64_t)sizeof(Item) * wi);
+  v.o_v = o; o = dp_align(o + (int64_t)sizeof(Item) * vi);
+  v.o_sv = o; o = dp_align(o + 4 * sv);
+  v.o_titems = o; o = dp_align(o + (int64_t)sizeof(TileItem) * titems);
+  v.bytes = o;
+  return v;
+}
+// per-update scratch: the same arrays and sizes as a host plan's workspace
+// (plan.cpp), offsets from the update's scratch base
+struct WorkLayout {
+  int64_t C, T, G, V, W, S, lam, flag, slot, td, te, tau, kcount, triwork, hist, bytes;
+};
+__host__ __device__ inline WorkLayout work_layout(int64_t c_elems, int64_t g_elems, int64_t v_elems,
+                                                  int64_t w_elems, int64_t sv, int64_t n_sec,
+                                                  int64_t tw) {
+  WorkLayout L;
+  int64_t w = 0;
+#define BSMPO_WPLACE(f, b) L.f = w; w = dp_align(w + ((b) > 1 ? (b) : 1));
+  BSMPO_WPLACE(C, 16 * c_elems)
+  BSMPO_WPLACE(T, 16 * c_elems)
+  BSMPO_WPLACE(G, 16 * g_elems)
+  BSMPO_WPLACE(V, 16 * v_elems)
+  BSMPO_WPLACE(W, 16 * w_elems)
+  BSMPO_WPLACE(S, 8 * sv)
+  BSMPO_WPLACE(lam, 8 * sv)
+  BSMPO_WPLACE(flag, 4 * sv)
+  BSMPO_WPLACE(slot, 4 * sv)
+  BSMPO_WPLACE(td, 8 * sv)
+  BSMPO_WPLACE(te, 8 * sv)
+  BSMPO_WPLACE(tau, 16 * sv)
+  BSMPO_WPLACE(kcount, 4 * n_sec)
+  BSMPO_WPLACE(triwork, 8 * tw)
+  BSMPO_WPLACE(hist, 4 * kHistBins + 8)
+#undef BSMPO_WPLACE
+  L.bytes = w;
+  return L;
+}
+// tridiagonal-route work area of a sector of size n (one-stage reduction):
+// the inverse iteration's per-lane arrays, the Householder panels and the
+// back-transformation's T blocks share it
+__host__ __device__ inline int64_t tri_work_doubles(int n) {
+  const int nblk = (n - 1 + kWyBlock - 1) / kWyBlock;
+  const int64_t wy = 2 * (int64_t)kWyBlock * kWyBlock * nblk;
+  const int lanes = n < kTriVecThreads ? n : kTriVecThreads;
+  int64_t a = (int64_t)n * (6 * lanes > 4 * kTriPanel ? 6 * lanes : 4 * kTriPanel);
+  return a > wy ? a : wy;
+}
+// launchers of the planning kernels (devplan.hip)
+int launch_device_plan(const DpParams &p, const DpUpdate *d_upd, DpGroup *d_groups,
+                       DpSizes *d_sizes, hipStream_t stream);
+int launch_bs_table_layer(const DpParams &p, const DpUpdate *d_upd, hipStream_t stream);
 inline const GroupHeader *group_header(const void *blob) {
   return reinterpret_cast<const GroupHeader *>(blob);
 }

@@ -137,8 +137,7 @@ __device__ __forceinline__ double wave_sum32(double (&x)[32], int lane) {
 
 // ------------------------------------------------------------- stage 1 --
 template <int RPT>
-__global__ __launch_bounds__(kBandThreads) void k_band(const UpdDesc *__restrict__ Ud,
-                                                       const int2 *__restrict__ list) {
+__global__ __launch_bounds__(kBandThreads) void k_band(const UpdDesc *__restrict__ Ud, WorkSeg ws) {
   constexpr int NT = kBandThreads, NW = kBandWaves;
   __shared__ double red[NW][32];
   __shared__ double fin[32];
@@ -148,7 +147,9 @@ __global__ __launch_bounds__(kBandThreads) void k_band(const UpdDesc *__restrict
   __shared__ double2 Ms[kBand][kBand + 1];
   __shared__ double2 Bw[2 * kBand][kBand + 1];  // [T; -(T^H Z T) / 2]
   __shared__ double mxs[NW];
-  const int2 ref = list[blockIdx.x];
+  const int2 sg = *ws.seg;
+  if ((int)blockIdx.x >= sg.y) return;
+  const int2 ref = ws.list[sg.x + blockIdx.x];
   const UpdDesc &u = Ud[ref.x];
   const Sector s = u.sectors[ref.y];
   const int n = s.n;
@@ -499,11 +500,12 @@ __global__ __launch_bounds__(kBandThreads) void k_band(const UpdDesc *__restrict
 // Element mapping of a 16 x 16 block in a wave: lane l holds row l & 15,
 // columns (l >> 4) + 4 i, i = 0..3.  Row sums: over i, then lanes xor 16, 32;
 // column sums: lanes xor 1, 2, 4, 8.
-__global__ __launch_bounds__(64 * kBulgeWaves) void k_bulge(const UpdDesc *__restrict__ Ud,
-                                                            const int2 *__restrict__ list) {
+__global__ __launch_bounds__(64 * kBulgeWaves) void k_bulge(const UpdDesc *__restrict__ Ud, WorkSeg ws) {
   constexpr int NWv = kBulgeWaves;
   __shared__ int prog[2][NWv][2];  // per wave: current sweep, steps done in it
-  const int2 ref = list[blockIdx.x];
+  const int2 sg = *ws.seg;
+  if ((int)blockIdx.x >= sg.y) return;
+  const int2 ref = ws.list[sg.x + blockIdx.x];
   const UpdDesc &u = Ud[ref.x];
   const Sector s = u.sectors[ref.y];
   const int n = s.n;
@@ -662,10 +664,11 @@ __global__ __launch_bounds__(64 * kBulgeWaves) void k_bulge(const UpdDesc *__res
 }
 
 // ------------------------------------------------- Q2 back-transformation --
-__global__ __launch_bounds__(kBcThreads) void k_bulge_apply(const UpdDesc *__restrict__ Ud,
-                                                            const int2 *__restrict__ list,
-                                                            int nlist) {
+__global__ __launch_bounds__(kBcThreads) void k_bulge_apply(const UpdDesc *__restrict__ Ud, WorkSeg ws) {
   extern __shared__ double2 bsm[];
+  const int2 sg = *ws.seg;
+  const int2 *__restrict__ list = ws.list + sg.x;
+  const int nlist = sg.y;
   for (int it = blockIdx.x; it < nlist; it += gridDim.x) {
     const int2 ref = list[it];
     const int sector = ref.y & 0xfffff, chunk = ref.y >> 20;
@@ -732,13 +735,16 @@ __global__ __launch_bounds__(kBcThreads) void k_bulge_apply(const UpdDesc *__res
 static inline const UpdDesc *ts_desc(const GroupHeader &g, const char *w) {
   return reinterpret_cast<const UpdDesc *>(w + g.o_desc);
 }
+static inline const int2 *ts_segs(const GroupHeader &g, const char *w) {
+  return reinterpret_cast<const GroupCounts *>(w + g.o_counts)->seg;
+}
 
 // Stage 1 and 2 over the two-stage head of the tridiagonal list (largest
 // first: the sectors with two panel rows per thread lead).
 int launch_band(const GroupHeader &g, const char *w, hipStream_t stream) {
   if (g.n_ts == 0) return 0;
   if (g.ts_max_n > kBandMaxN) return fail(-1, "two-stage sector too large");
-  const int2 *tl = reinterpret_cast<const int2 *>(w + g.o_tri);
+  const WorkSeg tl{reinterpret_cast<const int2 *>(w + g.o_tri), ts_segs(g, w) + SG_TS};
   if (g.ts_nbig > 0) return fail(-1, "two-stage sector with more panel rows than threads");
   hipLaunchKernelGGL(k_band<1>, dim3(g.n_ts), dim3(kBandThreads), 0, stream, ts_desc(g, w), tl);
   BSMPO_HIP(hipGetLastError());
@@ -753,7 +759,8 @@ int launch_bulge_apply(const GroupHeader &g, const char *w, hipStream_t stream) 
   if (int rc = ensure_lds_limit((const void *)k_bulge_apply, (int)sizeof(double2) * (2048 + 2 * kBandMaxN)))
     return rc;
   hipLaunchKernelGGL(k_bulge_apply, dim3(std::min(g.n_bc, 16384)), dim3(kBcThreads), (size_t)lds,
-                     stream, ts_desc(g, w), reinterpret_cast<const int2 *>(w + g.o_bc), g.n_bc);
+                     stream, ts_desc(g, w),
+                     WorkSeg{reinterpret_cast<const int2 *>(w + g.o_bc), ts_segs(g, w) + SG_BC});
   BSMPO_HIP(hipGetLastError());
   return 0;
 }

@@ -136,8 +136,10 @@ __device__ __forceinline__ void block_product(BlockLds &sm, int K, bool skip, FP
 // G (n x n, column-major) = T T^H if L <= R else T^H T, T row-major L x R.
 // Tridiagonal-route sectors (n above the eigensolver threshold) need only the
 // lower triangle: the strictly upper tile of a diagonal block is skipped.
-__global__ __launch_bounds__(256) void k_gram(const UpdDesc *__restrict__ Ud,
-                                              const int2 *__restrict__ list, int nlist) {
+__global__ __launch_bounds__(256) void k_gram(const UpdDesc *__restrict__ Ud, WorkSeg ws) {
+  const int2 sg = *ws.seg;
+  const int2 *__restrict__ list = ws.list + sg.x;
+  const int nlist = sg.y;
   __shared__ BlockLds sm;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i16 = lane & 15, kq = lane >> 4;
@@ -202,10 +204,11 @@ __device__ __forceinline__ void rr_pair(int step, int i, int n2, int &p, int &q)
   if (p > q) { const int t = p; p = q; q = t; }
 }
 
-__global__ __launch_bounds__(256) void k_jacobi(const UpdDesc *__restrict__ Ud,
-                                                const int2 *__restrict__ list) {
+__global__ __launch_bounds__(256) void k_jacobi(const UpdDesc *__restrict__ Ud, WorkSeg ws) {
   extern __shared__ double2 smem[];
-  const int2 ref = list[blockIdx.x];
+  const int2 sg = *ws.seg;
+  if ((int)blockIdx.x >= sg.y) return;  // a launch sized for the group's capacity
+  const int2 ref = ws.list[sg.x + blockIdx.x];
   const UpdDesc &u = Ud[ref.x];
   const Sector s = u.sectors[ref.y];
   const int n = s.n;
@@ -323,8 +326,10 @@ __global__ __launch_bounds__(256) void k_jacobi(const UpdDesc *__restrict__ Ud,
 
 // V (L x n column-major) = T Y diag(1 / sigma) for sectors with L > R (Y
 // the Gram eigenvectors, n x n column-major in W's storage), kept columns only.
-__global__ __launch_bounds__(256) void k_vectors(const UpdDesc *__restrict__ Ud,
-                                                 const int2 *__restrict__ list, int nlist) {
+__global__ __launch_bounds__(256) void k_vectors(const UpdDesc *__restrict__ Ud, WorkSeg ws) {
+  const int2 sg = *ws.seg;
+  const int2 *__restrict__ list = ws.list + sg.x;
+  const int nlist = sg.y;
   __shared__ BlockLds sm;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i16 = lane & 15, kq = lane >> 4;
@@ -361,8 +366,10 @@ __global__ __launch_bounds__(256) void k_vectors(const UpdDesc *__restrict__ Ud,
 }
 
 // W (n x R row-major) = V^H C, V L x n column-major, C L x R row-major.
-__global__ __launch_bounds__(256) void k_wmat(const UpdDesc *__restrict__ Ud,
-                                              const int2 *__restrict__ list, int nlist) {
+__global__ __launch_bounds__(256) void k_wmat(const UpdDesc *__restrict__ Ud, WorkSeg ws) {
+  const int2 sg = *ws.seg;
+  const int2 *__restrict__ list = ws.list + sg.x;
+  const int nlist = sg.y;
   __shared__ BlockLds sm;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i16 = lane & 15, kq = lane >> 4;
@@ -429,9 +436,11 @@ __device__ int block_scan_excl(int v, int *sc, int &total) {
 // sigma) order of the stable charge sort.  Values within tie_rel (relative)
 // of the chi-th largest t are ties, broken by the lower concatenated index
 // (degenerate twins of Hermitian-conjugate sectors).
-__global__ __launch_bounds__(256) void k_topk(const UpdDesc *__restrict__ Ud) {
+__global__ __launch_bounds__(256) void k_topk(const UpdDesc *__restrict__ Ud,
+                                              const int2 *__restrict__ useg) {
   __shared__ int scratch[16];
   __shared__ int sc[256];
+  if ((int)blockIdx.x >= useg->y) return;  // a skipped group
   const UpdDesc &u = Ud[blockIdx.x];
   const int64_t nt = u.sv_total;
   const double *S = u.S;
@@ -497,7 +506,8 @@ __device__ __forceinline__ void out_region(const UpdDesc &u, int &w0, int &w1, i
 }
 
 // grid (x, n_updates): zero the stored part of Gamma0Out / Gamma1Out
-__global__ void k_zero(const UpdDesc *__restrict__ Ud) {
+__global__ void k_zero(const UpdDesc *__restrict__ Ud, const int2 *__restrict__ useg) {
+  if ((int)blockIdx.y >= useg->y) return;  // a skipped group
   const UpdDesc &u = Ud[blockIdx.y];
   int w0, w1, c1;
   out_region(u, w0, w1, c1);
@@ -519,7 +529,9 @@ __global__ void k_zero(const UpdDesc *__restrict__ Ud) {
 }
 
 // grid (x, n_updates): one block per kept singular-value slot: V column, W row
-__global__ __launch_bounds__(256) void k_scatter(const UpdDesc *__restrict__ Ud) {
+__global__ __launch_bounds__(256) void k_scatter(const UpdDesc *__restrict__ Ud,
+                                                 const int2 *__restrict__ useg) {
+  if ((int)blockIdx.y >= useg->y) return;  // a skipped group
   const UpdDesc &u = Ud[blockIdx.y];
   int w0, w1, c1;
   out_region(u, w0, w1, c1);
@@ -552,15 +564,21 @@ int launch_tri_vectors(const GroupHeader &g, const char *d_work, hipStream_t str
 static inline const UpdDesc *gdesc(const GroupHeader &g, const char *w) {
   return reinterpret_cast<const UpdDesc *>(w + g.o_desc);
 }
-static inline const int2 *glist(const char *w, int64_t off) {
-  return reinterpret_cast<const int2 *>(w + off);
+static inline const int2 *gseg(const GroupHeader &g, const char *w, int seg) {
+  return reinterpret_cast<const GroupCounts *>(w + g.o_counts)->seg + seg;
+}
+static inline WorkSeg wseg(const GroupHeader &g, const char *w, int64_t o_list, int seg) {
+  return WorkSeg{reinterpret_cast<const int2 *>(w + o_list), gseg(g, w, seg)};
 }
 
+// The header's counts size the grids (exact for a host-built group, the
+// capacity for one planned on the device); the kernels read the counts of
+// their work from the group's GroupCounts.
 int launch_svd(const GroupHeader &g, const char *w, hipStream_t stream) {
   if (g.n_sec == 0) return 0;
   int tk = timing_begin(ST_GRAM, stream);
   hipLaunchKernelGGL(k_gram, dim3(std::max(1, std::min(g.n_gram, 16384))), dim3(256), 0, stream, gdesc(g, w),
-                     glist(w, g.o_gram), g.n_gram);
+                     wseg(g, w, g.o_gram, SG_GRAM));
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, g.flops_gram, 16.0 * (g.c_elems + g.g_elems));
   if (int rc = ensure_lds_limit((const void *)k_jacobi, 96 * 96 * 16)) return rc;
@@ -570,7 +588,7 @@ int launch_svd(const GroupHeader &g, const char *w, hipStream_t stream) {
     if (nsec == 0) continue;
     const size_t lds = (size_t)g.jac_max_n[c] * g.jac_max_n[c] * 16;
     hipLaunchKernelGGL(k_jacobi, dim3(nsec), dim3(256), lds, stream, gdesc(g, w),
-                       glist(w, g.o_jac) + g.jac_off[c]);
+                       wseg(g, w, g.o_jac, SG_JAC + c));
     BSMPO_HIP(hipGetLastError());
   }
   timing_end(tk, stream, g.flops_jacobi, 0.0);
@@ -579,7 +597,7 @@ int launch_svd(const GroupHeader &g, const char *w, hipStream_t stream) {
 
 int launch_truncate(const GroupHeader &g, const char *w, hipStream_t stream) {
   int tk = timing_begin(ST_TOPK, stream);
-  hipLaunchKernelGGL(k_topk, dim3(g.n_upd), dim3(256), 0, stream, gdesc(g, w));
+  hipLaunchKernelGGL(k_topk, dim3(g.n_upd), dim3(256), 0, stream, gdesc(g, w), gseg(g, w, SG_UPD));
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, 0.0, 12.0 * g.sv_total);
   int rc = launch_kcount(g, w, stream);
@@ -588,23 +606,23 @@ int launch_truncate(const GroupHeader &g, const char *w, hipStream_t stream) {
   if (g.n_v) {
     tk = timing_begin(ST_VECTORS, stream);
     hipLaunchKernelGGL(k_vectors, dim3(std::min(g.n_v, 16384)), dim3(256), 0, stream, gdesc(g, w),
-                       glist(w, g.o_v), g.n_v);
+                       wseg(g, w, g.o_v, SG_V));
     BSMPO_HIP(hipGetLastError());
     timing_end(tk, stream, 0.0, 0.0);  // flops: kept pairs, counted by k_kcount
   }
   if (g.n_w) {
     tk = timing_begin(ST_WMAT, stream);
     hipLaunchKernelGGL(k_wmat, dim3(std::min(g.n_w, 16384)), dim3(256), 0, stream, gdesc(g, w),
-                       glist(w, g.o_w), g.n_w);
+                       wseg(g, w, g.o_w, SG_W));
     BSMPO_HIP(hipGetLastError());
     timing_end(tk, stream, 0.0, 16.0 * (g.v_elems + g.c_elems + g.w_elems));  // flops: k_kcount
   }
   tk = timing_begin(ST_SCATTER, stream);
-  hipLaunchKernelGGL(k_zero, dim3(256, g.n_upd), dim3(256), 0, stream, gdesc(g, w));
+  hipLaunchKernelGGL(k_zero, dim3(256, g.n_upd), dim3(256), 0, stream, gdesc(g, w), gseg(g, w, SG_UPD));
   BSMPO_HIP(hipGetLastError());
   if (g.max_sv > 0) {
     hipLaunchKernelGGL(k_scatter, dim3((unsigned)std::min<int64_t>(g.max_sv, 4096), g.n_upd),
-                       dim3(256), 0, stream, gdesc(g, w));
+                       dim3(256), 0, stream, gdesc(g, w), gseg(g, w, SG_UPD));
     BSMPO_HIP(hipGetLastError());
   }
   timing_end(tk, stream, 0.0, 16.0 * 2.0 * g.n_upd * (double)g.chi * g.chi);
