@@ -78,6 +78,12 @@ def parse():
     ap.add_argument("--two-stage", type=int, default=-1,
                     help="two-stage reduction threshold (bsmpo_set_two_stage; 0 = off, "
                          "-1 = the library default)")
+    ap.add_argument("--graph", type=int, default=1,
+                    help="1: also time the grouped path as device-planned layer programs (one "
+                         "captured HIP graph per layer); `value` is then that run's rate")
+    ap.add_argument("--graph-groups", type=int, default=8, help="groups per layer of the graph run")
+    ap.add_argument("--graph-headroom", type=float, default=1.3,
+                    help="capacities of the graph run's groups over what layer --start needs")
     ap.add_argument("--verbose", action="store_true")
     return ap.parse_args()
 
@@ -273,6 +279,7 @@ def main():
                 log(f"[bench] grouped untimed layer {layer}: {time.time() - t1:.1f}s "
                     f"groups {runner.last_info.get('groups')}")
         log(f"[bench] grouped untimed {T0} layers x {B} instances {time.time() - t1:.1f}s")
+        graph_states = [s.clone() for s in states] if args.graph else None
         bond0 = kept_bond_dimension(states[0])
         _capi.timing_enable(True)
         _capi.timing_read(reset=True)
@@ -299,6 +306,37 @@ def main():
         if args.streams > 1:
             roof["note"] = (f"groups go round {args.streams} launching streams: a launch's duration "
                             "includes sharing the GPU with the other streams' stages")
+        if args.graph:
+            # the same layers from the same state as layer programs: planned on
+            # the device, one captured HIP graph replayed per layer
+            final_eager = [s.observables() for s in states]
+            del states, runner
+            torch.cuda.empty_cache()
+            rules = engine.set_precision_rules(prec)
+            t1 = time.time()
+            lg = engine.LayerGraphRunner(graph_states, meshes, prec, n_groups=args.graph_groups,
+                                         n_streams=2, headroom=args.graph_headroom)
+            lg._set_caps(lg.needs(T0 % 2))
+            lg._create()
+            torch.cuda.synchronize()
+            log(f"[bench] layer program built in {time.time() - t1:.2f}s "
+                f"(groups {args.graph_groups}, work arena {lg.caps.work_bytes / 2**30:.1f} GiB)")
+            ts = time.time()
+            ng = lg.run(range(T0, T0 + K))
+            torch.cuda.synchronize()
+            elg = time.time() - ts
+            dm = max(float(np.abs(a[1] - b.observables()[1]).max())
+                     for a, b in zip(final_eager, graph_states))
+            result["graph"] = {"updates": ng, "seconds": elg, "rate": ng / elg,
+                               "fallbacks": lg.fallbacks, "groups": args.graph_groups,
+                               "marginals_vs_host_planned": dm,
+                               "bond": [bond0, kept_bond_dimension(graph_states[0])]}
+            engine.set_precision_rules(engine.C128)
+            log(f"[bench] graph {args.precision}: {ng} updates in {elg:.3f}s = {ng / elg:.2f} "
+                f"updates/s (fallbacks {lg.fallbacks}; marginals vs host-planned {dm:.2e})")
+            lg.destroy()
+            del lg, graph_states
+            torch.cuda.empty_cache()
 
     cpu = cpu_baseline(snap, d, chi, meshes[0], inputs, T0, args.cpu_seconds) if snap else None
     workload = (f"lossy boson-sampling MPO site updates, M={M} N={N} "
@@ -313,7 +351,8 @@ def main():
                                                 thr.get("eig_threshold", 96), alg)
         if src:
             roof["traffic_source"] = src
-    main_path = g or p
+    gr = result.get("graph")
+    main_path = gr or g or p
     out = {
         "metric": METRIC,
         "value": main_path["rate"],
@@ -341,7 +380,14 @@ def main():
                         if g else None),
             "plain_fp64": ({"instances": 1, "updates": p["updates"],
                             "updates_per_s": p["rate"], "autotune": p.get("tuned")} if p else None),
-            "grouped_over_plain": (g["rate"] / p["rate"]) if (g and p) else None,
+            "graph": ({"updates": gr["updates"], "updates_per_s": gr["rate"],
+                       "groups_per_layer": gr["groups"], "fallbacks": gr["fallbacks"],
+                       "marginals_vs_host_planned": gr["marginals_vs_host_planned"],
+                       "note": "value: the grouped path as layer programs (device planning, "
+                               "one HIP graph replayed per layer); `grouped` is the same "
+                               "work host-planned group by group, whose stage events give "
+                               "the roofline"} if gr else None),
+            "grouped_over_plain": (main_path["rate"] / p["rate"]) if (g and p) else None,
             "precision_check": precision_check(g, p, args.precision, T0 + K),
         },
         "roofline": roof,

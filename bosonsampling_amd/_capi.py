@@ -54,6 +54,13 @@ SIGNATURES = {
     "bsmpo_group_graph_create": (C.c_int, [_vp, _i, _vp, _sz, _vp, C.POINTER(C.c_void_p)]),
     "bsmpo_graph_launch": (C.c_int, [_vp, _vp]),
     "bsmpo_graph_destroy": (C.c_int, [_vp]),
+    "bsmpo_layer_query": (C.c_int, [_vp, C.POINTER(C.c_size_t)]),
+    "bsmpo_layer_create": (C.c_int, [_vp, _vp, _sz, _vp, C.POINTER(C.c_void_p)]),
+    "bsmpo_layer_seek": (C.c_int, [_vp, _i, _vp]),
+    "bsmpo_layer_launch": (C.c_int, [_vp, _vp]),
+    "bsmpo_layer_state_read": (C.c_int, [_vp, _vp, _vp]),
+    "bsmpo_layer_destroy": (C.c_int, [_vp]),
+    "bsmpo_layer_needs": (C.c_int, [_i, _vp, _vp]),
     "bsmpo_timing_enable": (C.c_int, [_i]),
     "bsmpo_timing_read": (C.c_int, [_dp, _dp, _dp, _i64p, _i]),
 }
@@ -87,6 +94,40 @@ class UpdateIO(C.Structure):
     _fields_ = [("Glc", C.c_void_p), ("Gcr", C.c_void_p), ("LR", C.c_void_p), ("U3", C.c_void_p),
                 ("charge_c", C.c_void_p), ("lambda_c", C.c_void_p), ("num_lambda", C.c_void_p),
                 ("edge", C.c_int32), ("reserved", C.c_int32)]
+
+
+class LayerUpdate(C.Structure):
+    _fields_ = [("CL", C.c_void_p), ("CC", C.c_void_p), ("CR", C.c_void_p), ("Glc", C.c_void_p),
+                ("Gcr", C.c_void_p), ("LR", C.c_void_p), ("charge_c", C.c_void_p),
+                ("lambda_c", C.c_void_p), ("num_lambda", C.c_void_p), ("edge", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class LayerCaps(C.Structure):
+    _fields_ = [("var_bytes", C.c_int64), ("work_bytes", C.c_int64), ("tiles", C.c_int32),
+                ("gram", C.c_int32), ("jac", C.c_int32), ("tri", C.c_int32), ("sec", C.c_int32),
+                ("v", C.c_int32), ("w", C.c_int32), ("wyt", C.c_int32), ("wya", C.c_int32),
+                ("status", C.c_int32)]
+    INT_FIELDS = ("tiles", "gram", "jac", "tri", "sec", "v", "w", "wyt", "wya")
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
+class LayerSpec(C.Structure):
+    _fields_ = [("d", C.c_int), ("chi", C.c_int), ("precision", C.c_int), ("n_sets", C.c_int),
+                ("n_upd", C.c_int * 2), ("upd", C.c_void_p * 2), ("n_groups", C.c_int),
+                ("n_streams", C.c_int), ("n_layers", C.c_int), ("d_B", C.c_void_p),
+                ("caps", LayerCaps)]
+
+
+class LayerState(C.Structure):
+    _fields_ = [("status", C.c_int32), ("replays", C.c_int32), ("failed_at", C.c_int32),
+                ("n_groups", C.c_int32)]
+
+
+LAYER_STATUS = {0: "ok", 1: "over capacity", 2: "bad charges", 3: "tile too big",
+                4: "sector too big", 5: "no BS matrices", 6: "skipped"}
 
 
 class BsmpoError(RuntimeError):

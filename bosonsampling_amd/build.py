@@ -15,8 +15,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbsmpo.so")
-SOURCES = ["plan.cpp", "group.cpp", "capi.cpp", "adjoint.cpp", "update.hip", "contract_tile.hip",
-           "eig.hip", "backtransform.hip", "twostage.hip", "observables.hip", "adjoint.hip"]
+SOURCES = ["plan.cpp", "group.cpp", "capi.cpp", "adjoint.cpp", "layer.cpp", "update.hip",
+           "contract_tile.hip", "eig.hip", "backtransform.hip", "twostage.hip", "devplan.hip",
+           "observables.hip", "adjoint.hip"]
 ARCH = os.environ.get("BSMPO_ARCH", "gfx950")
 
 

@@ -76,7 +76,6 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
 // sector list, which is sorted largest first (n outside [nmin, nmax] is
 // skipped, a guard of the LDS sizing).
 constexpr int kTriNB = kTriPanel;
-constexpr int kTriMaxN = 2048;
 // Streaming shape of the reduction.  Measured on the M=64, chi=1024 grouped
 // workload (one launching stream): strips of 4 columns at 3 waves per SIMD
 // (168 VGPRs, one trailing tile per pass) 1.7% faster, 4 waves per SIMD
@@ -1106,59 +1105,38 @@ static int side_stream(hipStream_t parent, SideStream **out) {
   return 0;
 }
 
-int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
-  if (g.n_tri == 0) return 0;
+static int tri_limits(const GroupHeader &g) {
   if (g.tri_max_n > kTriMaxN) return fail(-1, "sector too large for the tridiagonal route (n > 2048)");
   if (int rc = ensure_lds_limit((const void *)k_tridiag<256, 2, kTriSW, kTriOcc>, 512 * 32)) return rc;
-  if (int rc = ensure_lds_limit((const void *)k_tridiag<512, 4, kTriSW, kTriOcc>, kTriMaxN * 32)) return rc;
-  int tk = timing_begin(ST_TRIDIAG, stream);
-  // The list holds the two-stage sectors first, then the one-stage ones, each
-  // part largest first.  The two-stage sectors (band reduction, then bulge
-  // chasing) run on the launching stream.  One-stage sectors: with few of them
-  // (a single update), the larger block shape for every sector, so that each
-  // column chain gets more threads; with many (a group across instances), the
-  // sectors above the split (bsmpo_set_tri_split, default 384) take the
-  // 512-thread shape and the rest 256-thread blocks, more per CU.  Work beside
-  // the launching stream's goes to a side stream forked before the first
-  // launch (fork / join by events), so that it fills the slots the long
-  // chains leave.
-  const int n1 = g.n_tri - g.n_ts;
-  const bool few = n1 < kTriFewSectors;
-  const int nbig = few ? n1 : g.tri_nbig;
-  const int split = few ? kTriMaxN : g.tri_split;
-  const bool big_on_side = few && g.n_ts > 0;
-  const bool main_work = g.n_ts > 0 || (nbig > 0 && !big_on_side);
-  const bool side_work = nbig < n1 || (big_on_side && nbig > 0);
-  const bool both = main_work && side_work;
-  SideStream *ss = nullptr;
-  if (both) {
-    int rc = side_stream(stream, &ss);
-    if (rc) return rc;
-    BSMPO_HIP(hipEventRecord(ss->fork, stream));
-    BSMPO_HIP(hipStreamWaitEvent(ss->stream, ss->fork, 0));
-  }
-  const hipStream_t sst = both ? ss->stream : stream;
-  if (int rc = launch_band(g, w, stream)) return rc;
-  if (nbig > 0) {
-    hipLaunchKernelGGL((k_tridiag<512, 4, kTriSW, kTriOcc>), dim3(nbig), dim3(512),
-                       (size_t)g.tri_max_n * 32, big_on_side ? sst : stream, gdesc(g, w),
-                       wseg(g, w, g.o_tri, SG_TRIBIG), 1, kTriMaxN);
-    BSMPO_HIP(hipGetLastError());
-  }
-  if (nbig < n1) {
-    hipLaunchKernelGGL((k_tridiag<256, 2, kTriSW, kTriOcc>), dim3(n1 - nbig), dim3(256),
-                       (size_t)std::min(split, g.tri_max_n) * 32, sst, gdesc(g, w),
-                       wseg(g, w, g.o_tri, SG_TRISMALL), 1, split);
-    BSMPO_HIP(hipGetLastError());
-  }
-  if (both) {
-    BSMPO_HIP(hipEventRecord(ss->join, ss->stream));
-    BSMPO_HIP(hipStreamWaitEvent(stream, ss->join, 0));
-  }
-  timing_end(tk, stream, g.flops_tridiag, g.bytes_tridiag);
-  tk = timing_begin(ST_BISECT, stream);
-  // the update's cutoff from sample counts (tridiagonal route) and exact
-  // values (Jacobi route), then the eigenvalues above half of it
+  return ensure_lds_limit((const void *)k_tridiag<512, 4, kTriSW, kTriOcc>, kTriMaxN * 32);
+}
+
+// One of the two Householder launches of a device-planned group (its
+// capacities for counts): part 0 the sectors above the block-shape split on
+// 512-thread blocks, part 1 the rest on 256-thread blocks, each with a grid
+// for the whole list; the kernels read the split (SG_TRIBIG, SG_TRISMALL)
+// from the device.  A layer program makes them two independent graph nodes.
+int launch_tridiag_part(const GroupHeader &g, const char *w, hipStream_t stream, int part) {
+  if (g.n_tri == 0) return 0;
+  if (!g.dev_planned || g.n_ts != 0) return fail(-1, "tridiag part: a device-planned one-stage group only");
+  if (int rc = tri_limits(g)) return rc;
+  if (part == 0)
+    hipLaunchKernelGGL((k_tridiag<512, 4, kTriSW, kTriOcc>), dim3(g.n_tri), dim3(512),
+                       (size_t)g.tri_max_n * 32, stream, gdesc(g, w), wseg(g, w, g.o_tri, SG_TRIBIG),
+                       1, kTriMaxN);
+  else
+    hipLaunchKernelGGL((k_tridiag<256, 2, kTriSW, kTriOcc>), dim3(g.n_tri), dim3(256),
+                       (size_t)std::min(g.tri_split, g.tri_max_n) * 32, stream, gdesc(g, w),
+                       wseg(g, w, g.o_tri, SG_TRISMALL), 1, g.tri_split);
+  BSMPO_HIP(hipGetLastError());
+  return 0;
+}
+
+// After the reductions: the update's cutoff from sample counts (tridiagonal
+// route) and exact values (Jacobi route), then the eigenvalues above half of it.
+int launch_tri_sturm(const GroupHeader &g, const char *w, hipStream_t stream) {
+  if (g.n_tri == 0) return 0;
+  const int tk = timing_begin(ST_BISECT, stream);
   hipLaunchKernelGGL(k_hist_zero, dim3(g.n_upd), dim3(256), 0, stream, gdesc(g, w),
                      gseg(g, w, SG_UPD));
   BSMPO_HIP(hipGetLastError());
@@ -1179,6 +1157,59 @@ int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, 0.0, 0.0);
   return 0;
+}
+
+int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
+  if (g.n_tri == 0) return 0;
+  if (int rc = tri_limits(g)) return rc;
+  int tk = timing_begin(ST_TRIDIAG, stream);
+  // The list holds the two-stage sectors first, then the one-stage ones, each
+  // part largest first.  The two-stage sectors (band reduction, then bulge
+  // chasing) run on the launching stream.  One-stage sectors: with few of them
+  // (a single update), the larger block shape for every sector, so that each
+  // column chain gets more threads; with many (a group across instances), the
+  // sectors above the split (bsmpo_set_tri_split, default 384) take the
+  // 512-thread shape and the rest 256-thread blocks, more per CU.  Work beside
+  // the launching stream's goes to a side stream forked before the first
+  // launch (fork / join by events), so that it fills the slots the long
+  // chains leave.  A device-planned group launches both shapes with grids for
+  // its capacity (see launch_tridiag_part).
+  const int n1 = g.n_tri - g.n_ts;
+  const bool few = !g.dev_planned && n1 < kTriFewSectors;
+  const int nbig = few || g.dev_planned ? n1 : g.tri_nbig;  // grid of the 512-thread launch
+  const int nsmall = g.dev_planned ? n1 : n1 - nbig;         // grid of the 256-thread launch
+  const int split = few ? kTriMaxN : g.tri_split;
+  const bool big_on_side = few && g.n_ts > 0;
+  const bool main_work = g.n_ts > 0 || (nbig > 0 && !big_on_side);
+  const bool side_work = nsmall > 0 || (big_on_side && nbig > 0);
+  const bool both = main_work && side_work;
+  SideStream *ss = nullptr;
+  if (both) {
+    int rc = side_stream(stream, &ss);
+    if (rc) return rc;
+    BSMPO_HIP(hipEventRecord(ss->fork, stream));
+    BSMPO_HIP(hipStreamWaitEvent(ss->stream, ss->fork, 0));
+  }
+  const hipStream_t sst = both ? ss->stream : stream;
+  if (int rc = launch_band(g, w, stream)) return rc;
+  if (nbig > 0) {
+    hipLaunchKernelGGL((k_tridiag<512, 4, kTriSW, kTriOcc>), dim3(nbig), dim3(512),
+                       (size_t)g.tri_max_n * 32, big_on_side ? sst : stream, gdesc(g, w),
+                       wseg(g, w, g.o_tri, SG_TRIBIG), 1, kTriMaxN);
+    BSMPO_HIP(hipGetLastError());
+  }
+  if (nsmall > 0) {
+    hipLaunchKernelGGL((k_tridiag<256, 2, kTriSW, kTriOcc>), dim3(nsmall), dim3(256),
+                       (size_t)std::min(split, g.tri_max_n) * 32, sst, gdesc(g, w),
+                       wseg(g, w, g.o_tri, SG_TRISMALL), 1, split);
+    BSMPO_HIP(hipGetLastError());
+  }
+  if (both) {
+    BSMPO_HIP(hipEventRecord(ss->join, ss->stream));
+    BSMPO_HIP(hipStreamWaitEvent(stream, ss->join, 0));
+  }
+  timing_end(tk, stream, g.flops_tridiag, g.bytes_tridiag);
+  return launch_tri_sturm(g, w, stream);
 }
 
 int launch_kcount(const GroupHeader &g, const char *w, hipStream_t stream) {

@@ -279,6 +279,10 @@ int kept_flops_read(double *out, bool reset);
 int launch_bs_table(int d, int nb, const double *d_B, double *d_U3, hipStream_t stream);
 int launch_contract(const GroupHeader &g, const char *d_work, int precision, hipStream_t stream);
 int launch_svd(const GroupHeader &g, const char *d_work, hipStream_t stream);
+// parts of launch_svd, for graphs built from them (layer programs)
+int launch_svd_head(const GroupHeader &g, const char *d_work, hipStream_t stream);
+int launch_tridiag_part(const GroupHeader &g, const char *d_work, hipStream_t stream, int part);
+int launch_tri_sturm(const GroupHeader &g, const char *d_work, hipStream_t stream);
 int launch_truncate(const GroupHeader &g, const char *d_work, hipStream_t stream);
 int launch_wy_backtransform(const GroupHeader &g, const char *d_work, hipStream_t stream);
 int launch_band(const GroupHeader &g, const char *d_work, hipStream_t stream);
@@ -314,10 +318,11 @@ struct DpUpdate {
   int32_t *charge_c;
   double *lambda_c;
   int32_t *num_lambda;
-  int32_t edge, group;          // group: index of the update's group
+  int32_t edge, group;          // group: index of the update's group in its set
   int32_t local, pad;           // position inside the group
 };
-// sizes found by the first planning kernel, offsets by the group scan
+// sizes found by the first planning kernel, offsets by the group scan (one
+// record per update of the replay's set)
 struct DpSizes {
   int32_t status, n_sec, n_lblk, n_cblk, n_rblk, rows_total, cols_total, n_gram;
   int32_t n_w, n_v, sv_total, n_tri, n_titems, pad0;
@@ -358,15 +363,19 @@ struct DpGroup {
   DpCaps caps;
   DpNeeds needs;
   int64_t o_desc, o_tiles, o_gram, o_jac, o_tri, o_sec, o_v, o_w, o_wyt, o_wya;
-  int32_t first, n_upd;  // updates [first, first + n_upd) of the program
+  int32_t first[2], n_upd[2];  // per update set: its updates [first, first + n_upd)
 };
 struct DpParams {
-  int32_t d, chi, thr, tri_split, n_upd, n_groups, n_layers, pad;
+  int32_t d, chi, thr, tri_split, n_groups, n_layers;
+  int32_t n_sets, max_upd;     // replay r runs update set r % n_sets
+  int32_t n_upd[2], off[2];    // per set: updates, first record in the update array
   double tie_rel, drop_rel;
   int64_t fixed_bytes;
-  int32_t *state;  // [0] sticky status, [1] replay counter, [2] first failed replay, [3] spare
-  const double *B; // BS matrices, [n_layers][n_upd][2][2] complex128
+  int32_t *state;  // [0] sticky status, [1] replay counter, [2] first failed replay,
+                   // [3] failure seen by the BS-table kernel
+  const double *B; // BS matrices, [n_layers][max_upd][2][2] complex128
 };
+__device__ inline int dp_set(const DpParams &P) { return P.state[1] % P.n_sets; }
 // fixed per-update plan part: byte offsets (multiples of 256)
 struct FixedLayout {
   int64_t o_sectors, o_lblk, o_cblk, o_rblk, o_tau_sector, o_cblk_of, o_rowoff, o_coloff;
@@ -455,6 +464,9 @@ __host__ __device__ inline int64_t tri_work_doubles(int n) {
 int launch_device_plan(const DpParams &p, const DpUpdate *d_upd, DpGroup *d_groups,
                        DpSizes *d_sizes, hipStream_t stream);
 int launch_bs_table_layer(const DpParams &p, const DpUpdate *d_upd, hipStream_t stream);
+int launch_layer_seek(int32_t *d_state, int replay, hipStream_t stream);
+constexpr int kTriMaxN = 2048;   // largest sector of the tridiagonal route
+constexpr int kDpMaxChi = kTriMaxN;  // largest chi a layer program plans
 inline const GroupHeader *group_header(const void *blob) {
   return reinterpret_cast<const GroupHeader *>(blob);
 }

@@ -20,11 +20,8 @@ namespace bsmpo {
 // ============================================================ BS table ==
 // One block per beam splitter; U3[n1][n2][p] = <n1, n2|U(B)|p, q>, q = n1+n2-p:
 //   sqrt(n1! n2! / (p! q!)) sum_k C(p,k) C(q,n1-k) B00^k B10^(p-k) B01^(n1-k) B11^(q-n1+k)
-__global__ void k_bs_table(int d, const double *B, double *U3, BsArgs args) {
-  const int b = blockIdx.x;
-  const double *Bb = B + 8 * b;
+__device__ void bs_table_one(int d, const double *Bb, double *U, const BsArgs &args) {
   const cplx b00{Bb[0], Bb[1]}, b01{Bb[2], Bb[3]}, b10{Bb[4], Bb[5]}, b11{Bb[6], Bb[7]};
-  double *U = U3 + (int64_t)b * d * d * d * 2;
   for (int e = threadIdx.x; e < d * d * d; e += blockDim.x) {
     const int n1 = e / (d * d), n2 = (e / d) % d, p = e % d;
     const int q = n1 + n2 - p;
@@ -44,6 +41,33 @@ __global__ void k_bs_table(int d, const double *B, double *U3, BsArgs args) {
     }
     cstore(U, e, acc);
   }
+}
+
+__global__ void k_bs_table(int d, const double *B, double *U3, BsArgs args) {
+  const int b = blockIdx.x;
+  bs_table_one(d, B + 8 * b, U3 + (int64_t)b * d * d * d * 2, args);
+}
+
+// The tables of a layer program's replay r = state[1]: its matrices are
+// B[r][update]; each update's table goes to its own U3 buffer.  A replay past
+// the n_layers provided records DP_NO_TABLE in state[3] (the gate then skips
+// the replay).
+__global__ void k_bs_table_layer(DpParams P, const DpUpdate *__restrict__ U, BsArgs args) {
+  const int r = P.state[1];
+  if (r < 0 || r >= P.n_layers) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) P.state[3] = DP_NO_TABLE;
+    return;
+  }
+  const int set = r % P.n_sets, b = blockIdx.x;
+  if (b >= P.n_upd[set]) return;
+  bs_table_one(P.d, P.B + 8 * ((int64_t)r * P.max_upd + b), U[P.off[set] + b].U3, args);
+}
+
+int launch_bs_table_layer(const DpParams &p, const DpUpdate *d_upd, hipStream_t stream) {
+  if (p.d < 1 || p.d > 32 || p.max_upd < 1) return fail(-1, "bs_table_layer: bad d or n_upd");
+  hipLaunchKernelGGL(k_bs_table_layer, dim3(p.max_upd), dim3(256), 0, stream, p, d_upd, bs_args());
+  BSMPO_HIP(hipGetLastError());
+  return 0;
 }
 
 int launch_bs_table(int d, int nb, const double *d_B, double *d_U3, hipStream_t stream) {
@@ -558,6 +582,7 @@ __global__ __launch_bounds__(256) void k_scatter(const UpdDesc *__restrict__ Ud,
 
 // ============================================================ launchers ==
 int launch_tri_values(const GroupHeader &g, const char *d_work, hipStream_t stream);
+int launch_svd_head(const GroupHeader &g, const char *d_work, hipStream_t stream);
 int launch_kcount(const GroupHeader &g, const char *d_work, hipStream_t stream);
 int launch_tri_vectors(const GroupHeader &g, const char *d_work, hipStream_t stream);
 
@@ -576,6 +601,14 @@ static inline WorkSeg wseg(const GroupHeader &g, const char *w, int64_t o_list, 
 // their work from the group's GroupCounts.
 int launch_svd(const GroupHeader &g, const char *w, hipStream_t stream) {
   if (g.n_sec == 0) return 0;
+  if (int rc = launch_svd_head(g, w, stream)) return rc;
+  return launch_tri_values(g, w, stream);
+}
+
+// Gram matrices and the one-sided Jacobi route (the SVD stage up to the
+// tridiagonal route's reductions)
+int launch_svd_head(const GroupHeader &g, const char *w, hipStream_t stream) {
+  if (g.n_sec == 0) return 0;
   int tk = timing_begin(ST_GRAM, stream);
   hipLaunchKernelGGL(k_gram, dim3(std::max(1, std::min(g.n_gram, 16384))), dim3(256), 0, stream, gdesc(g, w),
                      wseg(g, w, g.o_gram, SG_GRAM));
@@ -592,7 +625,7 @@ int launch_svd(const GroupHeader &g, const char *w, hipStream_t stream) {
     BSMPO_HIP(hipGetLastError());
   }
   timing_end(tk, stream, g.flops_jacobi, 0.0);
-  return launch_tri_values(g, w, stream);
+  return 0;
 }
 
 int launch_truncate(const GroupHeader &g, const char *w, hipStream_t stream) {

@@ -353,6 +353,195 @@ class GroupRunner:
             st.charge_host[mode + 1] = ch
 
 
+class LayerGraphRunner:
+    """Brick-wall layers of several instances, each layer planned on the
+    device and run as ONE captured HIP graph (libbsmpo's bsmpo_layer_*).
+
+    One layer program holds the two update sets of a brick-wall mesh (every
+    even pair, every odd pair, across all instances) and is captured once;
+    replay r applies layer r: set r % 2, BS tables from layer r's matrices
+    (every layer's matrices are uploaded once), plans built on the device
+    from the charges in HBM, nothing read back between layers.
+    ``run(layers)`` takes consecutive full layers (every pair active, i.e.
+    past the light cone) and reads the program's state once, at its end.  A
+    replay whose plan does not fit the capacities changes nothing and stops
+    the later ones; ``run`` then re-creates the program with what the device
+    reported it needed and runs the rest again (``fallbacks`` counts that).
+
+    Capacities per group come from host plans of the first layer's updates
+    (bsmpo_layer_needs) times ``headroom``.  Each update set is split into
+    ``n_groups`` contiguous groups launched in turn on ``n_streams`` streams.
+    """
+
+    def __init__(self, states, meshes, precision: int = C128, n_groups: int = 8,
+                 n_streams: int = 2, headroom: float = 1.5):
+        self.states, self.meshes = list(states), list(meshes)
+        st0 = self.states[0]
+        self.M, self.d, self.chi, self.device = st0.M, st0.d, st0.chi, st0.device
+        self.precision = int(precision)
+        self.n_groups, self.n_streams, self.headroom = int(n_groups), int(n_streams), float(headroom)
+        self.prog = None
+        self.caps = None
+        self.work = None
+        self.stream = torch.cuda.Stream(device=self.device)
+        self.fallbacks = 0
+        self.updates = []
+        for p in (0, 1):
+            modes = list(range(p, self.M - 1, 2))
+            self.updates.append([(k, st, i, mode) for k, st in enumerate(self.states)
+                                 for i, mode in enumerate(modes)])
+        mu = max(len(u) for u in self.updates)
+        B = np.zeros((self.M, mu, 2, 2), np.complex128)
+        for layer in range(self.M):
+            for j, (k, _, i, _) in enumerate(self.updates[layer % 2]):
+                B[layer, j] = self.meshes[k][layer, i]
+        self.B = torch.from_numpy(B.view(np.float64).copy()).to(self.device)
+        self.records = [self._records(p) for p in (0, 1)]
+
+    # ----------------------------------------------------------- program
+    def _records(self, p):
+        recs = (_capi.LayerUpdate * len(self.updates[p]))()
+        for j, (_, st, _, mode) in enumerate(self.updates[p]):
+            edge = mode == st.M - 2
+            LR = st.edge if edge else st.Lambda[mode + 1]
+            recs[j] = _capi.LayerUpdate(st.charge[mode].data_ptr(), st.charge[mode + 1].data_ptr(),
+                                        st.charge[mode + 2].data_ptr(), st.Gamma[mode].data_ptr(),
+                                        st.Gamma[mode + 1].data_ptr(), LR.data_ptr(),
+                                        st.charge[mode + 1].data_ptr(), st.Lambda[mode].data_ptr(),
+                                        st.num_lambda[mode].data_ptr(), 1 if edge else 0, 0)
+        return recs
+
+    def _spec(self):
+        spec = _capi.LayerSpec()
+        spec.d, spec.chi, spec.precision = self.d, self.chi, self.precision
+        spec.n_sets = 2
+        for p in (0, 1):
+            spec.n_upd[p] = len(self.updates[p])
+            spec.upd[p] = C.cast(self.records[p], C.c_void_p)
+        spec.n_groups = min(self.n_groups, *(len(u) for u in self.updates))
+        spec.n_streams = self.n_streams
+        spec.n_layers = self.M
+        spec.d_B = self.B.data_ptr()
+        spec.caps = self.caps
+        return spec
+
+    def needs(self, p):
+        """What one group of update set p needs now (host plans of its
+        updates from the host charge mirror): the maximum over its groups."""
+        ups = self.updates[p]
+        ng = min(self.n_groups, *(len(u) for u in self.updates))
+        base, extra = divmod(len(ups), ng)
+        worst, first = {}, 0
+        for g in range(ng):
+            n = base + (1 if g < extra else 0)
+            plans = []
+            for _, st, _, mode in ups[first:first + n]:
+                ch = st.charge_host
+                plans.append(Plan(st.d, st.chi, ch[mode], ch[mode + 1], ch[mode + 2]))
+            arr = (C.c_void_p * n)(*[pl.address for pl in plans])
+            c = _capi.LayerCaps()
+            check(lib().bsmpo_layer_needs(n, arr, C.byref(c)), "layer_needs")
+            for k, v in c.as_dict().items():
+                worst[k] = max(worst.get(k, 0), v)
+            first += n
+        return worst
+
+    def _set_caps(self, worst):
+        c = _capi.LayerCaps()
+        h = self.headroom
+        c.var_bytes = int(worst["var_bytes"] * h) + (1 << 20)
+        c.work_bytes = int(worst["work_bytes"] * h) + (1 << 20)
+        for f in _capi.LayerCaps.INT_FIELDS:
+            setattr(c, f, int(worst[f] * h) + 64)
+        self.caps = c
+
+    def _create(self):
+        self.destroy()
+        spec = self._spec()
+        sz = C.c_size_t()
+        check(lib().bsmpo_layer_query(C.byref(spec), C.byref(sz)), "layer_query")
+        if self.work is None or self.work.numel() < sz.value:
+            self.work = None
+            self.work = torch.empty(sz.value, dtype=torch.uint8, device=self.device)
+        self.stream.wait_stream(torch.cuda.current_stream(self.device))
+        h = C.c_void_p()
+        check(lib().bsmpo_layer_create(C.byref(spec), C.c_void_p(self.work.data_ptr()),
+                                       self.work.numel(), C.c_void_p(self.stream.cuda_stream),
+                                       C.byref(h)), "layer_create")
+        self.prog = h
+
+    def destroy(self):
+        if self.prog is not None:
+            check(lib().bsmpo_layer_destroy(self.prog), "layer_destroy")
+            self.prog = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def state(self):
+        """(bsmpo_layer_state, [what each group needed]) after a device sync."""
+        st = _capi.LayerState()
+        needs = (_capi.LayerCaps * max(1, self._spec().n_groups))()
+        check(lib().bsmpo_layer_state_read(self.prog, C.byref(st), needs), "layer_state_read")
+        return st, [n.as_dict() for n in needs]
+
+    # ---------------------------------------------------------------- run
+    def _sync_host_charges(self):
+        for st in self.states:
+            st.charge_host[:] = st.charge.cpu().numpy()
+
+    def run(self, layers):
+        """Applies the given consecutive full layers; returns the number of
+        beam-splitter updates applied."""
+        layers = list(layers)
+        for a, b in zip(layers, layers[1:]):
+            if b != a + 1:
+                raise ValueError("layers must be consecutive")
+        if not layers:
+            return 0
+        if layers[0] < 0 or layers[-1] >= self.M:
+            raise ValueError("layer out of range")
+        if self.caps is None:
+            worst = self.needs(layers[0] % 2)
+            if len(layers) > 1:
+                w2 = self.needs(layers[1] % 2)
+                worst = {k: max(v, w2[k]) for k, v in worst.items()}
+            self._set_caps(worst)
+        if self.prog is None:
+            self._create()
+        main = torch.cuda.current_stream(self.device)
+        s = self.stream
+        s.wait_stream(main)
+        sp = C.c_void_p(s.cuda_stream)
+        check(lib().bsmpo_layer_seek(self.prog, layers[0], sp), "layer_seek")
+        for _ in layers:
+            check(lib().bsmpo_layer_launch(self.prog, sp), "layer_launch")
+        main.wait_stream(s)
+        st, needs = self.state()  # the one synchronising read
+        self._sync_host_charges()
+        if st.status == 0:
+            return sum(len(self.updates[x % 2]) for x in layers)
+        if st.status not in (1, 6):
+            raise _capi.BsmpoError(f"layer program: {_capi.LAYER_STATUS.get(st.status, st.status)} "
+                                   f"at layer {st.failed_at}")
+        # the failed layer and every later one changed nothing: grow the
+        # capacities to what the device found it needed, and run them again
+        failed = st.failed_at
+        self.fallbacks += 1
+        grown = {}
+        for f, _ in _capi.LayerCaps._fields_:
+            if f != "status":
+                need = max(nd[f] for nd in needs)
+                grown[f] = max(need, int(getattr(self.caps, f) * 1.25 / self.headroom))
+        self._set_caps(grown)
+        self._create()
+        done = sum(len(self.updates[x % 2]) for x in layers if x < failed)
+        return done + self.run([x for x in layers if x >= failed])
+
+
 # Truncation rules per contraction precision: (tie band, drop rule), both
 # relative.  Rounding of the contraction moves degenerate twin singular values
 # apart by about its relative error, and leaves a floor of noise singular

@@ -295,6 +295,87 @@ int bsmpo_group_graph_create(const void *h_group, int precision, void *d_work, s
 int bsmpo_graph_launch(void *graph_exec, void *stream);
 int bsmpo_graph_destroy(void *graph_exec);
 
+/* ----------------------------------------------------- layer programs -- */
+
+/*
+ * A layer program: the site updates of brick-wall layers (all beam splitters
+ * of a layer, across any number of interferometer instances), planned ON THE
+ * DEVICE from the bond charges in HBM and run by the grouped stages, captured
+ * once as HIP graphs that are replayed for every layer.  Replay r applies
+ * layer r: update set r % n_sets (a brick-wall mesh has two, the even and
+ * the odd pairs), BS tables from d_B[r].  No host round trip per layer: no
+ * host planning, no charge read-back, no image upload.  A replay is a fixed
+ * launch sequence of captured graphs (the planning; per group its head, its
+ * two Householder block shapes and its tail) on the caller's stream and
+ * three streams of the program, ordered by events (HIP's graph executor runs
+ * one graph's nodes one after another, so the layer's concurrency comes from
+ * the streams).  Replaces, for the
+ * brick-wall sweep of src/beam_splitter_parallel/Full.py:149-166 (one
+ * RankProcess per beam splitter with host-side Aligner bookkeeping,
+ * src/mpo_sort.py:215-384 and src/beam_splitter_parallel/Rank.py:78-105),
+ * the per-layer host work; the per-update semantics are bsmpo_site_update's.
+ *
+ * Each update set is split into n_groups contiguous groups that run as
+ * n_streams chains (1 or 2; group g on chain g % n_streams, each chain with
+ * its own scratch arena); every group has the capacities given (bytes of its
+ * variable plan part and of its scratch, entries of each work list).
+ * A replay whose plan does not fit (or has bad charges, or no BS matrices)
+ * changes nothing: the program records the status and the replay index, and
+ * every later replay does nothing too until bsmpo_layer_seek.
+ * bsmpo_layer_state_read returns that record and what each group needed in
+ * the last planned replay (a synchronising read).
+ */
+typedef struct {
+  const int32_t *CL, *CC, *CR; /* charges of bonds mode, mode+1, mode+2 [chi][2]  */
+  double *Glc, *Gcr;           /* Gamma[mode], Gamma[mode+1] (in/out)             */
+  const double *LR;            /* Lambda[mode+1], or ones at the edge             */
+  int32_t *charge_c;           /* = CC: the new centre charges are written there */
+  double *lambda_c;            /* Lambda[mode] (out)                              */
+  int32_t *num_lambda;         /* kept count [1] (out)                            */
+  int32_t edge, reserved;
+} bsmpo_layer_update;
+typedef struct {
+  int64_t var_bytes, work_bytes;          /* plan pools and items; scratch          */
+  int32_t tiles, gram, jac, tri, sec, v, w, wyt, wya, status; /* list entries     */
+} bsmpo_layer_caps;  /* capacities per group; also what a group needed (+ status) */
+typedef struct {
+  int d, chi, precision;
+  int n_sets;                          /* 1 or 2 update sets: replay r runs set r % n_sets */
+  int n_upd[2];                        /* updates of each set                        */
+  const bsmpo_layer_update *upd[2];    /* host arrays of n_upd[s] records            */
+  int n_groups, n_streams;             /* groups per set; chains / arenas (1, 2)     */
+  int n_layers;                        /* replays with BS matrices in d_B            */
+  const double *d_B;                   /* device [n_layers][max n_upd][2][2] c128:
+                                          replay r, update u of its set              */
+  bsmpo_layer_caps caps;               /* per group                                  */
+} bsmpo_layer_spec;
+typedef struct {
+  int32_t status;      /* 0 ok; 1 over capacity; 2 bad charges; 3 tile too big;
+                          4 sector too big; 5 no BS matrices; 6 skipped          */
+  int32_t replays;     /* replay counter (the next replay's layer)               */
+  int32_t failed_at;   /* replay of the first failure (when status != 0)         */
+  int32_t n_groups;
+} bsmpo_layer_state;
+/* Bytes of the caller workspace a program needs. */
+int bsmpo_layer_query(const bsmpo_layer_spec *spec, size_t *work_bytes);
+/* Builds and captures the program (on `stream`, a non-default stream, stage
+ * timing disabled; the call synchronises that stream).  d_work stays owned
+ * by the caller and must outlive the program. */
+int bsmpo_layer_create(const bsmpo_layer_spec *spec, void *d_work, size_t work_bytes,
+                       void *stream, void **prog);
+/* Sets the replay counter to `replay` and clears the recorded failure. */
+int bsmpo_layer_seek(void *prog, int replay, void *stream);
+/* Replays the layer on `stream` (no host synchronisation; the program's own
+ * streams join `stream` before it returns control to later work on it). */
+int bsmpo_layer_launch(void *prog, void *stream);
+/* Synchronising read of the program's state; group_needs (may be NULL) gets
+ * n_groups records: what each group needed in the last planned replay. */
+int bsmpo_layer_state_read(void *prog, bsmpo_layer_state *state, bsmpo_layer_caps *group_needs);
+int bsmpo_layer_destroy(void *prog);
+/* What a group of host plans needs as a device-planned group (to size
+ * capacities), in the layout bsmpo_layer_create uses. */
+int bsmpo_layer_needs(int n_upd, const void *const *h_plans, bsmpo_layer_caps *needs);
+
 /* --------------------------------------------------------- observables -- */
 
 /*

@@ -218,3 +218,43 @@ def test_two_stage_plan_query_on_large_sectors():
     n = np.minimum(s1[:, 2], s1[:, 3])
     assert ((n >= 64) & (n <= 528)).sum() > 100
     assert p1.work_bytes >= p0.work_bytes
+
+
+def test_layer_program_query_and_needs_on_host():
+    """Layer programs (device-planned graph replays): the spec is validated
+    and the workspace sized without a device; bsmpo_layer_needs sums host
+    plans into the capacities a device-planned group needs."""
+    import ctypes as C
+    from bosonsampling_amd import _capi
+    from bosonsampling_amd.engine import Plan
+    L = _capi.lib()
+    st, _ = _state()
+    d, chi = st.d, st.chi
+    plans = [Plan(d, chi, st.charge[m], st.charge[m + 1], st.charge[m + 2]) for m in (0, 2, 4)]
+    arr = (C.c_void_p * 3)(*[p.address for p in plans])
+    need = _capi.LayerCaps()
+    _capi.check(L.bsmpo_layer_needs(3, arr, C.byref(need)), "layer_needs")
+    infos = [p.info() for p in plans]
+    assert need.sec == sum(i.n_sectors for i in infos)
+    assert need.tiles == sum(i.n_items for i in infos)
+    assert need.var_bytes > 0 and need.work_bytes > 0 and need.gram > 0 and need.w > 0
+    recs = (_capi.LayerUpdate * 2)()
+    for j in range(2):
+        recs[j] = _capi.LayerUpdate(*([8] * 9), 0, 0)  # non-NULL stand-ins: query reads no memory
+    spec = _capi.LayerSpec()
+    spec.d, spec.chi, spec.precision, spec.n_sets = d, chi, _capi.C64, 2
+    spec.n_upd[0] = spec.n_upd[1] = 2
+    spec.upd[0] = spec.upd[1] = C.cast(recs, C.c_void_p)
+    spec.n_groups, spec.n_streams, spec.n_layers, spec.d_B = 2, 2, 4, 8
+    spec.caps = need
+    size = C.c_size_t()
+    _capi.check(L.bsmpo_layer_query(C.byref(spec), C.byref(size)), "layer_query")
+    assert size.value > 2 * need.work_bytes
+    for field, bad in (("n_groups", 3), ("n_streams", 3), ("n_sets", 0), ("chi", 4096), ("d", 17)):
+        old = getattr(spec, field)
+        setattr(spec, field, bad)
+        assert L.bsmpo_layer_query(C.byref(spec), C.byref(size)) < 0
+        setattr(spec, field, old)
+    recs[1].Glc = None
+    assert L.bsmpo_layer_query(C.byref(spec), C.byref(size)) < 0
+    assert L.bsmpo_layer_launch(None, None) < 0

@@ -1,0 +1,139 @@
+"""GPU tests of layer programs: brick-wall layers planned on the device and
+replayed as one captured HIP graph per layer (bsmpo_layer_*), against the
+host-planned grouped path on the same states, and the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from oracle import mpo
+
+pytestmark = pytest.mark.gpu
+
+CASE = (10, 6, 48, [0, 2, 4, 6, 8], 0.7)
+
+
+def _meshes(M, seeds):
+    from bosonsampling_amd import mesh
+    return [mesh.haar_clements_mesh(M, s)[0] for s in seeds]
+
+
+def _max_diff(a, b):
+    ga, la, ca = a.to_host()
+    gb, lb, cb = b.to_host()
+    return (float(np.abs(ga - gb).max()), float(np.abs(la - lb).max()),
+            bool(np.array_equal(ca, cb)))
+
+
+def test_layer_program_matches_grouped_and_oracle(gpu):
+    from bosonsampling_amd import engine
+    M, d, chi, inputs, eta = CASE
+    seeds = [11, 12, 13]
+    meshes = _meshes(M, seeds)
+    ref = mpo.initialize(M, d, chi, inputs, eta)
+    grouped = [engine.DeviceMPO.from_host(ref.Gamma, ref.Lambda, ref.charge, d) for _ in seeds]
+    graph = [engine.DeviceMPO.from_host(ref.Gamma, ref.Lambda, ref.charge, d) for _ in seeds]
+    print(f"layer program case M={M} d={d} chi={chi} instances={len(seeds)}", flush=True)
+    engine.simulate_batch(grouped, meshes, inputs)
+    runner = engine.LayerGraphRunner(graph, meshes, engine.C128, n_groups=2)
+    n = runner.run(range(M))
+    assert n == sum(len(runner.updates[x % 2]) for x in range(M))
+    # capacities sized on layer 0 (a product state) are outgrown as the bond
+    # dimension builds up: run() falls back and re-plans as needed
+    print(f"fallbacks {runner.fallbacks}", flush=True)
+    st, needs = runner.state()
+    assert st.status == 0 and st.replays == M
+    assert all(nd["status"] == 0 for nd in needs)
+    for k, mm in enumerate(meshes):
+        dg, dl, same_ch = _max_diff(grouped[k], graph[k])
+        print(f"instance {k}: |dGamma| {dg:.3e} |dLambda| {dl:.3e} charges equal {same_ch}", flush=True)
+        assert same_ch and dg <= 1e-13 and dl <= 1e-13
+        orc = ref.copy()
+        mpo.simulate(orc, mm, inputs)
+        tb, mb = graph[k].observables()
+        mo = mpo.marginals(orc)
+        assert abs(tb - mpo.trace(orc)) <= 1e-10 * abs(mpo.trace(orc))
+        assert np.abs(mb - mo).max() <= 1e-10 * np.abs(mo).max()
+        assert np.array_equal(graph[k].charge_host, orc.charge)
+
+
+def test_layer_program_over_capacity_changes_nothing_then_falls_back(gpu):
+    """Capacities below what the first layer needs: that replay and the later
+    ones leave the state as it was; run() grows the capacities and finishes."""
+    from bosonsampling_amd import engine
+    M, d, chi, inputs, eta = CASE
+    seeds = [21, 22]
+    meshes = _meshes(M, seeds)
+    ref = mpo.initialize(M, d, chi, inputs, eta)
+    grouped = [engine.DeviceMPO.from_host(ref.Gamma, ref.Lambda, ref.charge, d) for _ in seeds]
+    graph = [engine.DeviceMPO.from_host(ref.Gamma, ref.Lambda, ref.charge, d) for _ in seeds]
+    engine.simulate_batch(grouped, meshes, inputs, layers=range(2))
+    engine.simulate_batch(graph, meshes, inputs, layers=range(2))
+    before = [s.clone() for s in graph]
+    runner = engine.LayerGraphRunner(graph, meshes, engine.C128, n_groups=2, headroom=1.0)
+    w = runner.needs(0)
+    runner._set_caps(w)
+    runner.caps.work_bytes = max(1, w["work_bytes"] // 2)  # the first replay cannot fit
+    runner._create()
+    # first, the failing replays alone: nothing may change
+    import ctypes as C
+    import torch
+    from bosonsampling_amd import _capi
+    lib = _capi.lib()
+    s = C.c_void_p(runner.stream.cuda_stream)
+    _capi.check(lib.bsmpo_layer_seek(runner.prog, 2, s), "seek")
+    for _ in range(3):
+        _capi.check(lib.bsmpo_layer_launch(runner.prog, s), "launch")
+    st, needs = runner.state()
+    print(f"status {st.status} failed_at {st.failed_at} replays {st.replays}", flush=True)
+    assert st.status == 1 and st.failed_at == 2 and st.replays == 5
+    assert max(nd["work_bytes"] for nd in needs) > runner.caps.work_bytes
+    for a, b in zip(before, graph):
+        dg, dl, same = _max_diff(a, b)
+        assert dg == 0.0 and dl == 0.0 and same
+    torch.cuda.synchronize()
+    # then run(): the first replay fails again, run() grows and finishes
+    runner.run(range(2, M))
+    assert runner.fallbacks >= 1
+    engine.simulate_batch(grouped, meshes, inputs, layers=range(2, M))
+    for a, b in zip(grouped, graph):
+        dg, dl, same = _max_diff(a, b)
+        assert same and dg <= 1e-13 and dl <= 1e-13
+
+
+def test_layer_program_bench_scale(gpu):
+    """M=64, N=12, d=13, chi=1024 (the benchmark's workload, two instances):
+    layers 16-23 as device-planned graph replays against host-planned groups,
+    complex128 throughout.  Charges bit-exact; Lambda, trace and marginals to
+    1e-10 relative: the Householder reduction's row sums are LDS atomics, so
+    two runs of the tridiagonal route round differently (and singular vectors
+    of near-degenerate values differ by a gauge rotation), whichever planner.
+    (With a complex64 contraction such 1e-16 differences flip single FP32
+    roundings of later layers, and two runs agree to ~1e-7 only.)"""
+    from bosonsampling_amd import engine
+    M, d, chi, eta = 64, 13, 1024, 0.6
+    inputs = list(range(0, 60, 5))
+    seeds = [100, 101]
+    meshes = _meshes(M, seeds)
+    try:
+        states = [engine.initialize(M, d, chi, inputs, eta) for _ in seeds]
+        print("bench-scale layer program: preparing layers 0-15", flush=True)
+        engine.simulate_batch(states, meshes, inputs, layers=range(16))
+        graph = [s.clone() for s in states]
+        print("host-planned groups, layers 16-23", flush=True)
+        engine.simulate_batch(states, meshes, inputs, layers=range(16, 24))
+        print("device-planned graph replays, layers 16-23", flush=True)
+        runner = engine.LayerGraphRunner(graph, meshes, engine.C128, n_groups=2)
+        runner.run(range(16, 24))
+        print(f"fallbacks {runner.fallbacks}", flush=True)
+        for a, b in zip(states, graph):
+            dg, dl, same = _max_diff(a, b)
+            ta, ma = a.observables()
+            tb, mb = b.observables()
+            print(f"|dLambda| {dl:.3e} |dtrace| {abs(ta - tb):.3e} "
+                  f"|dmarg| {np.abs(ma - mb).max():.3e} charges equal {same}", flush=True)
+            assert same
+            assert dl <= 1e-10 * float(a.Lambda.abs().max())
+            assert abs(ta - tb) <= 1e-10 * abs(ta)
+            assert np.abs(ma - mb).max() <= 1e-10 * abs(ta)
+    finally:
+        engine.set_precision_rules(engine.C128)
