@@ -264,7 +264,18 @@ def main():
         t1 = time.time()
         tuned = fixed_threshold(args)
         for layer in range(T0):
-            if args.autotune and args.eig_threshold < 0 and layer == T0 - 1:
+            if args.autotune and args.eig_threshold < 0 and layer == T0 - 1 and args.graph:
+                # tune launch and tile geometry on the last untimed layer (copies of
+                # the states, as layer-program replays); both timed runs use it
+                runner.works = [None] * len(runner.works)
+                torch.cuda.empty_cache()
+                best, times = engine.autotune_layer(states, meshes, layer, precision=prec,
+                                                    n_groups=args.graph_groups,
+                                                    headroom=args.graph_headroom, log=log)
+                tuned = {"layer": layer, **best,
+                         "seconds": {k: {str(a): round(b, 3) for a, b in v.items()}
+                                     for k, v in times.items()}}
+            elif args.autotune and args.eig_threshold < 0 and layer == T0 - 1:
                 # tune the eigensolver route split on the last untimed layer (clones)
                 best, times = engine.autotune_eig_threshold(states, meshes, inputs, layer,
                                                             precision=prec, runner=runner)

@@ -23,6 +23,10 @@ SIGNATURES = {
     "bsmpo_abi_version": (C.c_int, []),
     "bsmpo_set_eig_threshold": (C.c_int, [_i]),
     "bsmpo_set_tri_split": (C.c_int, [_i]),
+    "bsmpo_tri_split": (C.c_int, []),
+    "bsmpo_eig_threshold": (C.c_int, []),
+    "bsmpo_set_tile_order": (C.c_int, [_i]),
+    "bsmpo_tile_order": (C.c_int, []),
     "bsmpo_set_two_stage": (C.c_int, [_i]),
     "bsmpo_two_stage_threshold": (C.c_int, []),
     "bsmpo_set_tie_rel": (C.c_int, [C.c_double]),

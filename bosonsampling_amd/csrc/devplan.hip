@@ -45,23 +45,23 @@ __device__ inline char *fixed_of(const DpParams &P, const DpGroup &G, int local)
 
 // ----------------------------------------------------- contraction tiles --
 // The 16 x 16 tiles of (left row, right column) elements of an update, in the
-// host planner's order: 8 x 8-tile super-tiles row by row, tiles row by row
-// inside each; the linear index maps to the tile origin.
-__device__ inline void tile_origin(int idx, int nTR, int nTC, int &r0, int &c0) {
-  const int SR = idx / (8 * nTC);
-  int rem = idx - SR * 8 * nTC;
-  const int rows = min(8, nTR - 8 * SR);
-  const int SC = rem / (rows * 8);
-  rem -= SC * rows * 8;
-  const int cols = min(8, nTC - 8 * SC);
-  r0 = 16 * (8 * SR + rem / cols);
-  c0 = 16 * (8 * SC + rem % cols);
+// host planner's order: super-tiles of S x S tiles (S = the tile order) row by
+// row, tiles row by row inside each; the linear index maps to the tile origin.
+__device__ inline void tile_origin(int idx, int nTR, int nTC, int S, int &r0, int &c0) {
+  const int SR = idx / (S * nTC);
+  int rem = idx - SR * S * nTC;
+  const int rows = min(S, nTR - S * SR);
+  const int SC = rem / (rows * S);
+  rem -= SC * rows * S;
+  const int cols = min(S, nTC - S * SC);
+  r0 = 16 * (S * SR + rem / cols);
+  c0 = 16 * (S * SC + rem % cols);
 }
 
 struct TileCtx {
   const int4 *lb, *rb;        // blocks (c0, c1, start, end) of the left / right bond
   const int16_t *rowblk, *colblk;  // block of each slot, -1 for padding
-  int nl, nr, nLv, nRv;
+  int nl, nr, nLv, nRv, order;
 };
 
 // One pass over rows [r0 + s0, r0 + s0 + rp) of the tile (r0, c0): whether
@@ -120,7 +120,7 @@ template <class Emit>
 __device__ bool tile_passes(const TileCtx &t, int idx, Emit emit) {
   const int nTR = cdiv(t.nLv, 16), nTC = cdiv(t.nRv, 16);
   int r0, c0;
-  tile_origin(idx, nTR, nTC, r0, c0);
+  tile_origin(idx, nTR, nTC, t.order, r0, c0);
   const int tnr = min(16, t.nLv - r0), tnc = min(16, t.nRv - c0);
   int lo0 = kMaxD, lo1 = kMaxD, c_lo = t.nr, c_hi = -1;
   for (int j = c0; j < c0 + tnc; ++j) {
@@ -391,6 +391,7 @@ __global__ __launch_bounds__(kSizeThreads) void k_dp_sizes(DpParams P,
   tc.nr = nr;
   tc.nLv = nl > 0 ? s_blk[0][nl - 1].w : 0;
   tc.nRv = nr > 0 ? s_blk[2][nr - 1].w : 0;
+  tc.order = P.tile_order;
   const int ntiles = cdiv(tc.nLv, 16) * cdiv(tc.nRv, 16);
   int cnt[kTileClasses] = {};
   bool fits = true;
@@ -721,6 +722,7 @@ __global__ __launch_bounds__(256) void k_dp_tiles(DpParams P, const DpUpdate *__
   tc.nr = s.n_rblk;
   tc.nLv = s.n_lblk > 0 ? s_lb[s.n_lblk - 1].w : 0;
   tc.nRv = s.n_rblk > 0 ? s_rb[s.n_rblk - 1].w : 0;
+  tc.order = P.tile_order;
   const int ntiles = cdiv(tc.nLv, 16) * cdiv(tc.nRv, 16);
   const VarLayout V = var_layout(s.rows_total, s.cols_total, s.n_gram, s.n_w, s.n_v, s.sv_total,
                                  s.n_titems);

@@ -55,6 +55,7 @@ __host__ __device__ inline int bc_vectors(int n) {
 int eig_threshold();  // sectors with n > threshold take the tridiagonal route
 int two_stage_min();  // tridiagonal-route sectors with n >= this take the two-stage reduction
 int tri_split();      // many-sector groups: Householder blocks of 512 threads for n > split
+int tile_order();     // contraction tiles are visited in super-tiles of order x order tiles
 double tie_rel();     // relative width of the tie band at the chi-th singular value
 double drop_rel();    // singular values <= drop_rel * the sector's largest are dropped
 
@@ -366,7 +367,7 @@ struct DpGroup {
   int32_t first[2], n_upd[2];  // per update set: its updates [first, first + n_upd)
 };
 struct DpParams {
-  int32_t d, chi, thr, tri_split, n_groups, n_layers;
+  int32_t d, chi, thr, tri_split, n_groups, n_layers, tile_order, pad0;
   int32_t n_sets, max_upd;     // replay r runs update set r % n_sets
   int32_t n_upd[2], off[2];    // per set: updates, first record in the update array
   double tie_rel, drop_rel;

@@ -324,6 +324,7 @@ int create(const bsmpo_layer_spec *s, void *d_work, size_t work_bytes, hipStream
   P.chi = s->chi;
   P.thr = eig_threshold();
   P.tri_split = tri_split();
+  P.tile_order = tile_order();
   P.n_groups = s->n_groups;
   P.n_layers = s->n_layers;
   P.n_sets = s->n_sets;

@@ -25,9 +25,11 @@ static std::atomic<int> g_tri_split{384};
 static std::atomic<double> g_tie_rel{1e-6};
 static std::atomic<double> g_drop_rel{1e-6};
 static std::atomic<int> g_two_stage{0};
+static std::atomic<int> g_tile_order{8};
 int eig_threshold() { return g_eig_threshold; }
 int two_stage_min() { return g_two_stage; }
 int tri_split() { return g_tri_split; }
+int tile_order() { return g_tile_order; }
 double tie_rel() { return g_tie_rel; }
 double drop_rel() { return g_drop_rel; }
 
@@ -320,7 +322,8 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
       }
     }
     // contraction tiles: 16 x 16 (left row, right column) tiles, visited in
-    // 8 x 8 super-tiles so that neighbouring items share operand panels
+    // super-tiles of tile_order() x tile_order() tiles so that neighbouring
+    // items share operand panels
     b.row_blk.assign((size_t)chi, -1);
     b.col_blk.assign((size_t)chi, -1);
     int nLv = 0, nRv = 0;
@@ -388,10 +391,11 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
       return ps;
     };
     std::vector<Pass> passes;
-    for (int R0 = 0; R0 < nLv; R0 += 128)
-      for (int C0 = 0; C0 < nRv; C0 += 128)
-        for (int r0 = R0; r0 < std::min(R0 + 128, nLv); r0 += 16)
-          for (int c0 = C0; c0 < std::min(C0 + 128, nRv); c0 += 16) {
+    const int sup = 16 * tile_order();  // super-tile side in elements
+    for (int R0 = 0; R0 < nLv; R0 += sup)
+      for (int C0 = 0; C0 < nRv; C0 += sup)
+        for (int r0 = R0; r0 < std::min(R0 + sup, nLv); r0 += 16)
+          for (int c0 = C0; c0 < std::min(C0 + sup, nRv); c0 += 16) {
             const int tnr = std::min(16, nLv - r0), tnc = std::min(16, nRv - c0);
             int lo0 = d, lo1 = d, c_lo = nr, c_hi = -1;
             for (int j = c0; j < c0 + tnc; ++j) {
@@ -516,7 +520,7 @@ int build(int d, int chi, const int32_t *CL, const int32_t *CC, const int32_t *C
 // The last plan built on this thread, so that query + build cost one build.
 struct Cached {
   bool valid = false;
-  int d = 0, chi = 0, t0 = 0, t1 = 0, thr = 0, tsm = 0;
+  int d = 0, chi = 0, t0 = 0, t1 = 0, thr = 0, tsm = 0, order = 0;
   double tie = 0.0, drop = 0.0;
   std::vector<int32_t> key;
   Built b;
@@ -529,6 +533,7 @@ int build_cached(int d, int chi, const int32_t *CL, const int32_t *CC, const int
   const size_t n = (size_t)2 * chi;
   bool hit = c.valid && c.d == d && c.chi == chi && c.t0 == t0 && c.t1 == t1 &&
              c.thr == eig_threshold() && c.tsm == two_stage_min() && c.tie == tie_rel() &&
+             c.order == tile_order() &&
              c.drop == drop_rel() &&
              c.key.size() == 3 * n &&
              !std::memcmp(c.key.data(), CL, n * 4) && !std::memcmp(c.key.data() + n, CC, n * 4) &&
@@ -545,6 +550,7 @@ int build_cached(int d, int chi, const int32_t *CL, const int32_t *CC, const int
     c.d = d; c.chi = chi; c.t0 = t0; c.t1 = t1; c.thr = eig_threshold(); c.tie = tie_rel();
     c.drop = drop_rel();
     c.tsm = two_stage_min();
+    c.order = tile_order();
     c.valid = true;
   }
   *out = &c.b;
@@ -603,6 +609,18 @@ int bsmpo_set_two_stage(int n) {
   g_two_stage = n == 0 ? 0 : std::max(n, kTwoStageMinN);
   return 0;
 }
+
+int bsmpo_set_tile_order(int n) {
+  if (n != 2 && n != 4 && n != 8 && n != 16) return fail(-1, "tile order must be 2, 4, 8 or 16");
+  g_tile_order = n;
+  return 0;
+}
+
+int bsmpo_tile_order(void) { return g_tile_order; }
+
+int bsmpo_tri_split(void) { return g_tri_split; }
+
+int bsmpo_eig_threshold(void) { return g_eig_threshold; }
 
 int bsmpo_set_tri_split(int n) {
   if (n < 1 || n > 512) return fail(-1, "tridiagonal block-shape split must be in [1, 512]");

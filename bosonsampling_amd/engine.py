@@ -627,6 +627,75 @@ def autotune_eig_threshold(states, meshes, input_modes, layer, precision=C128, r
     return best, times
 
 
+# Launch and tile geometry the layer-program autotuner searches, in the order
+# it tunes them: the eigensolver route split (bsmpo_set_eig_threshold), the
+# Householder block-shape split (bsmpo_set_tri_split) and the contraction's
+# super-tile order (bsmpo_set_tile_order).  Every setting gives the same
+# results to rounding.
+LAYER_KNOBS = (("eig_threshold", (0, 32, 64, 96)),
+               ("tri_split", (256, 384, 512)),
+               ("tile_order", (4, 8, 16)))
+
+
+def _knob(name, value=None):
+    if value is None:
+        return int(getattr(lib(), f"bsmpo_{name}")())
+    check(getattr(lib(), f"bsmpo_set_{name}")(int(value)), f"set_{name}")
+    return int(value)
+
+
+def autotune_layer(states, meshes, layer, precision=C128, knobs=LAYER_KNOBS, n_groups=8,
+                   repeats=2, tie=0.01, headroom=1.3, log=None):
+    """Tunes the launch and tile geometry of the layer-program path (device
+    planning, graph replays) by timing one replay of `layer` per candidate on
+    copies of `states` (restored before every trial), one knob at a time in
+    `knobs` order, each knob's best kept for the next (coordinate descent).
+    Each candidate is timed `repeats` times, the passes alternating in order,
+    and keeps its shortest time; candidates within `tie` of the fastest count
+    as equal, and among them the value in effect before tuning stays (else
+    the fastest), so that timing noise does not move a knob.  The chosen
+    values are set (process-wide) and returned with every timing:
+    ({knob: value}, {knob: {value: seconds}})."""
+    import time
+    trial = [st.clone() for st in states]
+    best, table = {}, {}
+    for name, cands in knobs:
+        incumbent = _knob(name)
+        times = {}
+        order = []
+        for r in range(max(1, int(repeats))):
+            order += list(cands) if r % 2 == 0 else list(cands)[::-1]
+        for v in order:
+            _knob(name, v)
+            for t, st in zip(trial, states):
+                t.Gamma.copy_(st.Gamma)
+                t.Lambda.copy_(st.Lambda)
+                t.charge.copy_(st.charge)
+                t.charge_host[:] = st.charge_host
+            runner = LayerGraphRunner(trial, meshes, precision, n_groups=n_groups,
+                                      headroom=headroom)
+            runner._set_caps(runner.needs(layer % 2))
+            runner._create()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            runner.run([layer])
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            runner.destroy()
+            del runner
+            times[int(v)] = min(el, times.get(int(v), el))
+        fastest = min(times.values())
+        near = [v for v in cands if times[int(v)] <= (1.0 + tie) * fastest]
+        pick = incumbent if incumbent in near else min(near, key=lambda v: times[int(v)])
+        best[name] = _knob(name, pick)
+        table[name] = times
+        if log:
+            log(f"[autotune] {name}: {pick} " + " ".join(f"{k}:{v:.3f}" for k, v in times.items()))
+    del trial
+    torch.cuda.empty_cache()
+    return best, table
+
+
 def input_density(d: int, eta: float) -> np.ndarray:
     """Single photon through a pure-loss channel of transmission eta:
     (1 - eta)|0><0| + eta|1><1| (src/MPO.py:41-44 with loss = 1 - eta)."""

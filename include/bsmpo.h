@@ -56,6 +56,21 @@ int bsmpo_set_eig_threshold(int n);
  * bsmpo_set_eig_threshold).
  */
 int bsmpo_set_tri_split(int n);
+/* The block-shape split and the route threshold in effect. */
+int bsmpo_tri_split(void);
+int bsmpo_eig_threshold(void);
+
+/*
+ * Contraction tile order (process-wide, affects plans built afterwards): the
+ * contraction's 16 x 16 tiles of (left row, right column) elements are listed
+ * in super-tiles of order x order tiles, row by row, so that neighbouring work
+ * items (which run on one XCD and share its L2) read the same operand panels.
+ * Default 8, allowed 2, 4, 8, 16; every value gives the same results.  No
+ * counterpart in the reference (its kernel tiles each sector on its own,
+ * src/update_MPO_kernel.cu:193-734); engine.autotune_layer picks it.
+ */
+int bsmpo_set_tile_order(int n);
+int bsmpo_tile_order(void);
 
 /*
  * Two-stage reduction threshold (process-wide, affects plans built

@@ -137,3 +137,50 @@ def test_layer_program_bench_scale(gpu):
             assert np.abs(ma - mb).max() <= 1e-10 * abs(ta)
     finally:
         engine.set_precision_rules(engine.C128)
+
+
+def test_layer_autotune_and_geometry_knobs(gpu):
+    """The autotuner's knobs (route split, Householder block-shape split,
+    contraction tile order) change launch geometry only: every setting gives
+    the state of the default one (tile order: bit-exact on the Jacobi route;
+    the tridiagonal route to rounding), and autotune_layer sets the values it
+    returns."""
+    from bosonsampling_amd import engine
+    M, d, chi, inputs, eta = CASE
+    seeds = [31, 32]
+    meshes = _meshes(M, seeds)
+    ref = mpo.initialize(M, d, chi, inputs, eta)
+    base = [engine.DeviceMPO.from_host(ref.Gamma, ref.Lambda, ref.charge, d) for _ in seeds]
+    engine.simulate_batch(base, meshes, inputs, layers=range(3))
+    saved = {n: engine._knob(n) for n, _ in engine.LAYER_KNOBS}
+    try:
+        results = {}
+        for thr, split, order in ((96, 384, 8), (96, 384, 2), (96, 384, 16), (0, 384, 8),
+                                  (0, 256, 4), (0, 512, 16)):
+            engine._knob("eig_threshold", thr)
+            engine._knob("tri_split", split)
+            engine._knob("tile_order", order)
+            sts = [s.clone() for s in base]
+            runner = engine.LayerGraphRunner(sts, meshes, engine.C128, n_groups=2)
+            runner.run(range(3, M))
+            results[(thr, split, order)] = sts
+            print(f"knobs {thr} {split} {order}: fallbacks {runner.fallbacks}", flush=True)
+        for key, sts in results.items():
+            ref_key = (96, 384, 8) if key[0] == 96 else (0, 384, 8)
+            for a, b in zip(results[ref_key], sts):
+                dg, dl, same = _max_diff(a, b)
+                ta, ma = a.observables()
+                tb, mb = b.observables()
+                assert same
+                if key[0] == 96:  # Jacobi route: deterministic, identical work per sector
+                    assert dg == 0.0 and dl == 0.0
+                assert abs(ta - tb) <= 1e-10 * abs(ta) and np.abs(ma - mb).max() <= 1e-10 * abs(ta)
+        best, table = engine.autotune_layer(base, meshes, 3, precision=engine.C128, n_groups=2,
+                                            repeats=1)
+        print(f"autotuned {best}", flush=True)
+        for name, cands in engine.LAYER_KNOBS:
+            assert best[name] in cands and engine._knob(name) == best[name]
+            assert set(table[name]) == set(cands)
+    finally:
+        for n, v in saved.items():
+            engine._knob(n, v)
