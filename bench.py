@@ -81,7 +81,9 @@ def parse():
     ap.add_argument("--graph", type=int, default=1,
                     help="1: also time the grouped path as device-planned layer programs (one "
                          "captured HIP graph per layer); `value` is then that run's rate")
-    ap.add_argument("--graph-groups", type=int, default=8, help="groups per layer of the graph run")
+    ap.add_argument("--graph-groups", type=int, default=12, help="groups per layer of the graph run")
+    ap.add_argument("--graph-chains", type=int, default=3, choices=[1, 2, 3, 4],
+                    help="concurrent chains of groups (scratch arenas) of the graph run")
     ap.add_argument("--graph-headroom", type=float, default=1.3,
                     help="capacities of the graph run's groups over what layer --start needs")
     ap.add_argument("--verbose", action="store_true")
@@ -271,6 +273,7 @@ def main():
                 torch.cuda.empty_cache()
                 best, times = engine.autotune_layer(states, meshes, layer, precision=prec,
                                                     n_groups=args.graph_groups,
+                                                    n_streams=args.graph_chains,
                                                     headroom=args.graph_headroom, log=log)
                 tuned = {"layer": layer, **best,
                          "seconds": {k: {str(a): round(b, 3) for a, b in v.items()}
@@ -326,7 +329,7 @@ def main():
             rules = engine.set_precision_rules(prec)
             t1 = time.time()
             lg = engine.LayerGraphRunner(graph_states, meshes, prec, n_groups=args.graph_groups,
-                                         n_streams=2, headroom=args.graph_headroom)
+                                         n_streams=args.graph_chains, headroom=args.graph_headroom)
             lg._set_caps(lg.needs(T0 % 2))
             lg._create()
             torch.cuda.synchronize()
@@ -340,6 +343,7 @@ def main():
                      for a, b in zip(final_eager, graph_states))
             result["graph"] = {"updates": ng, "seconds": elg, "rate": ng / elg,
                                "fallbacks": lg.fallbacks, "groups": args.graph_groups,
+                               "chains": args.graph_chains,
                                "marginals_vs_host_planned": dm,
                                "bond": [bond0, kept_bond_dimension(graph_states[0])]}
             engine.set_precision_rules(engine.C128)
@@ -392,12 +396,14 @@ def main():
             "plain_fp64": ({"instances": 1, "updates": p["updates"],
                             "updates_per_s": p["rate"], "autotune": p.get("tuned")} if p else None),
             "graph": ({"updates": gr["updates"], "updates_per_s": gr["rate"],
-                       "groups_per_layer": gr["groups"], "fallbacks": gr["fallbacks"],
+                       "groups_per_layer": gr["groups"], "chains": gr["chains"],
+                       "fallbacks": gr["fallbacks"],
                        "marginals_vs_host_planned": gr["marginals_vs_host_planned"],
-                       "note": "value: the grouped path as layer programs (device planning, "
-                               "one HIP graph replayed per layer); `grouped` is the same "
-                               "work host-planned group by group, whose stage events give "
-                               "the roofline"} if gr else None),
+                       "note": "value: the grouped path as layer programs (planned on the "
+                               "device, captured HIP graphs replayed per layer, no host "
+                               "round trip); `grouped` is the same work host-planned group "
+                               "by group, whose stage events give the roofline"}
+                      if gr else None),
             "grouped_over_plain": (main_path["rate"] / p["rate"]) if (g and p) else None,
             "precision_check": precision_check(g, p, args.precision, T0 + K),
         },

@@ -36,6 +36,8 @@ void trace(const char *what) {
 }
 int64_t region(int64_t bytes) { return align256(std::max<int64_t>(bytes, 8)); }
 
+constexpr int kMaxChains = 4;  // scratch arenas, each a chain of groups
+
 struct LayerProgram {
   int device = 0, precision = 0, n_streams = 1;  // n_streams: scratch arenas (concurrent chains)
   DpParams params{};
@@ -52,8 +54,8 @@ struct LayerProgram {
   // streams of a replay besides the caller's: the second arena's chain, and
   // per chain the stream of its 256-thread Householder launches; events that
   // order them
-  hipStream_t chain2 = nullptr, side[2] = {nullptr, nullptr};
-  hipEvent_t ev_plan = nullptr, ev_done = nullptr;
+  hipStream_t chain[kMaxChains] = {}, side[kMaxChains] = {};  // chain[0]: the caller's
+  hipEvent_t ev_plan = nullptr, ev_done[kMaxChains] = {};
   std::vector<hipEvent_t> ev_head, ev_small;
   ~LayerProgram() {
     if (plan) (void)hipGraphExecDestroy(plan);
@@ -63,8 +65,10 @@ struct LayerProgram {
     for (hipEvent_t e : ev_head) (void)hipEventDestroy(e);
     for (hipEvent_t e : ev_small) (void)hipEventDestroy(e);
     if (ev_plan) (void)hipEventDestroy(ev_plan);
-    if (ev_done) (void)hipEventDestroy(ev_done);
-    if (chain2) (void)hipStreamDestroy(chain2);
+    for (hipEvent_t e : ev_done)
+      if (e) (void)hipEventDestroy(e);
+    for (int c = 1; c < kMaxChains; ++c)
+      if (chain[c]) (void)hipStreamDestroy(chain[c]);
     for (hipStream_t x : side)
       if (x) (void)hipStreamDestroy(x);
   }
@@ -98,7 +102,8 @@ int check_spec(const bsmpo_layer_spec *s) {
                             std::to_string(k));
     }
   }
-  if (s->n_streams < 1 || s->n_streams > 2) return fail(-1, "layer: n_streams must be 1 or 2");
+  if (s->n_streams < 1 || s->n_streams > kMaxChains)
+    return fail(-1, "layer: n_streams must be in [1, 4]");
   if (s->n_layers < 1 || !s->d_B) return fail(-1, "layer: no BS matrices");
   const bsmpo_layer_caps &c = s->caps;
   if (c.var_bytes < 1 || c.work_bytes < 1 || c.tiles < 1 || c.gram < 1 || c.jac < 0 || c.tri < 0 ||
@@ -274,19 +279,19 @@ int build_graphs(LayerProgram &p, hipStream_t st) {
 }
 
 // One replay: the planning graph on `st`, then the groups in turn on their
-// arena's chain (group g on chain g % n_streams: the caller's stream or
-// chain2), each as head -> (512-thread Householder on the chain, 256-thread
+// arena's chain (group g on chain g % n_streams: chain 0 is the caller's
+// stream), each as head -> (512-thread Householder on the chain, 256-thread
 // on the chain's side stream) -> tail; the caller's stream joins the chains.
 int layer_launch(LayerProgram &p, hipStream_t st) {
   BSMPO_HIP(hipGraphLaunch(p.plan, st));
-  hipStream_t chain[2] = {st, p.chain2};
-  if (p.n_streams == 2) {
+  p.chain[0] = st;
+  if (p.n_streams > 1) {
     BSMPO_HIP(hipEventRecord(p.ev_plan, st));
-    BSMPO_HIP(hipStreamWaitEvent(p.chain2, p.ev_plan, 0));
+    for (int c = 1; c < p.n_streams; ++c) BSMPO_HIP(hipStreamWaitEvent(p.chain[c], p.ev_plan, 0));
   }
   for (int g = 0; g < (int)p.parts.size(); ++g) {
     const int c = g % p.n_streams;
-    const hipStream_t cs = chain[c], ss = p.side[c];
+    const hipStream_t cs = p.chain[c], ss = p.side[c];
     auto &q = p.parts[g];
     if (q[0]) BSMPO_HIP(hipGraphLaunch(q[0], cs));
     if (q[2]) {
@@ -299,9 +304,9 @@ int layer_launch(LayerProgram &p, hipStream_t st) {
     if (q[2]) BSMPO_HIP(hipStreamWaitEvent(cs, p.ev_small[g], 0));
     if (q[3]) BSMPO_HIP(hipGraphLaunch(q[3], cs));
   }
-  if (p.n_streams == 2) {
-    BSMPO_HIP(hipEventRecord(p.ev_done, p.chain2));
-    BSMPO_HIP(hipStreamWaitEvent(st, p.ev_done, 0));
+  for (int c = 1; c < p.n_streams; ++c) {
+    BSMPO_HIP(hipEventRecord(p.ev_done[c], p.chain[c]));
+    BSMPO_HIP(hipStreamWaitEvent(st, p.ev_done[c], 0));
   }
   return 0;
 }
@@ -419,13 +424,14 @@ int create(const bsmpo_layer_spec *s, void *d_work, size_t work_bytes, hipStream
     trace(bsmpo_last_error());
     return rc;
   }
-  BSMPO_HIP(hipStreamCreateWithFlags(&p->side[0], hipStreamNonBlocking));
-  if (p->n_streams == 2) {
-    BSMPO_HIP(hipStreamCreateWithFlags(&p->chain2, hipStreamNonBlocking));
-    BSMPO_HIP(hipStreamCreateWithFlags(&p->side[1], hipStreamNonBlocking));
+  for (int c = 0; c < p->n_streams; ++c) {
+    BSMPO_HIP(hipStreamCreateWithFlags(&p->side[c], hipStreamNonBlocking));
+    if (c > 0) {
+      BSMPO_HIP(hipStreamCreateWithFlags(&p->chain[c], hipStreamNonBlocking));
+      BSMPO_HIP(hipEventCreateWithFlags(&p->ev_done[c], hipEventDisableTiming));
+    }
   }
   BSMPO_HIP(hipEventCreateWithFlags(&p->ev_plan, hipEventDisableTiming));
-  BSMPO_HIP(hipEventCreateWithFlags(&p->ev_done, hipEventDisableTiming));
   p->ev_head.assign(p->parts.size(), nullptr);
   p->ev_small.assign(p->parts.size(), nullptr);
   for (size_t g = 0; g < p->parts.size(); ++g) {

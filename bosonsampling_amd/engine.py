@@ -645,7 +645,7 @@ def _knob(name, value=None):
 
 
 def autotune_layer(states, meshes, layer, precision=C128, knobs=LAYER_KNOBS, n_groups=8,
-                   repeats=2, tie=0.01, headroom=1.3, log=None):
+                   n_streams=2, repeats=2, tie=0.01, headroom=1.3, log=None):
     """Tunes the launch and tile geometry of the layer-program path (device
     planning, graph replays) by timing one replay of `layer` per candidate on
     copies of `states` (restored before every trial), one knob at a time in
@@ -673,7 +673,7 @@ def autotune_layer(states, meshes, layer, precision=C128, knobs=LAYER_KNOBS, n_g
                 t.charge.copy_(st.charge)
                 t.charge_host[:] = st.charge_host
             runner = LayerGraphRunner(trial, meshes, precision, n_groups=n_groups,
-                                      headroom=headroom)
+                                      n_streams=n_streams, headroom=headroom)
             runner._set_caps(runner.needs(layer % 2))
             runner._create()
             torch.cuda.synchronize()

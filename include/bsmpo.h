@@ -331,7 +331,7 @@ int bsmpo_graph_destroy(void *graph_exec);
  * the per-layer host work; the per-update semantics are bsmpo_site_update's.
  *
  * Each update set is split into n_groups contiguous groups that run as
- * n_streams chains (1 or 2; group g on chain g % n_streams, each chain with
+ * n_streams chains (1 to 4; group g on chain g % n_streams, each chain with
  * its own scratch arena); every group has the capacities given (bytes of its
  * variable plan part and of its scratch, entries of each work list).
  * A replay whose plan does not fit (or has bad charges, or no BS matrices)
@@ -358,7 +358,7 @@ typedef struct {
   int n_sets;                          /* 1 or 2 update sets: replay r runs set r % n_sets */
   int n_upd[2];                        /* updates of each set                        */
   const bsmpo_layer_update *upd[2];    /* host arrays of n_upd[s] records            */
-  int n_groups, n_streams;             /* groups per set; chains / arenas (1, 2)     */
+  int n_groups, n_streams;             /* groups per set; chains / arenas (1..4)    */
   int n_layers;                        /* replays with BS matrices in d_B            */
   const double *d_B;                   /* device [n_layers][max n_upd][2][2] c128:
                                           replay r, update u of its set              */

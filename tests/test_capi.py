@@ -250,7 +250,7 @@ def test_layer_program_query_and_needs_on_host():
     size = C.c_size_t()
     _capi.check(L.bsmpo_layer_query(C.byref(spec), C.byref(size)), "layer_query")
     assert size.value > 2 * need.work_bytes
-    for field, bad in (("n_groups", 3), ("n_streams", 3), ("n_sets", 0), ("chi", 4096), ("d", 17)):
+    for field, bad in (("n_groups", 3), ("n_streams", 5), ("n_sets", 0), ("chi", 4096), ("d", 17)):
         old = getattr(spec, field)
         setattr(spec, field, bad)
         assert L.bsmpo_layer_query(C.byref(spec), C.byref(size)) < 0
