@@ -239,6 +239,7 @@ def main():
         if args.cpu_seconds > 0:
             snap = st.to_host()
         bond0 = kept_bond_dimension(st)
+        obs_start = st.observables()
         _capi.timing_enable(True)
         _capi.timing_read(reset=True)
         torch.cuda.synchronize()
@@ -250,6 +251,7 @@ def main():
         _capi.timing_enable(False)
         result["plain"] = {"updates": n, "seconds": el, "rate": n / el,
                            "observables": st.observables(), "tuned": plain_tuned,
+                           "observables_start": obs_start,
                            "bond": [bond0, kept_bond_dimension(st)]}
         log(f"[bench] plain c128: {n} updates in {el:.3f}s = {n / el:.2f} updates/s")
         stage_table(st_plain)
@@ -295,6 +297,7 @@ def main():
         log(f"[bench] grouped untimed {T0} layers x {B} instances {time.time() - t1:.1f}s")
         graph_states = [s.clone() for s in states] if args.graph else None
         bond0 = kept_bond_dimension(states[0])
+        g_obs_start = states[0].observables()
         _capi.timing_enable(True)
         _capi.timing_read(reset=True)
         torch.cuda.synchronize()
@@ -311,6 +314,7 @@ def main():
         _capi.timing_enable(False)
         result["grouped"] = {"updates": n, "seconds": el, "rate": n / el,
                              "observables0": states[0].observables(), "tuned": tuned,
+                             "observables0_start": g_obs_start,
                              "bond": [bond0, kept_bond_dimension(states[0])], "rules": rules}
         engine.set_precision_rules(engine.C128)
         log(f"[bench] grouped {args.precision}: {n} updates in {el:.3f}s = {n / el:.2f} updates/s")
@@ -431,9 +435,14 @@ def precision_check(g, p, precision, layers):
         return None
     tg, mg = g["observables0"]
     tp, mp = p["observables"]
-    return {"precision": precision, "layers": layers, "reference": "plain complex128, same mesh",
-            "marginal_max_abs_err": float(np.abs(mg - mp).max()),
-            "trace_abs_err": float(abs(tg - tp)), "trace": float(tp)}
+    out = {"precision": precision, "layers": layers, "reference": "plain complex128, same mesh",
+           "marginal_max_abs_err": float(np.abs(mg - mp).max()),
+           "trace_abs_err": float(abs(tg - tp)), "trace": float(tp)}
+    if "observables0_start" in g and "observables_start" in p:
+        (ts, ms), (tq, mq) = g["observables0_start"], p["observables_start"]
+        out["at_first_timed_layer"] = {"marginal_max_abs_err": float(np.abs(ms - mq).max()),
+                                       "trace_abs_err": float(abs(ts - tq))}
+    return out
 
 
 def cpu_baseline(snap, d, chi, mm, inputs, layer, budget_s):

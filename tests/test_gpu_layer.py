@@ -184,3 +184,30 @@ def test_layer_autotune_and_geometry_knobs(gpu):
     finally:
         for n, v in saved.items():
             engine._knob(n, v)
+
+
+@pytest.mark.timeout(600)
+def test_layer_program_chi2048(gpu):
+    """The largest bond dimension a layer program plans: M=64, d=13,
+    chi=2048 (seed 100), layers 14-16 as graph replays against host-planned
+    groups from the same state, complex128.  Charges bit-exact; trace and
+    marginals to 1e-10 relative."""
+    from bosonsampling_amd import engine
+    M, d, chi = 64, 13, 2048
+    inputs = list(range(0, 60, 5))
+    meshes = _meshes(M, [100])
+    st = engine.initialize(M, d, chi, inputs, 0.6)
+    print("chi=2048 layer program: preparing layers 0-13", flush=True)
+    engine.simulate_batch([st], meshes, inputs, layers=range(14))
+    graph = st.clone()
+    engine.simulate_batch([st], meshes, inputs, layers=range(14, 17))
+    print("graph replays, layers 14-16", flush=True)
+    runner = engine.LayerGraphRunner([graph], meshes, engine.C128, n_groups=2)
+    runner.run(range(14, 17))
+    dg, dl, same = _max_diff(st, graph)
+    ta, ma = st.observables()
+    tb, mb = graph.observables()
+    print(f"fallbacks {runner.fallbacks} |dLambda| {dl:.3e} |dtrace| {abs(ta - tb):.3e} "
+          f"|dmarg| {np.abs(ma - mb).max():.3e}", flush=True)
+    assert same
+    assert abs(ta - tb) <= 1e-10 * abs(ta) and np.abs(ma - mb).max() <= 1e-10 * abs(ta)
