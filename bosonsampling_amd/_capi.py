@@ -104,7 +104,7 @@ class LayerUpdate(C.Structure):
     _fields_ = [("CL", C.c_void_p), ("CC", C.c_void_p), ("CR", C.c_void_p), ("Glc", C.c_void_p),
                 ("Gcr", C.c_void_p), ("LR", C.c_void_p), ("charge_c", C.c_void_p),
                 ("lambda_c", C.c_void_p), ("num_lambda", C.c_void_p), ("edge", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("first_replay", C.c_int32)]
 
 
 class LayerCaps(C.Structure):

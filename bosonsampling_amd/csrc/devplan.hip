@@ -180,6 +180,13 @@ __global__ __launch_bounds__(kSizeThreads) void k_dp_sizes(DpParams P,
     if (tid == 0) sz.status = DP_SKIPPED;
     return;
   }
+  if (P.state[1] < up.first_replay) {  // beyond the light cone in this replay
+    if (tid == 0) {
+      sz.status = DP_OK;
+      sz.active = 0;
+    }
+    return;
+  }
   const FixedLayout F = fixed_layout(d, chi);
   char *fx = fixed_of(P, G, up.local);
   Sector *sectors = reinterpret_cast<Sector *>(fx + F.o_sectors);
@@ -345,6 +352,7 @@ __global__ __launch_bounds__(kSizeThreads) void k_dp_sizes(DpParams P,
     }
     s_nsec = ns;
     sz.status = err;
+    sz.active = 1;
     sz.n_sec = ns;
     sz.n_lblk = nl;
     sz.n_cblk = nc;
@@ -434,10 +442,12 @@ __global__ void k_dp_scan(DpParams P, DpGroup *__restrict__ Gs, DpSizes *__restr
   int64_t var = 0, work = 0;
   int64_t tiles[kTileClasses] = {}, jac[kTriClass] = {};
   int64_t gram = 0, tri = 0, sec = 0, nv = 0, nw = 0;
+  int nact = 0;
   for (int i = 0; i < nupd; ++i) {
     DpSizes &s = Sz[first + i];
     if (s.status != DP_OK && status == DP_OK) status = s.status;
-    if (s.status != DP_OK) continue;
+    if (s.status != DP_OK || !s.active) continue;
+    s.desc = nact++;
     s.var_off = var;
     var += region(s.var_bytes);
     s.work_off = work;
@@ -467,6 +477,7 @@ __global__ void k_dp_scan(DpParams P, DpGroup *__restrict__ Gs, DpSizes *__restr
   for (int c = 0; c < kTriClass; ++c) jac_off[c + 1] = jac_off[c] + jac[c];
   for (int i = 0; i < nupd && status == DP_OK; ++i) {
     DpSizes &s = Sz[first + i];
+    if (!s.active) continue;
     for (int c = 0; c < kTileClasses; ++c) s.l_tile[c] += (int32_t)tile_off[c];
     for (int c = 0; c < kTriClass; ++c) s.l_jac[c] += (int32_t)jac_off[c];
   }
@@ -489,7 +500,7 @@ __global__ void k_dp_scan(DpParams P, DpGroup *__restrict__ Gs, DpSizes *__restr
   for (int k = 0; k < SG_COUNT; ++k) K.seg[k] = make_int2(0, 0);
   K.status = status;
   if (status != DP_OK) return;
-  K.seg[SG_UPD] = make_int2(0, nupd);
+  K.seg[SG_UPD] = make_int2(0, nact);
   for (int c = 0; c < kTileClasses; ++c)
     K.seg[SG_TILE + c] = make_int2((int)tile_off[c], (int)tiles[c]);
   K.seg[SG_GRAM] = make_int2(0, (int)gram);
@@ -514,6 +525,7 @@ __global__ __launch_bounds__(256) void k_dp_fill(DpParams P, const DpUpdate *__r
   const DpGroup &G = Gs[up.group];
   if (reinterpret_cast<const GroupCounts *>(G.staging)->status != DP_OK) return;
   const DpSizes s = Sz[u];
+  if (!s.active) return;
   const int d = P.d, chi = P.chi;
   const FixedLayout F = fixed_layout(d, chi);
   char *fx = fixed_of(P, G, up.local);
@@ -537,7 +549,7 @@ __global__ __launch_bounds__(256) void k_dp_fill(DpParams P, const DpUpdate *__r
   int2 *l_v = reinterpret_cast<int2 *>(G.staging + G.o_v);
   int2 *l_sec = reinterpret_cast<int2 *>(G.staging + G.o_sec);
   int2 *l_jac = reinterpret_cast<int2 *>(G.staging + G.o_jac);
-  const int me = up.local;
+  const int me = s.desc;  // descriptor and list index among the group's active updates
   if (blockIdx.y == 0 && tid == 0) {
     const WorkLayout W = work_layout(s.c_elems, s.g_elems, s.v_elems, s.w_elems, s.sv_total,
                                      s.n_sec, s.tw);
@@ -694,6 +706,7 @@ __global__ __launch_bounds__(256) void k_dp_tiles(DpParams P, const DpUpdate *__
   const DpGroup &G = Gs[up.group];
   if (reinterpret_cast<const GroupCounts *>(G.staging)->status != DP_OK) return;
   const DpSizes s = Sz[u];
+  if (!s.active) return;
   const int d = P.d, chi = P.chi;
   const FixedLayout F = fixed_layout(d, chi);
   const char *fx = fixed_of(P, G, up.local);
@@ -745,7 +758,7 @@ __global__ __launch_bounds__(256) void k_dp_tiles(DpParams P, const DpUpdate *__
       tile_passes(tc, idx, [&](const TileItem &it, int c) {
         const int pos = base[c] + k[c];
         titems[pos] = it;
-        list[s.l_tile[c] + k[c]] = make_int2(up.local, pos);
+        list[s.l_tile[c] + k[c]] = make_int2(s.desc, pos);
         ++k[c];
       });
     }
@@ -770,10 +783,8 @@ __global__ __launch_bounds__(1024) void k_dp_tri(DpParams P, DpGroup *__restrict
   const int ntri = K.seg[SG_TRI].y, chi = P.chi;
   int2 *out = reinterpret_cast<int2 *>(G.staging + G.o_tri);
   const int2 *in = G.tri_tmp;
-  auto size_of = [&](int2 r) {
-    const char *fx = fixed_of(P, G, r.x);
-    return reinterpret_cast<const Sector *>(fx + fixed_layout(P.d, chi).o_sectors)[r.y].n;
-  };
+  const UpdDesc *desc = reinterpret_cast<const UpdDesc *>(G.staging + G.o_desc);
+  auto size_of = [&](int2 r) { return desc[r.x].sectors[r.y].n; };
   for (int i = tid; i <= chi; i += NT) hist[i] = 0;
   if (tid == 0) s_over = 0;
   __syncthreads();

@@ -320,13 +320,15 @@ struct DpUpdate {
   double *lambda_c;
   int32_t *num_lambda;
   int32_t edge, group;          // group: index of the update's group in its set
-  int32_t local, pad;           // position inside the group
+  int32_t local;                // position inside the group
+  int32_t first_replay;         // replays before it skip the update (light cone)
 };
 // sizes found by the first planning kernel, offsets by the group scan (one
 // record per update of the replay's set)
 struct DpSizes {
   int32_t status, n_sec, n_lblk, n_cblk, n_rblk, rows_total, cols_total, n_gram;
-  int32_t n_w, n_v, sv_total, n_tri, n_titems, pad0;
+  int32_t n_w, n_v, sv_total, n_tri, n_titems, active;  // active: runs in this replay
+  int32_t desc, pad2;  // its descriptor (and list) index among the group's active updates
   int32_t jac_cnt[kTriClass], tile_cnt[kTileClasses];
   int64_t c_elems, g_elems, v_elems, w_elems, tw;
   int64_t var_bytes, work_bytes, var_off, work_off;

@@ -393,7 +393,7 @@ int create(const bsmpo_layer_spec *s, void *d_work, size_t work_bytes, hipStream
         U.edge = x.edge;
         U.group = g;
         U.local = i;
-        U.pad = 0;
+        U.first_replay = x.first_replay;
       }
     }
     p->cap.push_back(capacity_header(s, L, mg, P.thr));

@@ -355,27 +355,32 @@ class GroupRunner:
 
 class LayerGraphRunner:
     """Brick-wall layers of several instances, each layer planned on the
-    device and run as ONE captured HIP graph (libbsmpo's bsmpo_layer_*).
+    device and replayed from captured HIP graphs (libbsmpo's bsmpo_layer_*).
 
     One layer program holds the two update sets of a brick-wall mesh (every
     even pair, every odd pair, across all instances) and is captured once;
     replay r applies layer r: set r % 2, BS tables from layer r's matrices
     (every layer's matrices are uploaded once), plans built on the device
-    from the charges in HBM, nothing read back between layers.
-    ``run(layers)`` takes consecutive full layers (every pair active, i.e.
-    past the light cone) and reads the program's state once, at its end.  A
-    replay whose plan does not fit the capacities changes nothing and stops
-    the later ones; ``run`` then re-creates the program with what the device
-    reported it needed and runs the rest again (``fallbacks`` counts that).
+    from the charges in HBM, nothing read back between layers.  Given the
+    input modes, a replay skips the pairs beyond the light cone
+    (Full.py:152-153), so any layers of a mesh can run this way.
+    ``run(layers)`` takes consecutive layers and reads the program's state
+    once, at its end.  A replay whose plan does not fit the capacities
+    changes nothing and stops the later ones; ``run`` then re-creates the
+    program with what the device reported it needed and runs the rest again
+    (``fallbacks`` counts that).
 
     Capacities per group come from host plans of the first layer's updates
     (bsmpo_layer_needs) times ``headroom``.  Each update set is split into
-    ``n_groups`` contiguous groups launched in turn on ``n_streams`` streams.
+    ``n_groups`` contiguous groups that run as ``n_streams`` chains.
     """
 
     def __init__(self, states, meshes, precision: int = C128, n_groups: int = 8,
-                 n_streams: int = 2, headroom: float = 1.5):
+                 n_streams: int = 2, headroom: float = 1.5, input_modes=None):
         self.states, self.meshes = list(states), list(meshes)
+        # light cone (Full.py:152-153): the pair at `mode` first moves at layer
+        # mode - (last occupied input); None: every pair from layer 0
+        self.max_input = None if input_modes is None else max(int(m) for m in input_modes)
         st0 = self.states[0]
         self.M, self.d, self.chi, self.device = st0.M, st0.d, st0.chi, st0.device
         self.precision = int(precision)
@@ -404,11 +409,12 @@ class LayerGraphRunner:
         for j, (_, st, _, mode) in enumerate(self.updates[p]):
             edge = mode == st.M - 2
             LR = st.edge if edge else st.Lambda[mode + 1]
+            first = 0 if self.max_input is None else max(0, mode - self.max_input)
             recs[j] = _capi.LayerUpdate(st.charge[mode].data_ptr(), st.charge[mode + 1].data_ptr(),
                                         st.charge[mode + 2].data_ptr(), st.Gamma[mode].data_ptr(),
                                         st.Gamma[mode + 1].data_ptr(), LR.data_ptr(),
                                         st.charge[mode + 1].data_ptr(), st.Lambda[mode].data_ptr(),
-                                        st.num_lambda[mode].data_ptr(), 1 if edge else 0, 0)
+                                        st.num_lambda[mode].data_ptr(), 1 if edge else 0, first)
         return recs
 
     def _spec(self):
@@ -493,9 +499,16 @@ class LayerGraphRunner:
         for st in self.states:
             st.charge_host[:] = st.charge.cpu().numpy()
 
+    def _active(self, layer):
+        ups = self.updates[layer % 2]
+        if self.max_input is None:
+            return len(ups)
+        return sum(1 for _, _, _, mode in ups if mode <= self.max_input + layer)
+
     def run(self, layers):
-        """Applies the given consecutive full layers; returns the number of
-        beam-splitter updates applied."""
+        """Applies the given consecutive layers (pairs beyond the light cone
+        skipped when the runner knows the input modes, else every pair);
+        returns the number of beam-splitter updates applied."""
         layers = list(layers)
         for a, b in zip(layers, layers[1:]):
             if b != a + 1:
@@ -523,7 +536,7 @@ class LayerGraphRunner:
         st, needs = self.state()  # the one synchronising read
         self._sync_host_charges()
         if st.status == 0:
-            return sum(len(self.updates[x % 2]) for x in layers)
+            return sum(self._active(x) for x in layers)
         if st.status not in (1, 6):
             raise _capi.BsmpoError(f"layer program: {_capi.LAYER_STATUS.get(st.status, st.status)} "
                                    f"at layer {st.failed_at}")
@@ -538,7 +551,7 @@ class LayerGraphRunner:
                 grown[f] = max(need, int(getattr(self.caps, f) * 1.25 / self.headroom))
         self._set_caps(grown)
         self._create()
-        done = sum(len(self.updates[x % 2]) for x in layers if x < failed)
+        done = sum(self._active(x) for x in layers if x < failed)
         return done + self.run([x for x in layers if x >= failed])
 
 
@@ -563,7 +576,22 @@ def simulate_batch(states, meshes, input_modes, layers=None, precision=C128, run
                    observe=False):
     """Layer-synchronous simulation of several independent instances: every
     non-skipped beam splitter of a layer, over all instances, in one grouped
-    run (one launch per stage)."""
+    run (one launch per stage).  With a LayerGraphRunner for these states as
+    `runner` (built with the same input modes and precision), consecutive
+    layers are planned on the device and replayed as graphs instead."""
+    if isinstance(runner, LayerGraphRunner):
+        M = states[0].M
+        todo = list(range(M) if layers is None else layers)
+        probs = [[] for _ in states]
+        n_updates = 0
+        if observe:
+            for layer in todo:
+                n_updates += runner.run([layer])
+                for k, st in enumerate(states):
+                    probs[k].append(st.observables()[0])
+        elif todo:
+            n_updates = runner.run(todo)
+        return n_updates, probs
     runner = runner or GroupRunner(states[0].device)
     M, d = states[0].M, states[0].d
     max_input = max(input_modes)

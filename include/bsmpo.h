@@ -347,7 +347,9 @@ typedef struct {
   int32_t *charge_c;           /* = CC: the new centre charges are written there */
   double *lambda_c;            /* Lambda[mode] (out)                              */
   int32_t *num_lambda;         /* kept count [1] (out)                            */
-  int32_t edge, reserved;
+  int32_t edge;
+  int32_t first_replay;        /* replays r < first_replay skip the update (the
+                                  light cone, Full.py:152-153); 0: every replay   */
 } bsmpo_layer_update;
 typedef struct {
   int64_t var_bytes, work_bytes;          /* plan pools and items; scratch          */

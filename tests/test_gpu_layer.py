@@ -211,3 +211,25 @@ def test_layer_program_chi2048(gpu):
           f"|dmarg| {np.abs(ma - mb).max():.3e}", flush=True)
     assert same
     assert abs(ta - tb) <= 1e-10 * abs(ta) and np.abs(ma - mb).max() <= 1e-10 * abs(ta)
+
+
+def test_layer_program_light_cone_matches_golden(gpu):
+    """The M=8 golden run (photons in modes 0-3, d=5, chi=16, 28 beam
+    splitters; the first layers skip the pairs beyond the light cone) as
+    layer-program replays of two copies, every layer observed: trace per
+    layer and final marginals to 1e-10 of the oracle's fixture."""
+    import os
+    from bosonsampling_amd import engine
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "m8_sp4.npz"))
+    inputs = [0, 1, 2, 3]
+    states = [engine.DeviceMPO.from_host(g["init_Gamma"], g["init_Lambda"], g["init_charge"], 5)
+              for _ in range(2)]
+    meshes = [g["mesh"], g["mesh"]]
+    runner = engine.LayerGraphRunner(states, meshes, engine.C128, n_groups=1, n_streams=1,
+                                     input_modes=inputs)
+    n, probs = engine.simulate_batch(states, meshes, inputs, runner=runner, observe=True)
+    assert n == 2 * sum(len(engine.layer_modes(8, l, 3)) for l in range(8))
+    for k in range(2):
+        assert np.abs(np.array(probs[k]) - g["prob"][1:]).max() <= 1e-10
+        tr, marg = states[k].observables()
+        assert np.abs(marg - g["marginals"]).max() <= 1e-10
