@@ -63,8 +63,10 @@ def parse():
     ap.add_argument("--precision", default="c64", choices=["c128", "c64", "bf16x3"])
     ap.add_argument("--paths", default="plain,grouped")
     ap.add_argument("--group-gib", type=float, default=24.0, help="workspace bound of one group")
-    ap.add_argument("--streams", type=int, default=2, choices=[1, 2, 3, 4],
-                    help="launching streams the groups of a layer alternate between")
+    ap.add_argument("--streams", type=int, default=1, choices=[1, 2, 3, 4],
+                    help="launching streams the host-planned run's groups alternate between "
+                         "(1: each stage's HIP events time the stage alone, the roofline's "
+                         "source; 2 is that path's fastest setting)")
     ap.add_argument("--first-frac", type=float, default=0.25,
                     help="workspace share of a layer's first group (planned while the device idles)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,

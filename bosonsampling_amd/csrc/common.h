@@ -162,6 +162,22 @@ __host__ __device__ inline DevPlan dev_plan(const PlanHeader &h, const void *wor
   return p;
 }
 
+// XCD-aware order of a grid-stride walk over a work list: walk position p
+// runs on XCD p % 8 (workgroups are dispatched round robin over the 8 XCDs,
+// and the grid is a multiple of 8), so each XCD takes one contiguous eighth
+// of the list, and neighbouring items, which read the same operand panels,
+// share its L2.  Returns the item of position p (>= nlist: none).
+__device__ __forceinline__ int xcd_item(int p, int nlist) {
+  const int chunk = (nlist + 7) >> 3;
+  return (p & 7) * chunk + (p >> 3);
+}
+__device__ __forceinline__ int xcd_positions(int nlist) { return ((nlist + 7) >> 3) << 3; }
+// a grid for such a walk: a multiple of 8, at most `cap` workgroups
+inline int xcd_grid(int n, int cap = 16384) {
+  const int g = (n + 7) / 8 * 8;
+  return g < 8 ? 8 : (g > cap ? cap : g);
+}
+
 inline int grid_for(int64_t waves_needed) {
   int64_t blocks = (waves_needed + 3) / 4;
   if (blocks < 1) blocks = 1;

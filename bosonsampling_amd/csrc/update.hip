@@ -168,7 +168,9 @@ __global__ __launch_bounds__(256) void k_gram(const UpdDesc *__restrict__ Ud, Wo
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i16 = lane & 15, kq = lane >> 4;
   const int tr = wave >> 1, tc = wave & 1;
-  for (int it = blockIdx.x; it < nlist; it += gridDim.x) {
+  for (int pos = blockIdx.x; pos < xcd_positions(nlist); pos += gridDim.x) {
+    const int it = xcd_item(pos, nlist);
+    if (it >= nlist) continue;  // uniform over the block
     const int2 ref = list[it];
     const UpdDesc &u = Ud[ref.x];
     const Item item = u.gram_items[ref.y];
@@ -358,7 +360,9 @@ __global__ __launch_bounds__(256) void k_vectors(const UpdDesc *__restrict__ Ud,
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i16 = lane & 15, kq = lane >> 4;
   const int tr = wave >> 1, tc = wave & 1;
-  for (int it = blockIdx.x; it < nlist; it += gridDim.x) {
+  for (int pos = blockIdx.x; pos < xcd_positions(nlist); pos += gridDim.x) {
+    const int it = xcd_item(pos, nlist);
+    if (it >= nlist) continue;  // uniform over the block
     const int2 ref = list[it];
     const UpdDesc &u = Ud[ref.x];
     const Item item = u.v_items[ref.y];
@@ -398,7 +402,9 @@ __global__ __launch_bounds__(256) void k_wmat(const UpdDesc *__restrict__ Ud, Wo
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i16 = lane & 15, kq = lane >> 4;
   const int tr = wave >> 1, tc = wave & 1;
-  for (int it = blockIdx.x; it < nlist; it += gridDim.x) {
+  for (int pos = blockIdx.x; pos < xcd_positions(nlist); pos += gridDim.x) {
+    const int it = xcd_item(pos, nlist);
+    if (it >= nlist) continue;  // uniform over the block
     const int2 ref = list[it];
     const UpdDesc &u = Ud[ref.x];
     const Item item = u.w_items[ref.y];
@@ -610,7 +616,7 @@ int launch_svd(const GroupHeader &g, const char *w, hipStream_t stream) {
 int launch_svd_head(const GroupHeader &g, const char *w, hipStream_t stream) {
   if (g.n_sec == 0) return 0;
   int tk = timing_begin(ST_GRAM, stream);
-  hipLaunchKernelGGL(k_gram, dim3(std::max(1, std::min(g.n_gram, 16384))), dim3(256), 0, stream, gdesc(g, w),
+  hipLaunchKernelGGL(k_gram, dim3(xcd_grid(g.n_gram)), dim3(256), 0, stream, gdesc(g, w),
                      wseg(g, w, g.o_gram, SG_GRAM));
   BSMPO_HIP(hipGetLastError());
   timing_end(tk, stream, g.flops_gram, 16.0 * (g.c_elems + g.g_elems));
@@ -638,14 +644,14 @@ int launch_truncate(const GroupHeader &g, const char *w, hipStream_t stream) {
   if ((rc = launch_tri_vectors(g, w, stream))) return rc;
   if (g.n_v) {
     tk = timing_begin(ST_VECTORS, stream);
-    hipLaunchKernelGGL(k_vectors, dim3(std::min(g.n_v, 16384)), dim3(256), 0, stream, gdesc(g, w),
+    hipLaunchKernelGGL(k_vectors, dim3(xcd_grid(g.n_v)), dim3(256), 0, stream, gdesc(g, w),
                        wseg(g, w, g.o_v, SG_V));
     BSMPO_HIP(hipGetLastError());
     timing_end(tk, stream, 0.0, 0.0);  // flops: kept pairs, counted by k_kcount
   }
   if (g.n_w) {
     tk = timing_begin(ST_WMAT, stream);
-    hipLaunchKernelGGL(k_wmat, dim3(std::min(g.n_w, 16384)), dim3(256), 0, stream, gdesc(g, w),
+    hipLaunchKernelGGL(k_wmat, dim3(xcd_grid(g.n_w)), dim3(256), 0, stream, gdesc(g, w),
                        wseg(g, w, g.o_w, SG_W));
     BSMPO_HIP(hipGetLastError());
     timing_end(tk, stream, 0.0, 16.0 * (g.v_elems + g.c_elems + g.w_elems));  // flops: k_kcount
