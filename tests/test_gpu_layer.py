@@ -233,3 +233,29 @@ def test_layer_program_light_cone_matches_golden(gpu):
         assert np.abs(np.array(probs[k]) - g["prob"][1:]).max() <= 1e-10
         tr, marg = states[k].observables()
         assert np.abs(marg - g["marginals"]).max() <= 1e-10
+
+
+@pytest.mark.parametrize("prec_name", ["c64", "bf16x3"])
+def test_layer_program_low_precision_matches_grouped(gpu, prec_name):
+    """The reduced-precision contractions as layer programs: the state of the
+    host-planned grouped run with the same precision and truncation rules,
+    bit-exact (Jacobi route: every stage deterministic)."""
+    from bosonsampling_amd import engine
+    prec = {"c64": engine.C64, "bf16x3": engine.BF16X3}[prec_name]
+    M, d, chi, inputs, eta = CASE
+    seeds = [41, 42]
+    meshes = _meshes(M, seeds)
+    ref = mpo.initialize(M, d, chi, inputs, eta)
+    engine.set_precision_rules(prec)
+    try:
+        grouped = [engine.DeviceMPO.from_host(ref.Gamma, ref.Lambda, ref.charge, d) for _ in seeds]
+        graph = [engine.DeviceMPO.from_host(ref.Gamma, ref.Lambda, ref.charge, d) for _ in seeds]
+        engine.simulate_batch(grouped, meshes, inputs, precision=prec)
+        runner = engine.LayerGraphRunner(graph, meshes, prec, n_groups=2, input_modes=inputs)
+        engine.simulate_batch(graph, meshes, inputs, runner=runner)
+        for a, b in zip(grouped, graph):
+            dg, dl, same = _max_diff(a, b)
+            print(f"{prec_name}: |dGamma| {dg:.3e} |dLambda| {dl:.3e} charges equal {same}", flush=True)
+            assert same and dg == 0.0 and dl == 0.0
+    finally:
+        engine.set_precision_rules(engine.C128)
