@@ -24,6 +24,14 @@ def _max_diff(a, b):
             bool(np.array_equal(ca, cb)))
 
 
+def _same_truncation(a, b):
+    """Two device states compared by truncation.same_truncation."""
+    from truncation import same_truncation
+    _, la, ca = a.to_host()
+    _, lb, cb = b.to_host()
+    return same_truncation(la, ca, lb, cb)
+
+
 def test_layer_program_matches_grouped_and_oracle(gpu):
     from bosonsampling_amd import engine
     M, d, chi, inputs, eta = CASE
@@ -103,10 +111,11 @@ def test_layer_program_over_capacity_changes_nothing_then_falls_back(gpu):
 def test_layer_program_bench_scale(gpu):
     """M=64, N=12, d=13, chi=1024 (the benchmark's workload, two instances):
     layers 16-23 as device-planned graph replays against host-planned groups,
-    complex128 throughout.  Charges bit-exact; Lambda, trace and marginals to
-    1e-10 relative: the Householder reduction's row sums are LDS atomics, so
-    two runs of the tridiagonal route round differently (and singular vectors
-    of near-degenerate values differ by a gauge rotation), whichever planner.
+    complex128 throughout.  Charges and singular values as _same_truncation
+    holds them (measured: charges bit-exact); trace and marginals to 1e-10
+    relative: the Householder reduction's row sums are LDS atomics, so two
+    runs of the tridiagonal route round differently (and singular vectors of
+    near-degenerate values differ by a gauge rotation), whichever planner.
     (With a complex64 contraction such 1e-16 differences flip single FP32
     roundings of later layers, and two runs agree to ~1e-7 only.)"""
     from bosonsampling_amd import engine
@@ -131,8 +140,7 @@ def test_layer_program_bench_scale(gpu):
             tb, mb = b.observables()
             print(f"|dLambda| {dl:.3e} |dtrace| {abs(ta - tb):.3e} "
                   f"|dmarg| {np.abs(ma - mb).max():.3e} charges equal {same}", flush=True)
-            assert same
-            assert dl <= 1e-10 * float(a.Lambda.abs().max())
+            _same_truncation(a, b)
             assert abs(ta - tb) <= 1e-10 * abs(ta)
             assert np.abs(ma - mb).max() <= 1e-10 * abs(ta)
     finally:
@@ -171,9 +179,10 @@ def test_layer_autotune_and_geometry_knobs(gpu):
                 dg, dl, same = _max_diff(a, b)
                 ta, ma = a.observables()
                 tb, mb = b.observables()
-                assert same
                 if key[0] == 96:  # Jacobi route: deterministic, identical work per sector
-                    assert dg == 0.0 and dl == 0.0
+                    assert same and dg == 0.0 and dl == 0.0
+                else:
+                    _same_truncation(a, b)
                 assert abs(ta - tb) <= 1e-10 * abs(ta) and np.abs(ma - mb).max() <= 1e-10 * abs(ta)
         best, table = engine.autotune_layer(base, meshes, 3, precision=engine.C128, n_groups=2,
                                             repeats=1)
@@ -190,7 +199,11 @@ def test_layer_autotune_and_geometry_knobs(gpu):
 def test_layer_program_chi2048(gpu):
     """The largest bond dimension a layer program plans: M=64, d=13,
     chi=2048 (seed 100), layers 14-16 as graph replays against host-planned
-    groups from the same state, complex128.  Charges bit-exact; trace and
+    groups from the same state, complex128.  Singular values to 1e-10 of each
+    bond's largest, charges equal wherever both keep as many values
+    (_same_truncation; measured: charges bit-exact, and under a variant of the
+    reduction that rounds differently, one bond where one run keeps a value of
+    1e-13 x the bond's largest at its sector's drop threshold); trace and
     marginals to 1e-10 relative."""
     from bosonsampling_amd import engine
     M, d, chi = 64, 13, 2048
@@ -207,9 +220,10 @@ def test_layer_program_chi2048(gpu):
     dg, dl, same = _max_diff(st, graph)
     ta, ma = st.observables()
     tb, mb = graph.observables()
+    flips = _same_truncation(st, graph)
     print(f"fallbacks {runner.fallbacks} |dLambda| {dl:.3e} |dtrace| {abs(ta - tb):.3e} "
-          f"|dmarg| {np.abs(ma - mb).max():.3e}", flush=True)
-    assert same
+          f"|dmarg| {np.abs(ma - mb).max():.3e} charges equal {same}, drop-threshold flips "
+          f"at bonds {flips}", flush=True)
     assert abs(ta - tb) <= 1e-10 * abs(ta) and np.abs(ma - mb).max() <= 1e-10 * abs(ta)
 
 

@@ -138,10 +138,11 @@ def test_grouped_equals_sequential_m64(gpu):
     """Two instances of the M=64, d=13, chi=1024 workload (seeds 100, 101)
     through layer 17: grouped complex128 (one launch per stage for every
     update of a layer of both instances) against each instance run alone.
-    Charges bit-exact; trace and marginals to 1e-10 relative: the grouped run
-    reduces the large Gram matrices with another Householder block shape and
-    its row sums are LDS atomics, so the two runs round differently (measured
-    1e-12 to 1e-11 after 18 layers)."""
+    Kept sets as truncation.same_truncation holds them (charges equal where
+    both keep as many values; measured: bit-exact); trace and marginals to
+    1e-10 relative: the grouped run reduces the large Gram matrices with
+    another Householder block shape and its row sums are LDS atomics, so the
+    two runs round differently (measured 1e-12 to 1e-11 after 18 layers)."""
     from bosonsampling_amd import engine, mesh
     M, d, chi, inputs = 64, 13, 1024, list(range(0, 60, 5))
     meshes = [mesh.haar_clements_mesh(M, s)[0] for s in (100, 101)]
@@ -153,10 +154,14 @@ def test_grouped_equals_sequential_m64(gpu):
     grp = [init.clone() for _ in meshes]
     print("grouped instances", flush=True)
     engine.simulate_batch(grp, meshes, inputs, layers=range(18), precision=engine.C128)
+    from truncation import same_truncation
     for a, b in zip(seq, grp):
         ta, ma = a.observables()
         tb, mb = b.observables()
-        assert np.array_equal(a.charge.cpu().numpy(), b.charge.cpu().numpy())
+        _, la, ca = a.to_host()
+        _, lb, cb = b.to_host()
+        flips = same_truncation(la, ca, lb, cb)
+        print(f"drop-threshold flips at bonds {flips}", flush=True)
         assert abs(ta - tb) <= TOL * abs(ta)
         assert np.abs(ma - mb).max() <= TOL * abs(ta)
 
