@@ -38,6 +38,16 @@ __device__ __forceinline__ double2 gload2(gptr<const double2> p, int64_t i) {
 __device__ __forceinline__ void gstore2(gptr<double2> p, int64_t i, double x, double y) {
   ((gptr<dbl2v>)p)[i] = dbl2v{x, y};
 }
+// Pointer into LDS (address space 3), where code is written once for an
+// array in LDS or in global memory: a generic pointer would compile to flat
+// loads, which count against both wait counters.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using lptr = __attribute__((address_space(3))) T *;
+#else
+template <typename T>
+using lptr = T *;
+#endif
 
 struct cplx {
   double x, y;

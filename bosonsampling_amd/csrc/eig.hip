@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 
 #include "common.h"
 
@@ -164,8 +165,17 @@ __device__ __forceinline__ double2 wave_colsum(const double2 (&x)[SW], int lane)
   else return wave_colsum4(x, lane);
 }
 
-// One sector's reduction by the whole block (tsm: 2 n complex of LDS).
-template <int NT, int RPT, int SW>
+// LDS of a sector's reduction: v and A22 v (2 n complex), and with VWL the
+// panel's V and W (2 n kTriNB complex) too.
+__host__ __device__ constexpr int64_t tri_lds_need(int n, bool vwl) {
+  return 16 * (3 * (int64_t)n + 2 + (vwl ? 2 * (int64_t)n * kTriNB : 0));
+}
+
+// One sector's reduction by the whole block.  LDS (tsm, complex entries): v,
+// A22 v and the stashed next column (n each), the two pivot slots, then with
+// VWL the panel's V and W (n kTriNB each); without VWL, V and W live in the
+// sector's tridiagonal work area in global memory.
+template <int NT, int RPT, int SW, bool VWL>
 __device__ __forceinline__ void tridiag_sector(const UpdDesc &u, const Sector &s, double2 *tsm,
                                                double2 (*red)[16], double (*red4)[4 * kTriNB]) {
   constexpr int NW = NT / 64;
@@ -173,8 +183,15 @@ __device__ __forceinline__ void tridiag_sector(const UpdDesc &u, const Sector &s
   gptr<double2> A = gp(reinterpret_cast<double2 *>(u.G) + s.g_off);  // A(r, c) = A[c n + r]
   double2 *vs = tsm;                                                  // v_j, n entries
   double2 *ys = tsm + n;                                              // A22 v_j, n entries
-  gptr<double2> Vp = gp(reinterpret_cast<double2 *>(u.triwork + s.tw_off));
-  gptr<double2> Wp = Vp + (int64_t)n * kTriNB;
+  const lptr<double2> nx = (lptr<double2>)(tsm + 2 * n);  // column j + 1 as (c) of column j read it
+  double2 *piv = tsm + 3 * n;  // column j's diagonal and subdiagonal entries
+  using VWP = typename std::conditional<VWL, lptr<double2>, gptr<double2>>::type;
+  VWP Vp;
+  if constexpr (VWL)
+    Vp = (VWP)(tsm + 3 * n + 2);
+  else
+    Vp = gp(reinterpret_cast<double2 *>(u.triwork + s.tw_off));
+  const VWP Wp = Vp + (int64_t)n * kTriNB;
   gptr<double> d = gp(u.td + s.s_off), e = gp(u.te + s.s_off);
   gptr<double2> tau = gp(reinterpret_cast<double2 *>(u.tau) + s.s_off);
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -210,38 +227,63 @@ __device__ __forceinline__ void tridiag_sector(const UpdDesc &u, const Sector &s
     const int pb = min(kTriNB, n - 1 - j0);
     for (int i = 0; i < pb; ++i) {
       const int j = j0 + i;
-      // (a) column j, rows r >= j, gets the panel's columns l < i
+      // (a) column j, rows r >= j (thread t: rows j + t + k NT), gets the
+      //     panel's columns l < i and stays in registers; inside a panel it is
+      //     read from the copy the previous column's product (c) left in LDS
+      //     (the trailing matrix is unchanged within a panel), at a panel's
+      //     first column from global memory.  The diagonal entry and the
+      //     subdiagonal one (alpha) go to two LDS slots for every thread.
+      double2 col[RPT];
       double xn2 = 0.0;
-      for (int r = j + t; r < n; r += NT) {
-        double2 a = A[(int64_t)j * n + r];
-        a.x *= scl;
-        a.y *= scl;
-        for (int l = 0; l < i; ++l) {
-          a = csub_mul(a, Vp[(int64_t)l * n + r], Wp[(int64_t)l * n + j]);
-          a = csub_mul(a, Wp[(int64_t)l * n + r], Vp[(int64_t)l * n + j]);
+      // the load as a parameter, so that the LDS and the global variants stay
+      // two loops (one selected address would be a flat load)
+      auto column = [&](auto load) {
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+          const int r = j + t + k * NT;
+          col[k] = make_double2(0.0, 0.0);
+          if (r < n) {
+            double2 a = load(r);
+            a.x *= scl;
+            a.y *= scl;
+            for (int l = 0; l < i; ++l) {
+              a = csub_mul(a, Vp[(int64_t)l * n + r], Wp[(int64_t)l * n + j]);
+              a = csub_mul(a, Wp[(int64_t)l * n + r], Vp[(int64_t)l * n + j]);
+            }
+            col[k] = a;
+            if (r >= j + 2) xn2 += a.x * a.x + a.y * a.y;
+            if (r <= j + 1) piv[r - j] = a;
+          }
         }
-        A[(int64_t)j * n + r] = a;
-        if (r >= j + 2) xn2 += a.x * a.x + a.y * a.y;
-      }
-      xn2 = block_sum2<NW>(xn2, 0.0, red, par).x;  // also orders the column writes
-      // (b) reflector of column j, kept below the subdiagonal
-      const double2 alpha = A[(int64_t)j * n + j + 1];
+      };
+      if (i > 0)  // uniform
+        column([&](int r) -> double2 { return nx[r]; });
+      else
+        column([&](int r) -> double2 { return A[(int64_t)j * n + r]; });
+      xn2 = block_sum2<NW>(xn2, 0.0, red, par).x;  // also orders the pivot slots
+      // (b) reflector of column j; v below the subdiagonal to global memory
+      //     (the back-transformation's copy), v and zeroed A22 v to LDS
+      const double2 alpha = piv[1];
       const Reflector R = make_reflector(alpha, xn2);
       if (t == 0) {
-        d[j] = A[(int64_t)j * n + j].x;
+        d[j] = piv[0].x;
         e[j] = R.beta;
         tau[j] = R.tau;
       }
-      for (int r = j + 1 + t; r < n; r += NT) {
-        double2 vr;
-        if (r == j + 1) {
-          vr = make_double2(1.0, 0.0);
-        } else {
-          vr = cmul2(A[(int64_t)j * n + r], R.scale);
-          A[(int64_t)j * n + r] = vr;
+#pragma unroll
+      for (int k = 0; k < RPT; ++k) {
+        const int r = j + t + k * NT;
+        if (r > j && r < n) {
+          double2 vr;
+          if (r == j + 1) {
+            vr = make_double2(1.0, 0.0);
+          } else {
+            vr = cmul2(col[k], R.scale);
+            A[(int64_t)j * n + r] = vr;
+          }
+          vs[r] = vr;
+          ys[r] = make_double2(0.0, 0.0);
         }
-        vs[r] = vr;
-        ys[r] = make_double2(0.0, 0.0);
       }
       __syncthreads();
       // (c) p = A22 v over rows r >= j+1, A22 as of the panel start with only
@@ -281,6 +323,7 @@ __device__ __forceinline__ void tridiag_sector(const UpdDesc &u, const Sector &s
           fetch(c, sc, rc);
           const int r = ra + lane;
           if (r < n) {
+            if (sa == j + 1) nx[r] = a[0];  // wave-uniform: column j + 1, for (a)
             const double2 vr = vs[r];
             double2 rp = make_double2(0.0, 0.0);
 #pragma unroll
@@ -479,9 +522,12 @@ __device__ __forceinline__ void tridiag_sector(const UpdDesc &u, const Sector &s
 
 // Blocks walk the list (grid-stride, largest sectors first); sectors with n
 // outside [nmin, nmax] belong to the other block shape's launch.
+// Column steps (a) and (b) hold RPT rows per thread: the block shapes cover
+// n <= NT RPT (512 x 4 = 2048 = kTriMaxN; 256 x 2 = 512 >= the block-shape split).
+static_assert(kTriMaxN <= 512 * 4, "512-thread Householder blocks hold 4 rows per thread");
 template <int NT, int RPT, int SW, int OCC>
 __global__ __launch_bounds__(NT, OCC) void k_tridiag(const UpdDesc *__restrict__ Ud, WorkSeg ws,
-                                                int nmin, int nmax) {
+                                                int nmin, int nmax, int lds_bytes) {
   extern __shared__ double2 tsm[];
   __shared__ double2 red[2][16];
   __shared__ double red4[NT / 64][4 * kTriNB];
@@ -494,7 +540,10 @@ __global__ __launch_bounds__(NT, OCC) void k_tridiag(const UpdDesc *__restrict__
     const Sector s = u.sectors[ref.y];
     if (s.n < nmin || s.n > nmax) continue;  // uniform
     __syncthreads();  // the previous sector's readers of LDS are done
-    tridiag_sector<NT, RPT, SW>(u, s, tsm, red, red4);
+    if (tri_lds_need(s.n, true) <= lds_bytes)  // uniform: V and W fit in the LDS
+      tridiag_sector<NT, RPT, SW, true>(u, s, tsm, red, red4);
+    else
+      tridiag_sector<NT, RPT, SW, false>(u, s, tsm, red, red4);
   }
 }
 
@@ -1105,10 +1154,23 @@ static int side_stream(hipStream_t parent, SideStream **out) {
   return 0;
 }
 
+// Dynamic LDS of a Householder launch of NT-thread blocks whose largest
+// sector has order nmax: v and A22 v, plus the panel's V and W up to the
+// block's share of the CU's LDS at its occupancy (one 512-thread block or two
+// 256-thread blocks per CU; static arrays 2.5 / 1.5 KiB); sectors whose V and
+// W do not fit keep them in global memory.
+template <int NT>
+static int tri_lds(int nmax) {
+  constexpr int share = (160 * 1024 / (NT >= 512 ? 1 : 2) - (NT >= 512 ? 2560 : 1536)) & ~255;
+  const int64_t want = tri_lds_need(nmax, true);
+  return (int)std::max<int64_t>(tri_lds_need(nmax, false), std::min<int64_t>(want, share));
+}
+
 static int tri_limits(const GroupHeader &g) {
   if (g.tri_max_n > kTriMaxN) return fail(-1, "sector too large for the tridiagonal route (n > 2048)");
-  if (int rc = ensure_lds_limit((const void *)k_tridiag<256, 2, kTriSW, kTriOcc>, 512 * 32)) return rc;
-  return ensure_lds_limit((const void *)k_tridiag<512, 4, kTriSW, kTriOcc>, kTriMaxN * 32);
+  if (int rc = ensure_lds_limit((const void *)k_tridiag<256, 2, kTriSW, kTriOcc>, tri_lds<256>(512)))
+    return rc;
+  return ensure_lds_limit((const void *)k_tridiag<512, 4, kTriSW, kTriOcc>, tri_lds<512>(kTriMaxN));
 }
 
 // One of the two Householder launches of a device-planned group (its
@@ -1120,14 +1182,15 @@ int launch_tridiag_part(const GroupHeader &g, const char *w, hipStream_t stream,
   if (g.n_tri == 0) return 0;
   if (!g.dev_planned || g.n_ts != 0) return fail(-1, "tridiag part: a device-planned one-stage group only");
   if (int rc = tri_limits(g)) return rc;
-  if (part == 0)
-    hipLaunchKernelGGL((k_tridiag<512, 4, kTriSW, kTriOcc>), dim3(g.n_tri), dim3(512),
-                       (size_t)g.tri_max_n * 32, stream, gdesc(g, w), wseg(g, w, g.o_tri, SG_TRIBIG),
-                       1, kTriMaxN);
-  else
-    hipLaunchKernelGGL((k_tridiag<256, 2, kTriSW, kTriOcc>), dim3(g.n_tri), dim3(256),
-                       (size_t)std::min(g.tri_split, g.tri_max_n) * 32, stream, gdesc(g, w),
-                       wseg(g, w, g.o_tri, SG_TRISMALL), 1, g.tri_split);
+  if (part == 0) {
+    const int lds = tri_lds<512>(g.tri_max_n);
+    hipLaunchKernelGGL((k_tridiag<512, 4, kTriSW, kTriOcc>), dim3(g.n_tri), dim3(512), (size_t)lds,
+                       stream, gdesc(g, w), wseg(g, w, g.o_tri, SG_TRIBIG), 1, kTriMaxN, lds);
+  } else {
+    const int lds = tri_lds<256>(std::min(g.tri_split, g.tri_max_n));
+    hipLaunchKernelGGL((k_tridiag<256, 2, kTriSW, kTriOcc>), dim3(g.n_tri), dim3(256), (size_t)lds,
+                       stream, gdesc(g, w), wseg(g, w, g.o_tri, SG_TRISMALL), 1, g.tri_split, lds);
+  }
   BSMPO_HIP(hipGetLastError());
   return 0;
 }
@@ -1193,15 +1256,16 @@ int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
   const hipStream_t sst = both ? ss->stream : stream;
   if (int rc = launch_band(g, w, stream)) return rc;
   if (nbig > 0) {
-    hipLaunchKernelGGL((k_tridiag<512, 4, kTriSW, kTriOcc>), dim3(nbig), dim3(512),
-                       (size_t)g.tri_max_n * 32, big_on_side ? sst : stream, gdesc(g, w),
-                       wseg(g, w, g.o_tri, SG_TRIBIG), 1, kTriMaxN);
+    const int lds = tri_lds<512>(g.tri_max_n);
+    hipLaunchKernelGGL((k_tridiag<512, 4, kTriSW, kTriOcc>), dim3(nbig), dim3(512), (size_t)lds,
+                       big_on_side ? sst : stream, gdesc(g, w), wseg(g, w, g.o_tri, SG_TRIBIG), 1,
+                       kTriMaxN, lds);
     BSMPO_HIP(hipGetLastError());
   }
   if (nsmall > 0) {
-    hipLaunchKernelGGL((k_tridiag<256, 2, kTriSW, kTriOcc>), dim3(nsmall), dim3(256),
-                       (size_t)std::min(split, g.tri_max_n) * 32, sst, gdesc(g, w),
-                       wseg(g, w, g.o_tri, SG_TRISMALL), 1, split);
+    const int lds = tri_lds<256>(std::min(split, g.tri_max_n));
+    hipLaunchKernelGGL((k_tridiag<256, 2, kTriSW, kTriOcc>), dim3(nsmall), dim3(256), (size_t)lds,
+                       sst, gdesc(g, w), wseg(g, w, g.o_tri, SG_TRISMALL), 1, split, lds);
     BSMPO_HIP(hipGetLastError());
   }
   if (both) {
