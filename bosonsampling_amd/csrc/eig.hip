@@ -159,6 +159,30 @@ __device__ __forceinline__ double2 wave_colsum4(const double2 (&x)[4], int lane)
   }
   return y;
 }
+// x[q] (q = 0..31) summed over the 64 lanes of a wave by recursive halving:
+// at each exchange distance o = 32 .. 2 a lane keeps the half of its values
+// selected by bit o of its index and adds the partner's copy of that half;
+// lanes l and l ^ 1 end with the total of x[(l >> 1) & 31].
+template <int H>
+__device__ __forceinline__ void halve32(double (&x)[32], int lane) {
+  constexpr int o = 2 * H;
+  const bool up = (lane & o) != 0;
+#pragma unroll
+  for (int q = 0; q < H; ++q) {
+    const double keep = up ? x[q + H] : x[q];
+    const double send = up ? x[q] : x[q + H];
+    x[q] = keep + __shfl_xor(send, o);
+  }
+}
+__device__ __forceinline__ double wave_sum32(double (&x)[32], int lane) {
+  halve32<16>(x, lane);
+  halve32<8>(x, lane);
+  halve32<4>(x, lane);
+  halve32<2>(x, lane);
+  halve32<1>(x, lane);
+  return x[0] + __shfl_xor(x[0], 1);
+}
+
 template <int SW>
 __device__ __forceinline__ double2 wave_colsum(const double2 (&x)[SW], int lane) {
   if constexpr (SW == 8) return wave_colsum8(x, lane);
@@ -168,11 +192,12 @@ __device__ __forceinline__ double2 wave_colsum(const double2 (&x)[SW], int lane)
 // LDS of a sector's reduction: v and A22 v (2 n complex), and with VWL the
 // panel's V and W (2 n kTriNB complex) too.
 __host__ __device__ constexpr int64_t tri_lds_need(int n, bool vwl) {
-  return 16 * (3 * (int64_t)n + 2 + (vwl ? 2 * (int64_t)n * kTriNB : 0));
+  return 16 * (3 * (int64_t)n + 3 + (vwl ? 2 * (int64_t)n * kTriNB : 0));
 }
 
 // One sector's reduction by the whole block.  LDS (tsm, complex entries): v,
-// A22 v and the stashed next column (n each), the two pivot slots, then with
+// A22 v and the stashed next column (n each), the two pivot slots, the strip
+// counter of the product (one entry), then with
 // VWL the panel's V and W (n kTriNB each); without VWL, V and W live in the
 // sector's tridiagonal work area in global memory.
 template <int NT, int RPT, int SW, bool VWL>
@@ -185,10 +210,11 @@ __device__ __forceinline__ void tridiag_sector(const UpdDesc &u, const Sector &s
   double2 *ys = tsm + n;                                              // A22 v_j, n entries
   const lptr<double2> nx = (lptr<double2>)(tsm + 2 * n);  // column j + 1 as (c) of column j read it
   double2 *piv = tsm + 3 * n;  // column j's diagonal and subdiagonal entries
+  int *ctr = reinterpret_cast<int *>(tsm + 3 * n + 2);  // next strip of the product (c)
   using VWP = typename std::conditional<VWL, lptr<double2>, gptr<double2>>::type;
   VWP Vp;
   if constexpr (VWL)
-    Vp = (VWP)(tsm + 3 * n + 2);
+    Vp = (VWP)(tsm + 3 * n + 3);
   else
     Vp = gp(reinterpret_cast<double2 *>(u.triwork + s.tw_off));
   const VWP Wp = Vp + (int64_t)n * kTriNB;
@@ -269,6 +295,7 @@ __device__ __forceinline__ void tridiag_sector(const UpdDesc &u, const Sector &s
         d[j] = piv[0].x;
         e[j] = R.beta;
         tau[j] = R.tau;
+        *ctr = NW;  // strips 0 .. NW - 1 are the waves' first
       }
 #pragma unroll
       for (int k = 0; k < RPT; ++k) {
@@ -293,13 +320,18 @@ __device__ __forceinline__ void tridiag_sector(const UpdDesc &u, const Sector &s
       //     atomics (row sums per lane, column sums after a wave reduction).
       //     A wave's items (strip s, row block r0 = s, s + 64, ...; then the
       //     next strip) form one sequence whose loads run two items ahead,
-      //     across strip boundaries.
+      //     across strip boundaries.  Wave w starts on strip w; each further
+      //     strip is claimed from an LDS counter (strips longest first), so
+      //     that waves slowed by memory take fewer and the barrier after the
+      //     product waits less for the slowest.
       {
-        constexpr int stride = SW * NW;
         auto advance = [&](int &s, int &r0) {  // next item of this wave
           r0 += 64;
           if (r0 >= n) {
-            s += stride;
+            int k = 0;
+            if (lane == 0) k = atomicAdd(ctr, 1);
+            k = __shfl(k, 0);
+            s = j + 1 + SW * k;
             r0 = s;
           }
         };
@@ -358,33 +390,34 @@ __device__ __forceinline__ void tridiag_sector(const UpdDesc &u, const Sector &s
       }
       // (d) corrections for the panel's pending updates: the per-wave parts of
       //     W_l^H v and V_l^H v, before the barrier that completes A22 v
+      //     All 4 i sums of a wave are reduced together by recursive halving
+      //     (wave_sum32: 32 exchanges instead of 6 per sum, one chain of 6
+      //     steps instead of i).
       if (i > 0) {
-        for (int l = 0; l < i; ++l) {
-          double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+        double x[4 * kTriNB];
 #pragma unroll
-          for (int k = 0; k < RPT; ++k) {
-            const int r = j + 1 + t + k * NT;
-            if (r >= n) continue;
-            const double2 v = vs[r];
-            const double2 w_ = Wp[(int64_t)l * n + r], v_ = Vp[(int64_t)l * n + r];
-            p0 += w_.x * v.x + w_.y * v.y;  // conj(w_l) v
-            p1 += w_.x * v.y - w_.y * v.x;
-            p2 += v_.x * v.x + v_.y * v.y;  // conj(v_l) v
-            p3 += v_.x * v.y - v_.y * v.x;
+        for (int l = 0; l < kTriNB; ++l) {
+          double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+          if (l < i) {  // uniform
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+              const int r = j + 1 + t + k * NT;
+              if (r >= n) continue;
+              const double2 v = vs[r];
+              const double2 w_ = Wp[(int64_t)l * n + r], v_ = Vp[(int64_t)l * n + r];
+              p0 += w_.x * v.x + w_.y * v.y;  // conj(w_l) v
+              p1 += w_.x * v.y - w_.y * v.x;
+              p2 += v_.x * v.x + v_.y * v.y;  // conj(v_l) v
+              p3 += v_.x * v.y - v_.y * v.x;
+            }
           }
-          for (int o = 32; o >= 1; o >>= 1) {
-            p0 += __shfl_xor(p0, o);
-            p1 += __shfl_xor(p1, o);
-            p2 += __shfl_xor(p2, o);
-            p3 += __shfl_xor(p3, o);
-          }
-          if (lane == 0) {
-            red4[wave][4 * l + 0] = p0;
-            red4[wave][4 * l + 1] = p1;
-            red4[wave][4 * l + 2] = p2;
-            red4[wave][4 * l + 3] = p3;
-          }
+          x[4 * l + 0] = p0;
+          x[4 * l + 1] = p1;
+          x[4 * l + 2] = p2;
+          x[4 * l + 3] = p3;
         }
+        const double tot = wave_sum32(x, lane);  // total of x[(lane >> 1) & 31]
+        if ((lane & 1) == 0 && (lane >> 3) < i) red4[wave][lane >> 1] = tot;
       }
       __syncthreads();
       double2 acc[RPT];
@@ -1231,7 +1264,7 @@ int launch_tri_values(const GroupHeader &g, const char *w, hipStream_t stream) {
   // chasing) run on the launching stream.  One-stage sectors: with few of them
   // (a single update), the larger block shape for every sector, so that each
   // column chain gets more threads; with many (a group across instances), the
-  // sectors above the split (bsmpo_set_tri_split, default 384) take the
+  // sectors above the split (bsmpo_set_tri_split, default 256) take the
   // 512-thread shape and the rest 256-thread blocks, more per CU.  Work beside
   // the launching stream's goes to a side stream forked before the first
   // launch (fork / join by events), so that it fills the slots the long

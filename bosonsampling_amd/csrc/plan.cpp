@@ -21,7 +21,7 @@ namespace bsmpo {
 static thread_local std::string g_last_error;
 // process-wide knobs, set and read from any thread
 static std::atomic<int> g_eig_threshold{96};
-static std::atomic<int> g_tri_split{384};
+static std::atomic<int> g_tri_split{256};
 static std::atomic<double> g_tie_rel{1e-6};
 static std::atomic<double> g_drop_rel{1e-6};
 static std::atomic<int> g_two_stage{0};

@@ -50,7 +50,7 @@ int bsmpo_set_eig_threshold(int n);
  * Householder block-shape split (process-wide, affects groups built
  * afterwards): in a grouped launch with many tridiagonal-route sectors, those
  * with n > split run on 512-thread blocks, launched first, the rest on
- * 256-thread blocks on a side stream.  Default 384, allowed [1, 512]; every
+ * 256-thread blocks on a side stream.  Default 256, allowed [1, 512]; every
  * value gives the same results to rounding.  No counterpart in the reference
  * (a launch-geometry knob of this build's eigensolver, like
  * bsmpo_set_eig_threshold).

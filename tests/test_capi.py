@@ -119,7 +119,7 @@ def test_launch_knobs_validate_their_range():
     assert L.bsmpo_set_tri_split(0) < 0 and b"split" in L.bsmpo_last_error()
     assert L.bsmpo_set_tri_split(513) < 0
     assert L.bsmpo_set_eig_threshold(97) < 0 and b"threshold" in L.bsmpo_last_error()
-    assert L.bsmpo_set_tri_split(384) == 0
+    assert L.bsmpo_set_tri_split(256) == 0
     assert L.bsmpo_set_eig_threshold(96) == 0
 
 

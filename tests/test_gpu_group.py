@@ -104,7 +104,7 @@ def test_tridiagonal_block_shapes_agree(gpu):
             engine.simulate_batch(sts, meshes, inputs, runner=engine.GroupRunner(streams=1))
             runs[split] = sts
     finally:
-        check(lib().bsmpo_set_tri_split(384), "set_tri_split")
+        check(lib().bsmpo_set_tri_split(256), "set_tri_split")
         check(lib().bsmpo_set_eig_threshold(96), "set_eig_threshold")
     # the last layer's group has many sectors on both sides of the split
     layer = M - 1

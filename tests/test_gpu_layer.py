@@ -163,8 +163,8 @@ def test_layer_autotune_and_geometry_knobs(gpu):
     saved = {n: engine._knob(n) for n, _ in engine.LAYER_KNOBS}
     try:
         results = {}
-        for thr, split, order in ((96, 384, 8), (96, 384, 2), (96, 384, 16), (0, 384, 8),
-                                  (0, 256, 4), (0, 512, 16)):
+        for thr, split, order in ((96, 256, 8), (96, 256, 2), (96, 256, 16), (0, 256, 8),
+                                  (0, 384, 4), (0, 512, 16)):
             engine._knob("eig_threshold", thr)
             engine._knob("tri_split", split)
             engine._knob("tile_order", order)
@@ -174,7 +174,7 @@ def test_layer_autotune_and_geometry_knobs(gpu):
             results[(thr, split, order)] = sts
             print(f"knobs {thr} {split} {order}: fallbacks {runner.fallbacks}", flush=True)
         for key, sts in results.items():
-            ref_key = (96, 384, 8) if key[0] == 96 else (0, 384, 8)
+            ref_key = (96, 256, 8) if key[0] == 96 else (0, 256, 8)
             for a, b in zip(results[ref_key], sts):
                 dg, dl, same = _max_diff(a, b)
                 ta, ma = a.observables()
