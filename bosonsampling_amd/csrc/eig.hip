@@ -69,13 +69,16 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
 //   after the panel: one pass A22 -= V W^H + W V^H over the trailing matrix.
 // Only the lower triangle of the (Hermitian) trailing matrix is read or
 // written: per column it is read once (the product A22 v, each entry below the
-// diagonal serving A v and A^H v alike); it is written once per panel.  V and W of the panel live in the sector's
-// tridiagonal work area (n x kTriNB each, column-major).  Five barriers per
-// column: block reductions alternate between two LDS buffers, so each costs
-// one barrier.  Two block shapes: 256 threads x 2 rows (n <= 512) and
-// 512 threads x 4 rows (n <= 2048); each launch takes its own segment of the
-// sector list, which is sorted largest first (n outside [nmin, nmax] is
-// skipped, a guard of the LDS sizing).
+// diagonal serving A v and A^H v alike); it is written once per panel.  V and
+// W of the panel (n x kTriNB each, column-major) live in LDS when they fit the
+// launch's LDS (tri_lds), else in the sector's tridiagonal work area.  The
+// product is the only step of a column that streams memory: the steps around
+// it are a chain of short dependent ones (five barriers; block reductions
+// alternate between two LDS buffers, so each costs one barrier), which is
+// why the column's data stay on chip between them.  Two block shapes: 256
+// threads x 2 rows (n <= 512) and 512 threads x 4 rows (n <= 2048); each
+// launch takes its own segment of the sector list, which is sorted largest
+// first (n outside [nmin, nmax] is skipped, a guard of the LDS sizing).
 constexpr int kTriNB = kTriPanel;
 // Streaming shape of the reduction.  Measured on the M=64, chi=1024 grouped
 // workload (one launching stream): strips of 4 columns at 3 waves per SIMD
@@ -197,9 +200,9 @@ __host__ __device__ constexpr int64_t tri_lds_need(int n, bool vwl) {
 
 // One sector's reduction by the whole block.  LDS (tsm, complex entries): v,
 // A22 v and the stashed next column (n each), the two pivot slots, the strip
-// counter of the product (one entry), then with
-// VWL the panel's V and W (n kTriNB each); without VWL, V and W live in the
-// sector's tridiagonal work area in global memory.
+// counter of the product (one entry), then with VWL the panel's V and W
+// (n kTriNB each); without VWL, V and W live in the sector's tridiagonal work
+// area in global memory.
 template <int NT, int RPT, int SW, bool VWL>
 __device__ __forceinline__ void tridiag_sector(const UpdDesc &u, const Sector &s, double2 *tsm,
                                                double2 (*red)[16], double (*red4)[4 * kTriNB]) {
