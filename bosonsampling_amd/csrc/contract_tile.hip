@@ -326,18 +326,23 @@ __global__ __launch_bounds__(256) void k_contract_tile(const UpdDesc *__restrict
   __shared__ unsigned char pempty[256];  // union-box position without a centre block
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const double *w_u3 = nullptr;  // BS table the W table in LDS was built from
-  // XCD-aware order: work-list position p runs on XCD p % 8 (blocks are
-  // dispatched round robin over the 8 XCDs and the grid is a multiple of 8),
-  // so each XCD walks one contiguous eighth of the list and the tiles that
-  // share operand panels share its L2.
+  // XCD-aware order: block b runs on XCD b % 8 (blocks are dispatched round
+  // robin over the 8 XCDs and the grid is a multiple of 8), so each XCD walks
+  // one contiguous eighth of the list and the tiles that share operand panels
+  // share its L2; within the eighth each block takes one contiguous run of
+  // items, so that its consecutive items mostly belong to one update and
+  // reuse the W table already in LDS (measured: the stage 4% faster than
+  // blocks striding through the eighth).
   const int2 sg = *ws.seg;
   const int2 *__restrict__ list = ws.list + sg.x;
   const int nlist = sg.y;
   const int chunk = (nlist + 7) >> 3;
-  const int total = chunk << 3;
-  for (int p = blockIdx.x; p < total; p += gridDim.x) {
-    const int idx = (p & 7) * chunk + (p >> 3);
-    if (idx >= nlist) continue;  // uniform over the block
+  const int g8 = (int)gridDim.x >> 3, xcd = blockIdx.x & 7, bx = blockIdx.x >> 3;
+  const int sub = (chunk + g8 - 1) / g8;
+  for (int k = 0; k < sub; ++k) {
+    const int loc = bx * sub + k;
+    const int idx = xcd * chunk + loc;
+    if (loc >= chunk || idx >= nlist) break;  // uniform over the block
     const int2 ref = list[idx];
     const UpdDesc &u = Ud[ref.x];
     const TileItem ti = u.titems[ref.y];
