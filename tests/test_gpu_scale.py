@@ -134,6 +134,59 @@ def test_synthetic_large_sectors_match_fixture(gpu, chi, nmin):
 
 
 @pytest.mark.timeout(600)
+def test_householder_block_shapes_and_panel_storage(gpu):
+    """The synthetic d=13, chi=1024 update (sectors up to n > 512) as a group
+    of 8 copies under block-shape splits 512, 256 and 8: split 512 runs the
+    sectors with 264 < n <= 512 on 256-thread blocks with the panel's V and W
+    in global memory (they fit the LDS only up to n = 264 there), split 256
+    runs them on 512-thread blocks with V and W in LDS (up to n = 530), and
+    split 8 sends every sector but the smallest to 512-thread blocks.  Kept
+    sets as truncation.same_truncation holds them, X^H (Gamma0 Gamma1) Y to
+    1e-10 (the runs differ in their reductions' summation order only)."""
+    import torch
+    from bosonsampling_amd import engine
+    from bosonsampling_amd._capi import check, lib
+    from golden.make_golden import SYNTH_CASES, SYNTH_SEED, synth_bs, synth_projectors
+    from synth import synthetic_update
+    from truncation import same_truncation
+    d, chi = 13, 1024
+    CL, CC, CR, Glc, Gcr, LR = synthetic_update(d, chi, *SYNTH_CASES[chi], SYNTH_SEED)
+    G = np.zeros((3, chi, chi), np.complex128)
+    G[0], G[1] = Glc, Gcr
+    Lam = np.zeros((2, chi))
+    Lam[1] = LR
+    charge = np.full((4, chi, 2), d, np.int32)
+    charge[0], charge[1], charge[2] = CL, CC, CR
+    U3 = torch.from_numpy(synth_bs(d)).to(gpu)
+    X, Y = synth_projectors(chi)
+    sec = engine.DeviceMPO.from_host(G, Lam, charge, d, gpu).plan(0).sectors()[0]
+    sizes = np.minimum(sec[:, 2], sec[:, 3])
+    assert ((sizes > 264) & (sizes <= 512)).sum() > 0 and (sizes > 530).sum() > 0, sizes.max()
+    assert 8 * (sizes > 0).sum() >= 256  # many sectors: the group splits them between the shapes
+    out = {}
+    check(lib().bsmpo_set_eig_threshold(0), "set_eig_threshold")
+    try:
+        for split in (512, 256, 8):
+            check(lib().bsmpo_set_tri_split(split), "set_tri_split")
+            print(f"block-shape split {split}: a group of 8 copies", flush=True)
+            grp = [engine.DeviceMPO.from_host(G, Lam, charge, d, gpu) for _ in range(8)]
+            engine.GroupRunner(gpu, streams=1).run([(s_, 0, U3) for s_ in grp])
+            torch.cuda.synchronize()
+            out[split] = [s_.to_host() for s_ in grp]
+    finally:
+        check(lib().bsmpo_set_tri_split(256), "set_tri_split")
+        check(lib().bsmpo_set_eig_threshold(96), "set_eig_threshold")
+    for split in (512, 8):
+        for (G0, L0, c0), (G1, L1, c1) in zip(out[256], out[split]):
+            flips = same_truncation(L0, c0, L1, c1)
+            p0 = (X.conj().T @ G0[0]) @ (G0[1] @ Y)
+            p1 = (X.conj().T @ G1[0]) @ (G1[1] @ Y)
+            err = np.abs(p0 - p1).max() / np.abs(p0).max()
+            print(f"split {split} vs 256: flips {flips}, projection {err:.2e}", flush=True)
+            assert err <= TOL
+
+
+@pytest.mark.timeout(600)
 def test_grouped_equals_sequential_m64(gpu):
     """Two instances of the M=64, d=13, chi=1024 workload (seeds 100, 101)
     through layer 17: grouped complex128 (one launch per stage for every
