@@ -83,7 +83,7 @@ def parse():
     ap.add_argument("--graph", type=int, default=1,
                     help="1: also time the grouped path as device-planned layer programs (one "
                          "captured HIP graph per layer); `value` is then that run's rate")
-    ap.add_argument("--graph-groups", type=int, default=12, help="groups per layer of the graph run")
+    ap.add_argument("--graph-groups", type=int, default=8, help="groups per layer of the graph run")
     ap.add_argument("--graph-chains", type=int, default=3, choices=[1, 2, 3, 4],
                     help="concurrent chains of groups (scratch arenas) of the graph run")
     ap.add_argument("--graph-headroom", type=float, default=1.3,
